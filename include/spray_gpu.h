@@ -1,0 +1,176 @@
+/*
+ * spray_gpu.h - C ABI of the MI355X-native SpRay per-rank tracing path.
+ *
+ * One spray_ctx stands in for one SpRay rank's ooc tracer: it owns the
+ * scene (an HBM-resident domain pool with one LBVH per domain), the ray
+ * queues, the history buffer and the image.  Every entry point returns
+ * SPRAY_OK or an SPRAY_ERR_* code; the message of the last failure is read
+ * with spray_last_error().  Nothing aborts across this boundary (the
+ * reference's glog CHECK/LOG(FATAL) paths become status codes).
+ *
+ * Plain types only: no torch or HIP types cross the boundary.  Host buffers
+ * passed in are not retained after a call returns.  A context is used from
+ * one host thread (the reference runs MPI_THREAD_FUNNELED,
+ * apps/spray_ooc.cc:33-45).
+ */
+#ifndef SPRAY_GPU_H_
+#define SPRAY_GPU_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPRAY_OK 0
+#define SPRAY_ERR_INVALID 1  /* invalid argument or configuration */
+#define SPRAY_ERR_IO 2       /* file cannot be opened or written */
+#define SPRAY_ERR_FORMAT 3   /* malformed .spray or PLY content */
+#define SPRAY_ERR_OOM 4      /* device or host allocation failed */
+#define SPRAY_ERR_CAPACITY 5 /* a queue or pool bound was exceeded */
+#define SPRAY_ERR_DEVICE 6   /* HIP runtime error or no device */
+#define SPRAY_ERR_STATE 7    /* call order (e.g. trace before load) */
+
+#define SPRAY_SCHEDULE_PLAIN 0   /* one launch per scheduled domain (reference order) */
+#define SPRAY_SCHEDULE_BATCHED 1 /* persistent kernel over all queues, hipGraph passes */
+#define SPRAY_SCHEDULE_LEARNED 2 /* batched + MLP-ordered speculative visits */
+
+#define SPRAY_QUERY_CLOSEST 0 /* rtcIntersect: closest hit */
+#define SPRAY_QUERY_ANY 1     /* rtcOccluded: any hit */
+
+typedef struct spray_ctx spray_ctx;
+
+/* Mirrors the ooc-path fields of spray::Config (render/config.h:34-95,
+ * defaults render/config.cc:33-73) plus the new knobs marked "new". */
+typedef struct spray_config {
+  int image_w, image_h;
+  int has_camera; /* 0: default framing from the scene bound (spray_renderer.inl:86-105) */
+  float camera_pos[3], camera_lookat[3], camera_up[3];
+  float fov;            /* vertical, degrees */
+  int pixel_samples;    /* --pixel-samples */
+  int ao_samples;       /* --ao-samples (also area-light samples in PT) */
+  int ao_mode;          /* --ao-mode */
+  int bounces;          /* --bounces */
+  float ks[3];          /* --blinn ks */
+  float shininess;      /* --blinn shininess */
+  long long max_samples_per_rank; /* --max-samples-per-rank (blocking tile cap) */
+  int nthreads;         /* host threads used while loading PLY files */
+  float ao_max_dist;    /* new: AO ray length (reference: unbounded = INFINITY) */
+  int rr_start;         /* new: Russian roulette from this bounce on (0 = off) */
+  int schedule;         /* new: SPRAY_SCHEDULE_* */
+  int device;           /* HIP device ordinal */
+} spray_config;
+
+/* Fills cfg with the reference defaults (render/config.cc:33-73), except
+ * that unset ks/shininess become 0/1 (the reference leaves them undefined). */
+void spray_config_defaults(spray_config* cfg);
+
+typedef struct spray_stats {
+  unsigned long long useful_rays;     /* eye + non-stale bounce + non-stale shadow rays */
+  unsigned long long eye_rays;
+  unsigned long long radiance_visits; /* (ray, domain) closest-hit traversals */
+  unsigned long long shadow_visits;   /* (ray, domain) any-hit traversals */
+  unsigned long long shades;          /* accepted history updates (shader calls) */
+  unsigned long long final_hits;      /* hits on the final history chains */
+  unsigned long long passes;
+  unsigned long long windows;
+  unsigned long long tiles;
+  unsigned long long queue_entries;   /* (ray, domain) pairs enqueued */
+  unsigned long long deferred_entries;/* learned: pairs sent to the fallback queue */
+  unsigned long long pruned_entries;  /* pairs dropped by the tdom / history filter */
+  unsigned long long node_visits;     /* BVH nodes fetched by all traversals */
+  unsigned long long tri_tests;       /* triangles tested by all traversals */
+  double trace_ms;                    /* device time of the frame (hipEvents) */
+  double build_ms;                    /* domain-pool build time of the last load */
+} spray_stats;
+
+typedef struct spray_rays_soa { /* n entries each; host memory */
+  const float *ox, *oy, *oz;
+  const float *dx, *dy, *dz;
+  const float *tnear, *tfar;
+} spray_rays_soa;
+
+typedef struct spray_hits_soa { /* n entries each; host memory */
+  float *t, *u, *v; /* closest: hit t (tfar on a miss) and barycentrics */
+  int *prim;        /* closest: primitive id, -1 on a miss; any: 1 if occluded else 0 */
+} spray_hits_soa;
+
+/* Scene generator parameters (synthetic marching-cubes domains). */
+typedef struct spray_scenegen_params {
+  int nx, ny, nz;        /* domain grid; domain (i,j,k) spans [i,i+1]x[j,j+1]x[k,k+1] */
+  int res;               /* cells per domain axis */
+  double a, b, phase;    /* f = sin(a x + phase) sin(a y) sin(a z) + 0.3 sin(b (x+y+z) + phase) */
+  int per_domain_random; /* 1: per-domain a, b and log-uniform triangle budget from seed */
+  unsigned long long seed;
+  double tri_total;      /* per_domain_random: rescale budgets to this total */
+  unsigned int rgb;      /* packed vertex colour r<<16|g<<8|b */
+} spray_scenegen_params;
+
+/* ---- context lifecycle (SprayRenderer/Tracer init, spray_renderer.inl:28-84) ---- */
+int spray_create(const spray_config* cfg, spray_ctx** out);
+void spray_destroy(spray_ctx* ctx);
+const char* spray_last_error(const spray_ctx* ctx); /* ctx may be NULL (create errors) */
+int spray_set_config(spray_ctx* ctx, const spray_config* cfg);
+
+/* Scene::init (render/scene.inl:30-100): parses the .spray file
+ * (io/scene_loader.cc:315-358), loads every PLY (io/ply_loader.cc:179-204)
+ * into the HBM domain pool (the infinite-cache warm-up, scene.inl:85-93),
+ * computes vertex normals (trimesh_buffer.cc:267-326) and builds one LBVH
+ * per domain (replacing rtcCommit, trimesh_buffer.cc:213-224). */
+int spray_load_scene(spray_ctx* ctx, const char* spray_file, const char* ply_path);
+int spray_num_domains(const spray_ctx* ctx);
+/* Domain record: counts and world AABB (render/domain.h:32-44). */
+int spray_domain_info(const spray_ctx* ctx, int id, long long* nverts, long long* nfaces,
+                      float lo[3], float hi[3]);
+/* Copies a domain's vertex normals (3 floats per vertex) computed on the GPU. */
+int spray_domain_normals(spray_ctx* ctx, int id, float* out);
+
+/* Camera::init (render/camera.h:128-166). */
+int spray_set_camera(spray_ctx* ctx, const float pos[3], const float lookat[3],
+                     const float up[3], float vfov_deg);
+
+/* Tracer::trace (ooc/ooc_tracer.inl:185-234) + HdrImage accumulation: renders
+ * one frame and copies W*H*4 floats (rgba, row-major, y up) into rgba_out
+ * (may be NULL). */
+int spray_trace_frame(spray_ctx* ctx, float* rgba_out, spray_stats* out);
+
+/* Scene::intersect / Scene::occluded for a batch of rays against one domain
+ * (rtcIntersect / rtcOccluded, render/scene.inl:189-209).  The parity entry
+ * point: mode is SPRAY_QUERY_CLOSEST or SPRAY_QUERY_ANY. */
+int spray_trace_rays(spray_ctx* ctx, const spray_rays_soa* rays, size_t n,
+                     spray_hits_soa* hits, int mode, int domain);
+
+/* Isector::isectDomains (ooc/ooc_isector.h:116-124 -> WbvhEmbree,
+ * render/wbvh_embree.cc:126-148): per-ray domain list sorted by (t, id).
+ * counts[n]; ids/ts[n*maxk] (entries beyond maxk are dropped). */
+int spray_domain_list(spray_ctx* ctx, const float* org, const float* dir, size_t n, int maxk,
+                      int* counts, int* ids, float* ts);
+
+/* Selects the speculative schedule (new). */
+int spray_set_schedule(spray_ctx* ctx, int schedule);
+
+/* Batched form: autotunes queue-chunk size and waves per work item on the
+ * loaded scene and current camera; the choice is kept in the context. */
+int spray_autotune(spray_ctx* ctx, int* chunk_out, int* tile_out);
+int spray_set_batch_params(spray_ctx* ctx, int chunk, int tile);
+
+/* Learned form (new): domain-hit predictor.  Logging collects
+ * (origin, direction, domain, hit) samples from plain-schedule frames;
+ * training runs the MLP's own forward/backward/Adam kernels on the GPU. */
+int spray_predictor_log(spray_ctx* ctx, int enable, long long max_samples);
+long long spray_predictor_num_samples(const spray_ctx* ctx);
+int spray_predictor_train(spray_ctx* ctx, int epochs, int batch, float lr, float* loss_out);
+int spray_predictor_eval(spray_ctx* ctx, float* loss_out, float* accuracy_out);
+int spray_predictor_save(spray_ctx* ctx, const char* path);
+int spray_predictor_load(spray_ctx* ctx, const char* path);
+
+/* Synthetic scene writer: domains as PLY files plus a .spray file
+ * (dir/name.spray); extra_lines (e.g. light lines) go to the header. */
+int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const char* name,
+                         const char* extra_lines, long long* triangles_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPRAY_GPU_H_ */

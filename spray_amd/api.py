@@ -1,0 +1,185 @@
+"""Python view of the C ABI: configuration, scene generation and a tracer
+context.  Mirrors the reference's driver flow (render/spray_renderer.inl:
+Config -> Scene::init -> Camera::init -> Tracer::trace) one rank at a time.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+
+class SprayError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"spray error {code}: {msg}")
+        self.code = code
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def default_config(**kw):
+    """spray_config with the reference defaults (render/config.cc:33-73),
+    overridden by keyword arguments (lists for the 3-vectors)."""
+    lib = L.load()
+    cfg = L.spray_config()
+    lib.spray_config_defaults(C.byref(cfg))
+    for k, v in kw.items():
+        if k in ("camera_pos", "camera_lookat", "camera_up", "ks"):
+            for i in range(3):
+                getattr(cfg, k)[i] = float(v[i])
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+def scenegen_write(directory, name, nx, ny, nz, res, a=6.0, b=17.0, phase=0.0, rgb=0xFFFFFF,
+                   per_domain_random=False, seed=0, tri_total=0.0, extra_lines=""):
+    """Writes a marching-cubes scene (name.spray + PLYs) and returns the triangle count."""
+    lib = L.load()
+    p = L.spray_scenegen_params(nx=nx, ny=ny, nz=nz, res=res, a=a, b=b, phase=phase,
+                                per_domain_random=int(per_domain_random), seed=seed,
+                                tri_total=tri_total, rgb=rgb)
+    tris = C.c_longlong(0)
+    rc = lib.spray_scenegen_write(C.byref(p), str(directory).encode(), name.encode(),
+                                  extra_lines.encode(), C.byref(tris))
+    if rc != L.SPRAY_OK:
+        raise SprayError(rc, lib.spray_last_error(None).decode())
+    return tris.value
+
+
+class Context:
+    """One rank's tracer (spray_ctx)."""
+
+    def __init__(self, cfg):
+        self.lib = L.load()
+        self.cfg = cfg
+        h = C.c_void_p()
+        rc = self.lib.spray_create(C.byref(cfg), C.byref(h))
+        if rc != L.SPRAY_OK:
+            raise SprayError(rc, self.lib.spray_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != L.SPRAY_OK:
+            raise SprayError(rc, self.lib.spray_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.spray_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_scene(self, spray_file, ply_path=""):
+        self._check(self.lib.spray_load_scene(self.h, str(spray_file).encode(), str(ply_path).encode()))
+
+    @property
+    def num_domains(self):
+        return self.lib.spray_num_domains(self.h)
+
+    def domain_info(self, i):
+        nv, nf = C.c_longlong(), C.c_longlong()
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        self._check(self.lib.spray_domain_info(self.h, i, C.byref(nv), C.byref(nf), _fp(lo), _fp(hi)))
+        return nv.value, nf.value, lo, hi
+
+    def normals(self, i):
+        nv = self.domain_info(i)[0]
+        out = np.zeros((nv, 3), np.float32)
+        self._check(self.lib.spray_domain_normals(self.h, i, _fp(out)))
+        return out
+
+    def set_camera(self, pos, lookat, up, fov):
+        a = [np.asarray(x, np.float32) for x in (pos, lookat, up)]
+        self._check(self.lib.spray_set_camera(self.h, _fp(a[0]), _fp(a[1]), _fp(a[2]), float(fov)))
+
+    def set_config(self, cfg):
+        self._check(self.lib.spray_set_config(self.h, C.byref(cfg)))
+        self.cfg = cfg
+
+    def set_schedule(self, s):
+        self._check(self.lib.spray_set_schedule(self.h, s))
+
+    def set_batch_params(self, chunk, tile):
+        self._check(self.lib.spray_set_batch_params(self.h, chunk, tile))
+
+    def autotune(self):
+        ch, tl = C.c_int(), C.c_int()
+        self._check(self.lib.spray_autotune(self.h, C.byref(ch), C.byref(tl)))
+        return ch.value, tl.value
+
+    def trace_frame(self, want_image=True):
+        st = L.spray_stats()
+        img = np.zeros((self.cfg.image_h, self.cfg.image_w, 4), np.float32) if want_image else None
+        self._check(self.lib.spray_trace_frame(self.h, _fp(img) if want_image else None, C.byref(st)))
+        return img, {f: getattr(st, f) for f, _ in L.spray_stats._fields_}
+
+    def trace_rays(self, org, dirs, domain, mode=L.QUERY_CLOSEST, tnear=0.001, tfar=math.inf):
+        org = np.ascontiguousarray(org, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = org.shape[0]
+        cols = [np.ascontiguousarray(org[:, k]) for k in range(3)] + \
+               [np.ascontiguousarray(dirs[:, k]) for k in range(3)]
+        tn = np.full(n, tnear, np.float32) if np.isscalar(tnear) else np.ascontiguousarray(tnear, np.float32)
+        tf = np.full(n, tfar, np.float32) if np.isscalar(tfar) else np.ascontiguousarray(tfar, np.float32)
+        rays = L.spray_rays_soa(*[_fp(c) for c in cols], _fp(tn), _fp(tf))
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        p = np.zeros(n, np.int32)
+        hits = L.spray_hits_soa(_fp(t), _fp(u), _fp(v), _ip(p))
+        self._check(self.lib.spray_trace_rays(self.h, C.byref(rays), n, C.byref(hits), mode, domain))
+        if mode == L.QUERY_ANY:
+            return p.astype(bool)
+        return t, u, v, p
+
+    def domain_list(self, org, dirs, maxk=32):
+        org = np.ascontiguousarray(org, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = org.shape[0]
+        cnt = np.zeros(n, np.int32)
+        ids = np.zeros((n, maxk), np.int32)
+        ts = np.zeros((n, maxk), np.float32)
+        self._check(self.lib.spray_domain_list(self.h, _fp(org), _fp(dirs), n, maxk, _ip(cnt), _ip(ids), _fp(ts)))
+        return cnt, ids, ts
+
+    # learned schedule
+    def predictor_log(self, enable, max_samples=0):
+        self._check(self.lib.spray_predictor_log(self.h, int(enable), int(max_samples)))
+
+    def predictor_num_samples(self):
+        return self.lib.spray_predictor_num_samples(self.h)
+
+    def predictor_train(self, epochs, batch, lr):
+        loss = C.c_float()
+        self._check(self.lib.spray_predictor_train(self.h, epochs, batch, lr, C.byref(loss)))
+        return loss.value
+
+    def predictor_eval(self):
+        loss, acc = C.c_float(), C.c_float()
+        self._check(self.lib.spray_predictor_eval(self.h, C.byref(loss), C.byref(acc)))
+        return loss.value, acc.value
+
+    def predictor_save(self, path):
+        self._check(self.lib.spray_predictor_save(self.h, str(path).encode()))
+
+    def predictor_load(self, path):
+        self._check(self.lib.spray_predictor_load(self.h, str(path).encode()))

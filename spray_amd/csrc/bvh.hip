@@ -1,0 +1,707 @@
+// HBM domain pool: vertex normals, LBVH build (Morton codes, LDS radix
+// sort, Karras hierarchy, atomic bottom-up refit, collapse to a 4-wide BVH)
+// and the batch ray queries behind spray_trace_rays / spray_domain_list.
+//
+// Replaces the reference's per-domain Embree scene (rtcCommit in
+// render/trimesh_buffer.cc:213-224) and TriMeshBuffer::computeNormals
+// (render/trimesh_buffer.cc:267-326).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "ctx.h"
+#include "kernels_common.h"
+#include "traverse.h"
+
+namespace spray {
+
+// ---------------------------------------------------------------------------
+// Vertex normals: the reference adds each face's unnormalised normal to its
+// three vertices in face order.  A vertex-parallel gather over the faces of
+// each vertex, visited in ascending face order, reproduces that float
+// summation order exactly.
+__global__ void kVertCount(const uint32_t* __restrict__ face, long long nf, uint32_t* cnt) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nf * 3;
+       i += (long long)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[face[i]], 1u);
+}
+__global__ void kVertFill(const uint32_t* __restrict__ face, long long nf,
+                          const uint32_t* __restrict__ off, uint32_t* cursor, uint32_t* adj) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nf * 3;
+       i += (long long)gridDim.x * blockDim.x) {
+    uint32_t v = face[i];
+    uint32_t slot = atomicAdd(&cursor[v], 1u);
+    adj[off[v] + slot] = (uint32_t)(i / 3);
+  }
+}
+__global__ void kVertNormal(const float* __restrict__ vtx, const uint32_t* __restrict__ face,
+                            long long nv, const uint32_t* __restrict__ off,
+                            const uint32_t* __restrict__ cnt, uint32_t* adj, float* nrm) {
+  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
+       v += (long long)gridDim.x * blockDim.x) {
+    uint32_t b = off[v], n = cnt[v];
+    uint32_t* L = adj + b;
+    for (uint32_t i = 1; i < n; ++i) {  // insertion sort: lists are short
+      uint32_t x = L[i];
+      int j = (int)i - 1;
+      while (j >= 0 && L[j] > x) {
+        L[j + 1] = L[j];
+        --j;
+      }
+      L[j + 1] = x;
+    }
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t* fc = face + (size_t)L[i] * 3;
+      const float* p0 = vtx + (size_t)fc[0] * 3;
+      const float* p1 = vtx + (size_t)fc[1] * 3;
+      const float* p2 = vtx + (size_t)fc[2] * 3;
+      float ux = p1[0] - p0[0], uy = p1[1] - p0[1], uz = p1[2] - p0[2];
+      float wx = p2[0] - p0[0], wy = p2[1] - p0[1], wz = p2[2] - p0[2];
+      ax += uy * wz - uz * wy;
+      ay += uz * wx - ux * wz;
+      az += ux * wy - uy * wx;
+    }
+    nrm[v * 3] = ax;
+    nrm[v * 3 + 1] = ay;
+    nrm[v * 3 + 2] = az;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Centroid bounds and Morton codes
+__device__ __forceinline__ uint32_t orderedBits(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fromOrdered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void kCentroidBounds(const float* __restrict__ vtx, const uint32_t* __restrict__ face,
+                                int nf, uint32_t* bmin, uint32_t* bmax) {
+  float lo[3] = {kInfF, kInfF, kInfF}, hi[3] = {-kInfF, -kInfF, -kInfF};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+    for (int k = 0; k < 3; ++k) {
+      float a = vtx[(size_t)face[i * 3] * 3 + k], b = vtx[(size_t)face[i * 3 + 1] * 3 + k],
+            c = vtx[(size_t)face[i * 3 + 2] * 3 + k];
+      float cen = 0.5f * (fminf(a, fminf(b, c)) + fmaxf(a, fmaxf(b, c)));
+      lo[k] = fminf(lo[k], cen);
+      hi[k] = fmaxf(hi[k], cen);
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    float l = waveMin(lo[k]), h = waveMax(hi[k]);
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&bmin[k], orderedBits(l));
+      atomicMax(&bmax[k], orderedBits(h));
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t expandBits10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void kMorton(const float* __restrict__ vtx, const uint32_t* __restrict__ face, int nf,
+                        const uint32_t* bmin, const uint32_t* bmax, uint32_t* keys,
+                        uint32_t* vals) {
+  float lo[3], ext[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = fromOrdered(bmin[k]);
+    float e = fromOrdered(bmax[k]) - lo[k];
+    ext[k] = e > 0.f ? e : 1.f;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+      float a = vtx[(size_t)face[i * 3] * 3 + k], b = vtx[(size_t)face[i * 3 + 1] * 3 + k],
+            c = vtx[(size_t)face[i * 3 + 2] * 3 + k];
+      float cen = 0.5f * (fminf(a, fminf(b, c)) + fmaxf(a, fmaxf(b, c)));
+      float x = (cen - lo[k]) / ext[k] * 1024.0f;
+      x = fminf(fmaxf(x, 0.0f), 1023.0f);
+      q[k] = (uint32_t)x;
+    }
+    keys[i] = (expandBits10(q[0]) << 2) | (expandBits10(q[1]) << 1) | expandBits10(q[2]);
+    vals[i] = (uint32_t)i;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of (key, value) pairs, 8-bit digits, 2048-key tiles.
+// Ranking inside a tile is stable: each 256-key round is split per wave with
+// a ballot-based digit match, waves are ordered through LDS counters and a
+// running per-digit count carries across rounds.
+constexpr int kRsBlock = 256;
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kRsBlock * kRsItems;
+
+__global__ void kRsHist(const uint32_t* __restrict__ keys, int n, int shift, uint32_t* hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int base = blockIdx.x * kRsTile;
+  for (int j = 0; j < kRsItems; ++j) {
+    int idx = base + j * kRsBlock + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void kRsScatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
+                           int shift, const uint32_t* __restrict__ hist_scanned) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[4][256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  base[tid] = hist_scanned[tid * gridDim.x + blockIdx.x];
+  run[tid] = 0;
+  const int tile = blockIdx.x * kRsTile;
+  for (int j = 0; j < kRsItems; ++j) {
+    wc[0][tid] = 0;
+    wc[1][tid] = 0;
+    wc[2][tid] = 0;
+    wc[3][tid] = 0;
+    __syncthreads();
+    int idx = tile + j * kRsBlock + tid;
+    bool valid = idx < n;
+    uint32_t key = valid ? kin[idx] : 0u;
+    uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    uint32_t rank = __popcll(peers & ((1ull << lane) - 1ull));
+    if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      uint32_t c0 = wc[0][tid], c1 = wc[1][tid], c2 = wc[2][tid], c3 = wc[3][tid];
+      uint32_t r = run[tid];
+      wc[0][tid] = r;
+      wc[1][tid] = r + c0;
+      wc[2][tid] = r + c0 + c1;
+      wc[3][tid] = r + c0 + c1 + c2;
+      run[tid] = r + c0 + c1 + c2 + c3;
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t dst = base[d] + wc[w][d] + rank;
+      kout[dst] = key;
+      vout[dst] = vin[idx];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Karras 2012 hierarchy over sorted keys; duplicate keys are ordered by index.
+__device__ __forceinline__ int deltaK(const uint32_t* keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = keys[i], b = keys[j];
+  if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+  return __clz(a ^ b);
+}
+
+// inner node i: children encoded c >= 0 inner, c < 0 leaf ~c (sorted position)
+__global__ void kKarras(const uint32_t* __restrict__ keys, int n, int* left, int* right,
+                        int* first, int* last, int* parent_inner, int* parent_leaf) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += gridDim.x * blockDim.x) {
+    int d = (deltaK(keys, n, i, i + 1) - deltaK(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = deltaK(keys, n, i, i - d);
+    int lmax = 2;
+    while (deltaK(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+      if (deltaK(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = deltaK(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+      t = (t + 1) >> 1;
+      if (deltaK(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? d : 0);
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    int lc = (lo == gamma) ? ~gamma : gamma;
+    int rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    left[i] = lc;
+    right[i] = rc;
+    first[i] = lo;
+    last[i] = hi;
+    if (lc < 0)
+      parent_leaf[~lc] = i;
+    else
+      parent_inner[lc] = i;
+    if (rc < 0)
+      parent_leaf[~rc] = i;
+    else
+      parent_inner[rc] = i;
+  }
+}
+
+// Leaf boxes in sorted order + the leaf-ordered triangle array of the pool.
+__global__ void kLeaves(const float* __restrict__ vtx, const uint32_t* __restrict__ face, int nf,
+                        const uint32_t* __restrict__ order, float* leafbox, LTri* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+    uint32_t p = order[i];
+    const float* a = vtx + (size_t)face[p * 3] * 3;
+    const float* b = vtx + (size_t)face[p * 3 + 1] * 3;
+    const float* c = vtx + (size_t)face[p * 3 + 2] * 3;
+    LTri t;
+    for (int k = 0; k < 3; ++k) {
+      t.v0[k] = a[k];
+      t.v1[k] = b[k];
+      t.v2[k] = c[k];
+      leafbox[(size_t)i * 6 + k] = fminf(a[k], fminf(b[k], c[k]));
+      leafbox[(size_t)i * 6 + 3 + k] = fmaxf(a[k], fmaxf(b[k], c[k]));
+    }
+    t.v0[3] = __int_as_float((int)p);
+    t.v1[3] = 0.f;
+    t.v2[3] = 0.f;
+    out[i] = t;
+  }
+}
+
+__device__ __forceinline__ float ldCoherent(const float* p) {
+  return __uint_as_float(
+      __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void stCoherent(float* p, float v) {
+  __hip_atomic_store((uint32_t*)p, __float_as_uint(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bottom-up refit: the second child to arrive at a node unions both boxes.
+__global__ void kRefit(int n, const int* __restrict__ left, const int* __restrict__ right,
+                       const int* __restrict__ parent_inner, const int* __restrict__ parent_leaf,
+                       const float* __restrict__ leafbox, float* innerbox, uint32_t* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int p = parent_leaf[i];
+    while (p >= 0) {
+      __threadfence();
+      uint32_t old = atomicAdd(&flags[p], 1u);
+      if (old == 0) break;
+      __threadfence();
+      float box[6];
+      for (int k = 0; k < 6; ++k) box[k] = (k < 3) ? kInfF : -kInfF;
+      int cs[2] = {left[p], right[p]};
+      for (int c = 0; c < 2; ++c) {
+        const float* src = cs[c] < 0 ? leafbox + (size_t)(~cs[c]) * 6 : innerbox + (size_t)cs[c] * 6;
+        for (int k = 0; k < 3; ++k) {
+          box[k] = fminf(box[k], ldCoherent(src + k));
+          box[3 + k] = fmaxf(box[3 + k], ldCoherent(src + 3 + k));
+        }
+      }
+      for (int k = 0; k < 6; ++k) stCoherent(innerbox + (size_t)p * 6 + k, box[k]);
+      p = parent_inner[p];
+    }
+  }
+}
+
+// Collapse the binary tree into 4-wide nodes, one frontier level per launch.
+struct FrontierItem {
+  int bnode;  // binary inner node
+  int wnode;  // 4-wide node slot
+};
+
+__device__ __forceinline__ float boxArea(const float* b) {
+  float dx = fmaxf(b[3] - b[0], 0.f), dy = fmaxf(b[4] - b[1], 0.f), dz = fmaxf(b[5] - b[2], 0.f);
+  return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void kCollapse(const FrontierItem* __restrict__ fin, int nin, FrontierItem* fout,
+                          int* nout, int* node_counter, int node_cap, const int* __restrict__ left,
+                          const int* __restrict__ right, const int* __restrict__ first,
+                          const int* __restrict__ last, const float* __restrict__ leafbox,
+                          const float* __restrict__ innerbox, BNode4* nodes, int* overflow) {
+  for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < nin; it += gridDim.x * blockDim.x) {
+    FrontierItem f = fin[it];
+    int ch[4];
+    int nc = 2;
+    ch[0] = left[f.bnode];
+    ch[1] = right[f.bnode];
+    auto count = [&](int c) { return c < 0 ? 1 : last[c] - first[c] + 1; };
+    while (nc < 4) {
+      int best = -1;
+      float barea = -1.f;
+      for (int i = 0; i < nc; ++i) {
+        if (ch[i] >= 0 && count(ch[i]) > kLeafMax) {
+          float a = boxArea(innerbox + (size_t)ch[i] * 6);
+          if (a > barea) {
+            barea = a;
+            best = i;
+          }
+        }
+      }
+      if (best < 0) break;
+      int c = ch[best];
+      ch[best] = left[c];
+      ch[nc++] = right[c];
+    }
+    BNode4 nd;
+    for (int i = 0; i < 4; ++i) {
+      float b[6];
+      int word;
+      if (i >= nc) {
+        for (int k = 0; k < 3; ++k) {
+          b[k] = kInfF;
+          b[3 + k] = -kInfF;
+        }
+        word = (int)0x80000000u;
+      } else {
+        int c = ch[i];
+        const float* src = c < 0 ? leafbox + (size_t)(~c) * 6 : innerbox + (size_t)c * 6;
+        for (int k = 0; k < 6; ++k) b[k] = src[k];
+        if (c < 0) {
+          word = (int)(0x80000000u | (1u << 24) | (uint32_t)(~c));
+        } else if (count(c) <= kLeafMax) {
+          word = (int)(0x80000000u | ((uint32_t)count(c) << 24) | (uint32_t)first[c]);
+        } else {
+          int slot = atomicAdd(node_counter, 1);
+          if (slot >= node_cap) {
+            atomicExch(overflow, 1);
+            word = (int)0x80000000u;
+          } else {
+            int q = atomicAdd(nout, 1);
+            fout[q] = FrontierItem{c, slot};
+            word = slot;
+          }
+        }
+      }
+      nd.lox[i] = b[0];
+      nd.loy[i] = b[1];
+      nd.loz[i] = b[2];
+      nd.hix[i] = b[3];
+      nd.hiy[i] = b[4];
+      nd.hiz[i] = b[5];
+      nd.child[i] = word;
+    }
+    nd.pad[0] = nd.pad[1] = nd.pad[2] = nd.pad[3] = 0;
+    nodes[f.wnode] = nd;
+  }
+}
+
+// Root for tiny domains (<= kLeafMax triangles): one leaf child.
+__global__ void kTinyRoot(int nf, const float* __restrict__ leafbox, BNode4* nodes) {
+  BNode4 nd;
+  float b[6] = {kInfF, kInfF, kInfF, -kInfF, -kInfF, -kInfF};
+  for (int i = 0; i < nf; ++i)
+    for (int k = 0; k < 3; ++k) {
+      b[k] = fminf(b[k], leafbox[i * 6 + k]);
+      b[3 + k] = fmaxf(b[3 + k], leafbox[i * 6 + 3 + k]);
+    }
+  for (int i = 0; i < 4; ++i) {
+    bool used = (i == 0);
+    nd.lox[i] = used ? b[0] : kInfF;
+    nd.loy[i] = used ? b[1] : kInfF;
+    nd.loz[i] = used ? b[2] : kInfF;
+    nd.hix[i] = used ? b[3] : -kInfF;
+    nd.hiy[i] = used ? b[4] : -kInfF;
+    nd.hiz[i] = used ? b[5] : -kInfF;
+    nd.child[i] = used ? (int)(0x80000000u | ((uint32_t)nf << 24)) : (int)0x80000000u;
+    nd.pad[i] = 0;
+  }
+  nodes[0] = nd;
+}
+
+// ---------------------------------------------------------------------------
+static int gridFor(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65535) g = 65535;
+  return (int)g;
+}
+
+void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
+  auto t_start = std::chrono::steady_clock::now();
+  DomainPool& P = c->pool;
+  P.release();
+  hipStream_t s = c->stream;
+  int nd = (int)meshes.size();
+  P.ndomains = nd;
+  P.desc.resize(nd);
+  long long tv = 0, tf = 0, tn = 0;
+  int maxf = 0;
+  long long maxv = 0;
+  for (int d = 0; d < nd; ++d) {
+    const HostDomain& hd = c->scene.domains[d];
+    DomainDesc& D = P.desc[d];
+    long long nv = (long long)meshes[d].v.size() / 3, nf = (long long)meshes[d].f.size() / 3;
+    if (nf >= (1 << 24)) throw StatusError(SPRAY_ERR_CAPACITY, "domain has more than 2^24 faces");
+    D.vtx_off = tv;
+    D.face_off = tf;
+    D.tri_off = tf;
+    D.node_off = tn;
+    D.num_tris = (int)nf;
+    D.num_nodes = 0;
+    for (int k = 0; k < 3; ++k) {
+      D.lo[k] = hd.lo[k];
+      D.hi[k] = hd.hi[k];
+    }
+    D.mtl = hd.mtl;
+    D.eta_a = hd.eta_a;
+    D.eta_b = hd.eta_b;
+    tv += nv;
+    tf += nf;
+    tn += nf / 2 + 16;  // 4-wide node bound for leaves of <= 4
+    maxf = std::max(maxf, (int)nf);
+    maxv = std::max(maxv, nv);
+  }
+  P.total_verts = tv;
+  P.total_tris = tf;
+  P.vtx.reserve(tv * 12);
+  P.face.reserve(tf * 12);
+  P.rgb.reserve(tv * 4);
+  P.nrm.reserve(tv * 12);
+  P.tris.reserve(tf * sizeof(LTri));
+  P.nodes.reserve(tn * sizeof(BNode4));
+  for (int d = 0; d < nd; ++d) {
+    const DomainDesc& D = P.desc[d];
+    SPRAY_HIP(hipMemcpyAsync(P.vtx.as<float>() + D.vtx_off * 3, meshes[d].v.data(),
+                             meshes[d].v.size() * 4, hipMemcpyHostToDevice, s));
+    SPRAY_HIP(hipMemcpyAsync(P.face.as<uint32_t>() + D.face_off * 3, meshes[d].f.data(),
+                             meshes[d].f.size() * 4, hipMemcpyHostToDevice, s));
+    SPRAY_HIP(hipMemcpyAsync(P.rgb.as<uint32_t>() + D.vtx_off, meshes[d].rgb.data(),
+                             meshes[d].rgb.size() * 4, hipMemcpyHostToDevice, s));
+  }
+  // build scratch sized for the largest domain
+  int nblk = (maxf + kRsTile - 1) / kRsTile;
+  DevBuf keys, vals, keys2, vals2, hist, lb, ib, lft, rgt, fst, lst, pin, plf, flags, fa, fb, cnts,
+      scal;
+  keys.reserve((size_t)maxf * 4 + 64);
+  vals.reserve((size_t)maxf * 4 + 64);
+  keys2.reserve((size_t)maxf * 4 + 64);
+  vals2.reserve((size_t)maxf * 4 + 64);
+  hist.reserve((size_t)256 * nblk * 4 + 64);
+  lb.reserve((size_t)maxf * 24 + 64);
+  ib.reserve((size_t)maxf * 24 + 64);
+  lft.reserve((size_t)maxf * 4 + 64);
+  rgt.reserve((size_t)maxf * 4 + 64);
+  fst.reserve((size_t)maxf * 4 + 64);
+  lst.reserve((size_t)maxf * 4 + 64);
+  pin.reserve((size_t)maxf * 4 + 64);
+  plf.reserve((size_t)maxf * 4 + 64);
+  flags.reserve((size_t)maxf * 4 + 64);
+  fa.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
+  fb.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
+  cnts.reserve((size_t)(maxv + 1) * 12 + 64);
+  scal.reserve(64);
+  DevBuf adj;
+  adj.reserve((size_t)maxf * 12 + 64);
+  P.bvh_depth.assign(nd, 0);
+  for (int d = 0; d < nd; ++d) {
+    DomainDesc& D = P.desc[d];
+    const int nf = D.num_tris;
+    const long long nv = (long long)meshes[d].v.size() / 3;
+    const float* vtx = P.vtx.as<float>() + D.vtx_off * 3;
+    const uint32_t* face = P.face.as<uint32_t>() + D.face_off * 3;
+    // normals
+    uint32_t* cnt = cnts.as<uint32_t>();
+    uint32_t* off = cnt + (nv + 1);
+    uint32_t* cur = off + (nv + 1);
+    SPRAY_HIP(hipMemsetAsync(cnt, 0, (size_t)(nv + 1) * 4, s));
+    SPRAY_HIP(hipMemsetAsync(cur, 0, (size_t)(nv + 1) * 4, s));
+    kVertCount<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, cnt);
+    exclusiveScan(cnt, off, nv + 1, s);
+    kVertFill<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, off, cur,
+                                                              adj.as<uint32_t>());
+    kVertNormal<<<gridFor(nv, 256), 256, 0, s>>>(vtx, face, nv, off, cnt, adj.as<uint32_t>(),
+                                                  P.nrm.as<float>() + D.vtx_off * 3);
+    // Morton codes
+    uint32_t* bmm = scal.as<uint32_t>();
+    uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    SPRAY_HIP(hipMemcpyAsync(bmm, init, 24, hipMemcpyHostToDevice, s));
+    kCentroidBounds<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, bmm, bmm + 3);
+    kMorton<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, bmm, bmm + 3, keys.as<uint32_t>(),
+                                             vals.as<uint32_t>());
+    // radix sort, 4 passes of 8 bits
+    int nb = (nf + kRsTile - 1) / kRsTile;
+    uint32_t *ka = keys.as<uint32_t>(), *va = vals.as<uint32_t>(), *kb = keys2.as<uint32_t>(),
+             *vb = vals2.as<uint32_t>();
+    for (int pass = 0; pass < 4; ++pass) {
+      kRsHist<<<nb, kRsBlock, 0, s>>>(ka, nf, pass * 8, hist.as<uint32_t>());
+      exclusiveScan(hist.as<uint32_t>(), hist.as<uint32_t>(), (long long)256 * nb, s);
+      kRsScatter<<<nb, kRsBlock, 0, s>>>(ka, va, kb, vb, nf, pass * 8, hist.as<uint32_t>());
+      std::swap(ka, kb);
+      std::swap(va, vb);
+    }
+    LTri* tris = P.tris.as<LTri>() + D.tri_off;
+    BNode4* nodes = P.nodes.as<BNode4>() + D.node_off;
+    float* leafbox = lb.as<float>();
+    kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris);
+    int node_cap = nf / 2 + 16;
+    if (nf <= kLeafMax) {
+      kTinyRoot<<<1, 1, 0, s>>>(nf, leafbox, nodes);
+      D.num_nodes = 1;
+      P.bvh_depth[d] = 1;
+      continue;
+    }
+    int* L = lft.as<int>();
+    int* R = rgt.as<int>();
+    int* F = fst.as<int>();
+    int* Lst = lst.as<int>();
+    int* PI = pin.as<int>();
+    int* PL = plf.as<int>();
+    SPRAY_HIP(hipMemsetAsync(PI, 0xff, (size_t)nf * 4, s));  // root parent = -1
+    kKarras<<<gridFor(nf, 256), 256, 0, s>>>(ka, nf, L, R, F, Lst, PI, PL);
+    SPRAY_HIP(hipMemsetAsync(flags.p, 0, (size_t)nf * 4, s));
+    kRefit<<<gridFor(nf, 256), 256, 0, s>>>(nf, L, R, PI, PL, leafbox, ib.as<float>(),
+                                            flags.as<uint32_t>());
+    // collapse
+    int* ctr = (int*)(scal.as<uint32_t>() + 8);  // [node_counter, nout, overflow]
+    int hinit[3] = {1, 0, 0};
+    SPRAY_HIP(hipMemcpyAsync(ctr, hinit, 12, hipMemcpyHostToDevice, s));
+    FrontierItem root{0, 0};
+    SPRAY_HIP(hipMemcpyAsync(fa.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
+    FrontierItem *fin = fa.as<FrontierItem>(), *fout = fb.as<FrontierItem>();
+    int nin = 1, depth = 0;
+    while (nin > 0) {
+      SPRAY_HIP(hipMemsetAsync(ctr + 1, 0, 4, s));
+      kCollapse<<<gridFor(nin, 256), 256, 0, s>>>(fin, nin, fout, ctr + 1, ctr, node_cap, L, R, F,
+                                                   Lst, leafbox, ib.as<float>(), nodes, ctr + 2);
+      int h[3];
+      SPRAY_HIP(hipMemcpyAsync(h, ctr, 12, hipMemcpyDeviceToHost, s));
+      SPRAY_HIP(hipStreamSynchronize(s));
+      if (h[2]) throw StatusError(SPRAY_ERR_CAPACITY, "BVH node capacity exceeded");
+      nin = h[1];
+      D.num_nodes = h[0];
+      std::swap(fin, fout);
+      ++depth;
+    }
+    P.bvh_depth[d] = depth;
+    // worst-case stack need: three pushes per level
+    if (3 * depth + 1 > kRing + kOverflow)
+      throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
+  }
+  SPRAY_HIP(hipStreamSynchronize(s));
+  long long used = 0;
+  for (auto& D : P.desc) used += D.num_nodes;
+  P.total_nodes = used;
+  P.d_desc.reserve(sizeof(DomainDesc) * nd);
+  SPRAY_HIP(hipMemcpy(P.d_desc.p, P.desc.data(), sizeof(DomainDesc) * nd, hipMemcpyHostToDevice));
+  for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
+                    &plf, &flags, &fa, &fb, &cnts, &scal, &adj})
+    b->release();
+  P.build_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+// ---------------------------------------------------------------------------
+// Batch queries (parity entry points)
+constexpr int kQBlock = 256;
+
+__global__ void __launch_bounds__(kQBlock) kTraceRays(
+    const BNode4* __restrict__ nodes, const LTri* __restrict__ tris, const float* __restrict__ org,
+    const float* __restrict__ dir, const float* __restrict__ tnear, const float* __restrict__ tfar,
+    long long n, int mode, uint32_t max_nodes, float* t_o, float* u_o, float* v_o, int* prim_o,
+    uint32_t* ovf, int* err) {
+  __shared__ uint32_t ring[kRing * kQBlock];
+  LaneStack st;
+  long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  st.init(ring, ovf + gtid * kOverflow);
+  for (long long i = gtid; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float o[3] = {org[i * 3], org[i * 3 + 1], org[i * 3 + 2]};
+    float d[3] = {dir[i * 3], dir[i * 3 + 1], dir[i * 3 + 2]};
+    if (mode == 0) {
+      float t, u, v;
+      int p;
+      closestHit<false>(nodes, tris, o, d, tnear[i], tfar[i], max_nodes, st, &t, &u, &v, &p, nullptr);
+      t_o[i] = t;
+      u_o[i] = u;
+      v_o[i] = v;
+      prim_o[i] = p;
+    } else {
+      prim_o[i] = anyHit<false>(nodes, tris, o, d, tnear[i], tfar[i], max_nodes, st, nullptr) ? 1 : 0;
+    }
+  }
+  if (st.overflowed) atomicExch(err, 1);
+}
+
+void traceRaysDevice(Ctx* c, const float* d_org, const float* d_dir, const float* d_tnear,
+                     const float* d_tfar, size_t n, int mode, int domain, float* d_t, float* d_u,
+                     float* d_v, int* d_prim) {
+  const DomainDesc& D = c->pool.desc[domain];
+  int grid = std::min<long long>(2048, std::max<long long>(1, ((long long)n + kQBlock - 1) / kQBlock));
+  c->q_out.reserve((size_t)grid * kQBlock * kOverflow * 4 + 64);
+  int* err = (int*)((char*)c->q_out.p + (size_t)grid * kQBlock * kOverflow * 4);
+  SPRAY_HIP(hipMemsetAsync(err, 0, 4, c->stream));
+  kTraceRays<<<grid, kQBlock, 0, c->stream>>>(
+      c->pool.nodes.as<BNode4>() + D.node_off, c->pool.tris.as<LTri>() + D.tri_off, d_org, d_dir,
+      d_tnear, d_tfar, (long long)n, mode, (uint32_t)D.num_nodes, d_t, d_u, d_v, d_prim,
+      c->q_out.as<uint32_t>(), err);
+  int h = 0;
+  SPRAY_HIP(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, c->stream));
+  SPRAY_HIP(hipStreamSynchronize(c->stream));
+  if (h) throw StatusError(SPRAY_ERR_CAPACITY, "traversal stack overflow or corrupt BVH");
+}
+
+// Domain list per ray: brute force over the domain AABBs staged in LDS,
+// insertion into a (t, id)-sorted list of at most maxk entries.
+constexpr int kMaxList = 64;
+__global__ void __launch_bounds__(256) kDomainList(const DomainDesc* __restrict__ desc, int nd,
+                                                   const float* __restrict__ org,
+                                                   const float* __restrict__ dir, long long n,
+                                                   int maxk, int* counts, int* ids, float* ts) {
+  extern __shared__ float boxes[];  // 6 floats per domain
+  for (int i = threadIdx.x; i < nd; i += blockDim.x)
+    for (int k = 0; k < 3; ++k) {
+      boxes[i * 6 + k] = desc[i].lo[k];
+      boxes[i * 6 + 3 + k] = desc[i].hi[k];
+    }
+  __syncthreads();
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n;
+       r += (long long)gridDim.x * blockDim.x) {
+    float o[3] = {org[r * 3], org[r * 3 + 1], org[r * 3 + 2]};
+    float d[3] = {dir[r * 3], dir[r * 3 + 1], dir[r * 3 + 2]};
+    int cnt = 0, kept = 0;
+    int* my_ids = ids + r * maxk;
+    float* my_ts = ts + r * maxk;
+    for (int i = 0; i < nd; ++i) {
+      float tmin;
+      if (!slabDomain(&boxes[i * 6], &boxes[i * 6 + 3], o, d, &tmin)) continue;
+      ++cnt;
+      // insert (tmin, i) keeping the maxk smallest, ordered by (t, id)
+      int j = kept;
+      if (kept == maxk) {
+        float lt = my_ts[maxk - 1];
+        if (!(tmin < lt)) continue;  // ids ascend, so equal t keeps the earlier id
+        j = maxk - 1;
+      } else {
+        ++kept;
+      }
+      while (j > 0 && my_ts[j - 1] > tmin) {
+        my_ts[j] = my_ts[j - 1];
+        my_ids[j] = my_ids[j - 1];
+        --j;
+      }
+      my_ts[j] = tmin;
+      my_ids[j] = i;
+    }
+    counts[r] = cnt;
+  }
+}
+
+void domainListDevice(Ctx* c, const float* d_org, const float* d_dir, size_t n, int maxk,
+                      int* d_counts, int* d_ids, float* d_ts) {
+  int nd = c->pool.ndomains;
+  int grid = std::min<long long>(4096, std::max<long long>(1, ((long long)n + 255) / 256));
+  kDomainList<<<grid, 256, nd * 6 * sizeof(float), c->stream>>>(
+      c->pool.d_desc.as<DomainDesc>(), nd, d_org, d_dir, (long long)n, maxk, d_counts, d_ids, d_ts);
+  SPRAY_HIP(hipGetLastError());
+  SPRAY_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace spray

@@ -1,0 +1,125 @@
+// spray_ctx: everything one rank's tracer owns on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "dmath.h"
+#include "spray_internal.h"
+
+struct spray_ctx;
+
+namespace spray {
+
+#define SPRAY_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      throw DeviceError(std::string(#call) + ": " + hipGetErrorString(e_));          \
+    }                                                                                \
+  } while (0)
+
+struct DeviceError {
+  std::string msg;
+  explicit DeviceError(std::string m) : msg(std::move(m)) {}
+};
+struct StatusError {
+  int code;
+  std::string msg;
+  StatusError(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// A growable device buffer (grows only; contents not preserved on growth).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t b) {
+    if (b <= bytes) return;
+    if (p) SPRAY_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, b) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw StatusError(SPRAY_ERR_OOM, "device allocation of " + std::to_string(b) + " bytes failed");
+    }
+    bytes = b;
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// HBM-resident domain pool (replaces the domain cache + Embree scenes)
+struct DomainPool {
+  int ndomains = 0;
+  long long total_tris = 0, total_verts = 0, total_nodes = 0;
+  std::vector<DomainDesc> desc;  // host copy
+  DevBuf d_desc;                 // DomainDesc[ndomains]
+  DevBuf vtx, face, rgb, nrm;    // float3 / uint3 / uint32 / float3
+  DevBuf nodes;                  // BNode4
+  DevBuf tris;                   // LTri (leaf order)
+  std::vector<int> bvh_depth;
+  double build_ms = 0;
+  void release() {
+    d_desc.release();
+    vtx.release();
+    face.release();
+    rgb.release();
+    nrm.release();
+    nodes.release();
+    tris.release();
+    desc.clear();
+    ndomains = 0;
+  }
+};
+
+struct TracerState;  // tracer.hip
+struct Predictor;    // mlp.hip
+
+struct Ctx {
+  spray_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  HostScene scene;
+  bool loaded = false;
+  DomainPool pool;
+  CameraDev cam;
+  bool camera_set = false;
+  TracerState* tracer = nullptr;
+  Predictor* pred = nullptr;
+  int batch_chunk = 1024;
+  int batch_tile = 4;      // waves per block of the persistent kernel (block = 64 * tile)
+  int param_version = 0;   // bumped when camera / config change (graph re-capture)
+  std::string err;
+  // scratch for spray_trace_rays / spray_domain_list
+  DevBuf q_in, q_out;
+};
+
+// bvh.hip
+void buildPool(Ctx* c, std::vector<HostMesh>& meshes);
+void traceRaysDevice(Ctx* c, const float* d_org, const float* d_dir, const float* d_tnear,
+                     const float* d_tfar, size_t n, int mode, int domain, float* d_t, float* d_u,
+                     float* d_v, int* d_prim);
+void domainListDevice(Ctx* c, const float* d_org, const float* d_dir, size_t n, int maxk,
+                      int* d_counts, int* d_ids, float* d_ts);
+// tracer.hip
+void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st);
+void tracerRelease(Ctx* c);
+void tracerAutotune(Ctx* c, int* chunk, int* tile);
+// mlp.hip
+void predictorRelease(Ctx* c);
+
+}  // namespace spray
+
+struct spray_ctx {
+  spray::Ctx c;
+};

@@ -1,0 +1,104 @@
+// Internal types of the spray_amd library (host side and device layouts).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/spray_gpu.h"
+
+namespace spray {
+
+constexpr int kHistory = 4;         // SPRAY_SPECU_HISTORY_SIZE (src/CMakeLists.txt:33)
+constexpr int kListSize = 16;       // SPRAY_RAY_DOMAIN_LIST_SIZE (src/CMakeLists.txt:32)
+constexpr int kStatSlots = kListSize + 1;  // depths >= 16 share one zero-weight slot
+constexpr int kMaxDomains = 4096;
+constexpr int kLeafMax = 4;         // triangles per BVH leaf
+
+enum MaterialType { kDiffuse = 0, kMirror = 1, kGlass = 2, kTransmission = 3 };
+enum LightType { kPointLight = 0, kHemisphereLight = 1, kRectLight = 2 };
+
+struct HostMesh {
+  std::vector<float> v;       // xyz per vertex
+  std::vector<uint32_t> f;    // 3 per face
+  std::vector<uint32_t> rgb;  // packed per vertex
+};
+
+struct HostDomain {
+  int id = 0;
+  long long num_vertices = 0, num_faces = 0;
+  std::string filename;
+  float obj_lo[3] = {0, 0, 0}, obj_hi[3] = {0, 0, 0};
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  float xf[16];
+  int mtl = kDiffuse;
+  float mtl_rgb[3] = {1, 1, 1};
+  float eta_a = 1, eta_b = 1;
+};
+
+struct HostLight {
+  int type = kPointLight;
+  float pos[3] = {0, 0, 0};
+  float e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
+  float radiance[3] = {0, 0, 0};
+  float normal[3] = {0, 0, 0};
+  float area = 0;
+};
+
+struct HostScene {
+  std::vector<HostDomain> domains;
+  std::vector<HostLight> lights;
+  float lo[3], hi[3];
+};
+
+// scene_io.cc
+int parseSceneFile(const std::string& spray_file, const std::string& ply_path, HostScene* s,
+                   std::string* err);
+int readPly(const std::string& filename, HostMesh* m, std::string* err);
+bool transformIsIdentity(const float xf[16]);
+void transformPoint(const float xf[16], const float p[3], float out[3]);
+
+// scenegen.cc
+void mcDomainParams(const spray_scenegen_params& p, int d, double* a, double* b, int* res);
+void mcGenerate(const spray_scenegen_params& p, int d, HostMesh* m);
+int scenegenWrite(const spray_scenegen_params& p, const char* dir, const char* name,
+                  const char* extra_lines, int64_t* tris_out, std::string* err);
+
+// ---------------------------------------------------------------------------
+// Device layouts (shared by host and kernels)
+
+// One 4-wide BVH node: 128 bytes = one HBM/L2 line.  child[i] >= 0 is an
+// inner node index (relative to the domain's node base); a negative value
+// is a leaf: bit 31 set, bits 24..30 the triangle count, bits 0..23 the
+// first triangle in leaf order.  Unused slots have an empty box.
+struct alignas(128) BNode4 {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t child[4];
+  int32_t pad[4];
+};
+static_assert(sizeof(BNode4) == 128, "BNode4 must be 128 bytes");
+
+// Triangle in leaf order: 48 bytes; v0.w holds the primitive id.
+struct alignas(16) LTri {
+  float v0[4], v1[4], v2[4];
+};
+
+struct DomainDesc {
+  long long node_off;  // into the pool's node array
+  long long tri_off;   // into the pool's leaf-ordered triangle array
+  long long vtx_off;   // into vertex / normal / colour arrays
+  long long face_off;  // into the face array (3 uint32 per face)
+  int num_tris;
+  int num_nodes;
+  float lo[3], hi[3];
+  int mtl;
+  float eta_a, eta_b;
+};
+
+struct DevLight {
+  int type;
+  float pos[3], e1[3], e2[3], radiance[3], normal[3];
+  float area;
+};
+
+}  // namespace spray

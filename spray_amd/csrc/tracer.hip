@@ -1,0 +1,1315 @@
+// The per-rank speculative tracer on the GPU (ooc/ooc_tracer.inl,
+// ooc/ooc_pcontext.h, ooc/ooc_tcontext.{h,inl}, ooc/ooc_shader_{ao,pt}.h).
+//
+// One blocking tile at a time (render/tile.cc:317-391).  A window covers
+// SPRAY_HISTORY_SIZE = 4 virtual depths; pass k of a window traces the rays
+// of virtual depth k plus the shadow rays spawned in pass k-1, so a window
+// has at most 5 passes.  Each pass:
+//   count    - domain list of every new ray (slab test against the domain
+//              AABBs staged in LDS); per-domain entry counts and the
+//              DomainStats score sum_d count(d) * max(0, 16 - d)
+//   schedule - exclusive offsets, domains ordered by score (ties by id),
+//              work items for the batched form
+//   scatter  - per-domain queues of (ray, tdom)
+//   trace    - per domain: filter (tdom <= history[depth] and VBuf chain
+//              correct), closest hit, VBuf update, shade and spawn; shadow
+//              entries: filter, any hit, occlusion flag
+//   retire   - shadow rays tested in this pass add their weight when
+//              unoccluded and still on the final history chain
+// The plain form launches one trace kernel per scheduled domain in score
+// order; the batched form traces every queue of a pass in one persistent
+// launch and captures a whole tile in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "ctx.h"
+#include "tracer_dev.h"
+
+namespace spray {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void flushCount(unsigned long long* dst, unsigned long long v) {
+  v = waveSumULL(v);
+  if (__lane_id() == 0 && v) atomicAdd(dst, v);
+}
+
+// ---------------------------------------------------------------------------
+// Tile / window bookkeeping (single-thread kernels keep the whole loop on
+// the device so that it can be captured in a graph).
+__global__ void kTileStart(TraceParams P) {
+  DevCounters* c = P.ctr;
+  uint32_t S = (uint32_t)(P.tile_rec[1] * P.image_w * P.spp);
+  c->r_count = S;
+  c->r_init = S;
+  c->r_mark = 0;
+  c->s_count = 0;
+  c->s_mark = 0;
+  c->p_count = 0;
+  c->useful += S;  // eye rays
+}
+
+__global__ void kGenEyes(TraceParams P) {
+  const int tile_y = P.tile_rec[0], tile_h = P.tile_rec[1];
+  const long long S = (long long)tile_h * P.image_w * P.spp;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < S;
+       i += (long long)gridDim.x * blockDim.x) {
+    int bufid = (int)i;
+    int pix_local = bufid / P.spp;
+    int x = pix_local % P.image_w;
+    int y = tile_y + pix_local / P.image_w;
+    f3 d;
+    if (P.spp > 1) {
+      uint32_t rng = rngInit(bufid);
+      float fx = (float)x + rng1D(rng);
+      float fy = (float)y + rng1D(rng);
+      d = cameraDir(P.cam, fx, fy);
+    } else {
+      d = cameraDir(P.cam, (float)x, (float)y);
+    }
+    P.R.op[i] = make_float4(P.cam.pos.x, P.cam.pos.y, P.cam.pos.z, __int_as_float(y * P.image_w + x));
+    P.R.ds[i] = make_float4(d.x, d.y, d.z, __int_as_float(bufid));
+    P.R.wd[i] = make_float4(1.f, 1.f, 1.f, __int_as_float(0));
+    P.R.h[i] = make_float4(kInfF, kInfF, kInfF, kInfF);
+  }
+}
+
+// Window restart (ooc_tcontext.h:206-216 procPendingQ, with the pending
+// ray's full 4-entry parent history validated: deviation Q1)
+__global__ void kWindowStart(TraceParams P) {
+  DevCounters* c = P.ctr;
+  c->r_count = 0;
+  c->r_mark = 0;
+  c->s_count = 0;
+  c->s_mark = 0;
+}
+__global__ void kRestartPending(TraceParams P) {
+  DevCounters* c = P.ctr;
+  const uint32_t n = min(c->p_count, P.p_cap);
+  unsigned long long useful = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool valid = i < n;
+    float4 hh = valid ? P.Pn.h[i] : make_float4(0, 0, 0, 0);
+    float h[4] = {hh.x, hh.y, hh.z, hh.w};
+    int samid = valid ? __float_as_int(P.Pn.ds[i].w) : 0;
+    bool ok = valid && vbufCorrect(P.vbuf, samid, h, kHistory);
+    uint32_t idx = waveAppend(ok, &c->r_count, P.r_cap, &c->err);
+    if (ok && idx != 0xffffffffu) {
+      P.R.op[idx] = P.Pn.op[i];
+      P.R.ds[idx] = P.Pn.ds[i];
+      float4 wd = P.Pn.wd[i];
+      P.R.wd[idx] = make_float4(wd.x, wd.y, wd.z, __int_as_float(0));
+      P.R.h[idx] = make_float4(kInfF, kInfF, kInfF, kInfF);
+      ++useful;
+    }
+  }
+  flushCount(&c->useful, useful);
+}
+__global__ void kRestartDone(TraceParams P) {
+  DevCounters* c = P.ctr;
+  c->r_init = min(c->r_count, P.r_cap);
+  c->p_count = 0;
+}
+
+__global__ void kVbufReset(TraceParams P) {
+  const long long n = (long long)P.tile_rec[1] * P.image_w * P.spp * kHistory;
+  unsigned long long inf = mkW(kInfF, 0u);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    P.vbuf[i] = inf;
+}
+
+__global__ void kPassBegin(TraceParams P) {
+  DevCounters* c = P.ctr;
+  // pool levels past capacity (an overflow, reported to the host) are clamped
+  // so that no kernel reads beyond a pool
+  c->r_beg = c->r_mark;
+  c->r_end = min(c->r_count, P.r_cap);
+  c->r_mark = c->r_end;
+  c->s_beg = c->s_mark;
+  c->s_end = min(c->s_count, P.s_cap);
+  c->s_mark = c->s_end;
+  c->active = (c->r_end > c->r_beg || c->s_end > c->s_beg) ? 1u : 0u;
+  if (c->active) c->passes++;
+  c->item_ctr[0] = 0;
+  c->item_ctr[1] = 0;
+  c->n_items[0] = 0;
+  c->n_items[1] = 0;
+  c->vlog_count = 0;
+}
+
+__global__ void kClearQ(TraceParams P) {
+  if (!P.ctr->active) return;
+  int nd = P.ndomains;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += gridDim.x * blockDim.x) {
+    P.cnt[i] = 0;
+    P.cnt[nd + i] = 0;
+    P.cursor[i] = 0;
+    P.cursor[nd + i] = 0;
+    P.score[i] = 0;
+  }
+}
+
+// Ray of the current pass: index into the radiance range, then the shadow range.
+__device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* kind,
+                                        uint32_t* ray) {
+  const DevCounters* c = P.ctr;
+  long long nr = (long long)c->r_end - c->r_beg;
+  long long ns = (long long)c->s_end - c->s_beg;
+  if (i < nr) {
+    *kind = 0;
+    *ray = c->r_beg + (uint32_t)i;
+    return true;
+  }
+  if (i < nr + ns) {
+    *kind = 1;
+    *ray = c->s_beg + (uint32_t)(i - nr);
+    return true;
+  }
+  return false;
+}
+
+// Isector::isectAll / isectWithoutCurrentDomain (ooc_isector.h:126-174) as
+// counts: entries per domain and the DomainStats score
+// (ooc_domain_stats.cc:60-88) weight max(0, 16 - position in the ray's list).
+__global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
+  extern __shared__ uint32_t sm[];
+  if (!P.ctr->active) return;
+  const int nd = P.ndomains;
+  float* boxes = (float*)sm;                    // 6 * nd
+  uint32_t* lc = (uint32_t*)(boxes + 6 * nd);   // 2 * nd counts
+  uint32_t* ls = lc + 2 * nd;                   // nd score (32-bit per block)
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+    for (int k = 0; k < 3; ++k) {
+      boxes[i * 6 + k] = P.desc[i].lo[k];
+      boxes[i * 6 + 3 + k] = P.desc[i].hi[k];
+    }
+    lc[i] = 0;
+    lc[nd + i] = 0;
+    ls[i] = 0;
+  }
+  __syncthreads();
+  const DevCounters* c = P.ctr;
+  const long long total = (long long)(c->r_end - c->r_beg) + (c->s_end - c->s_beg);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    int kind;
+    uint32_t r;
+    passRay(P, i, &kind, &r);
+    const RayPool& pool = kind ? P.S : P.R;
+    float4 op = pool.op[r], ds = pool.ds[r];
+    float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
+    int spawn = kind ? (__float_as_int(pool.wd[r].w) >> 8) : -1;
+    // the 16 nearest (t, id) entries carry score weight
+    float tt[kTopDepths];
+    int ti[kTopDepths];
+    int kept = 0;
+    bool spawn_listed = false;
+    for (int dd = 0; dd < nd; ++dd) {
+      float tmin;
+      if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
+      atomicAdd(&lc[kind * nd + dd], 1u);
+      if (dd == spawn) {
+        spawn_listed = true;
+        continue;  // the spawning domain's test is not a queued entry in the reference
+      }
+      // insertion into the (t, id)-sorted top list (unrolled: stays in registers)
+      if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
+        float ct = tmin;
+        int ci = dd;
+#pragma unroll
+        for (int k = 0; k < kTopDepths; ++k) {
+          if (k < kept) {
+            if (ct < tt[k] || (ct == tt[k] && ci < ti[k])) {
+              float xt = tt[k];
+              int xi = ti[k];
+              tt[k] = ct;
+              ti[k] = ci;
+              ct = xt;
+              ci = xi;
+            }
+          } else if (k == kept) {
+            tt[k] = ct;
+            ti[k] = ci;
+          }
+        }
+        if (kept < kTopDepths) ++kept;
+      }
+    }
+    if (kind == 1 && spawn >= 0 && !spawn_listed) atomicAdd(&lc[nd + spawn], 1u);
+    for (int k = 0; k < kTopDepths; ++k)
+      if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+    if (lc[i]) atomicAdd(&P.cnt[i], lc[i]);
+    if (lc[nd + i]) atomicAdd(&P.cnt[nd + i], lc[nd + i]);
+    if (ls[i]) atomicAdd(&P.score[i], (unsigned long long)ls[i]);
+  }
+}
+
+// Offsets, schedule (DomainStats::schedule: descending score, ties by id)
+// and the batched work list, in one block.
+__global__ void __launch_bounds__(1024) kSchedule(TraceParams P) {
+  DevCounters* c = P.ctr;
+  if (!c->active) return;
+  const int nd = P.ndomains;
+  __shared__ uint32_t carry[3];
+  __shared__ uint32_t wsum[16 + 1];
+  if (threadIdx.x == 0) carry[0] = carry[1] = carry[2] = 0;
+  __syncthreads();
+  // rank by (score desc, id asc)
+  for (int d = threadIdx.x; d < nd; d += blockDim.x) {
+    unsigned long long sd = P.score[d];
+    int rank = 0;
+    for (int e = 0; e < nd; ++e) {
+      unsigned long long se = P.score[e];
+      rank += (se > sd) || (se == sd && e < d);
+    }
+    P.order[rank] = d;
+  }
+  __syncthreads();
+  // exclusive offsets per kind in domain-id order
+  for (int kind = 0; kind < 2; ++kind) {
+    for (int base = 0; base < nd; base += blockDim.x) {
+      int d = base + threadIdx.x;
+      uint32_t v = d < nd ? P.cnt[kind * nd + d] : 0u;
+      // block exclusive scan (wave shuffles + LDS)
+      uint32_t x = v;
+      int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane == 63) wsum[w] = x;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+          uint32_t t = wsum[i];
+          wsum[i] = acc;
+          acc += t;
+        }
+        wsum[16] = acc;
+      }
+      __syncthreads();
+      if (d < nd) P.off[kind * (nd + 1) + d] = carry[kind] + wsum[w] + x - v;
+      __syncthreads();
+      if (threadIdx.x == 0) carry[kind] += wsum[16];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      P.off[kind * (nd + 1) + nd] = carry[kind];
+      c->q_total[kind] = carry[kind];
+      c->queue_entries += carry[kind];
+      if (carry[kind] > P.q_cap[kind]) atomicOr(&c->err, 1u);
+    }
+    __syncthreads();
+  }
+  // work items in schedule order: per domain, radiance chunks then shadow chunks
+  for (int base = 0; base < nd; base += blockDim.x) {
+    int k = base + threadIdx.x;
+    int d = k < nd ? P.order[k] : 0;
+    uint32_t nr = k < nd ? (P.cnt[d] + P.chunk - 1) / P.chunk : 0u;
+    uint32_t ns = k < nd ? (P.cnt[nd + d] + P.chunk - 1) / P.chunk : 0u;
+    uint32_t v = nr + ns;
+    uint32_t x = v;
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t acc = 0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        uint32_t t = wsum[i];
+        wsum[i] = acc;
+        acc += t;
+      }
+      wsum[16] = acc;
+    }
+    __syncthreads();
+    uint32_t pos = carry[2] + wsum[w] + x - v;
+    if (k < nd && pos + v <= P.items_cap) {
+      uint32_t ob = P.off[d];
+      for (uint32_t j = 0; j < nr; ++j) {
+        uint32_t b = j * P.chunk;
+        uint32_t e = min(b + (uint32_t)P.chunk, P.cnt[d]);
+        P.items[pos++] = WorkItem{d, 0, ob + b, ob + e};
+      }
+      uint32_t os = P.off[(nd + 1) + d];
+      for (uint32_t j = 0; j < ns; ++j) {
+        uint32_t b = j * P.chunk;
+        uint32_t e = min(b + (uint32_t)P.chunk, P.cnt[nd + d]);
+        P.items[pos++] = WorkItem{d, 1, os + b, os + e};
+      }
+    } else if (k < nd && v) {
+      atomicOr(&c->err, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry[2] += wsum[16];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c->n_items[0] = min(carry[2], P.items_cap);
+}
+
+// Scatter the queue entries (block-aggregated slot reservation).
+__global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
+  extern __shared__ uint32_t sm[];
+  if (!P.ctr->active) return;
+  const int nd = P.ndomains;
+  float* boxes = (float*)sm;
+  uint32_t* lc = (uint32_t*)(boxes + 6 * nd);  // 2 * nd local counts
+  uint32_t* lb = lc + 2 * nd;                  // 2 * nd block bases
+  for (int i = threadIdx.x; i < nd; i += blockDim.x)
+    for (int k = 0; k < 3; ++k) {
+      boxes[i * 6 + k] = P.desc[i].lo[k];
+      boxes[i * 6 + 3 + k] = P.desc[i].hi[k];
+    }
+  const DevCounters* c = P.ctr;
+  const long long total = (long long)(c->r_end - c->r_beg) + (c->s_end - c->s_beg);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = blockIdx.x * (long long)blockDim.x; base < total; base += stride) {
+    for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) lc[i] = 0;
+    __syncthreads();
+    long long i = base + threadIdx.x;
+    int kind = 0;
+    uint32_t r = 0;
+    bool valid = passRay(P, i, &kind, &r);
+    float o[3] = {0, 0, 0}, d[3] = {1, 0, 0};
+    int spawn = -1;
+    if (valid) {
+      const RayPool& pool = kind ? P.S : P.R;
+      float4 op = pool.op[r], ds = pool.ds[r];
+      o[0] = op.x;
+      o[1] = op.y;
+      o[2] = op.z;
+      d[0] = ds.x;
+      d[1] = ds.y;
+      d[2] = ds.z;
+      if (kind) spawn = __float_as_int(pool.wd[r].w) >> 8;
+    }
+    // pass 1: local counts
+    bool spawn_listed = false;
+    if (valid) {
+      for (int dd = 0; dd < nd; ++dd) {
+        float tmin;
+        if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
+        atomicAdd(&lc[kind * nd + dd], 1u);
+        spawn_listed |= (dd == spawn);
+      }
+      if (kind == 1 && spawn >= 0 && !spawn_listed) atomicAdd(&lc[nd + spawn], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 2 * nd; j += blockDim.x) {
+      uint32_t n = lc[j];
+      lb[j] = n ? atomicAdd(&P.cursor[j], n) : 0u;
+      lc[j] = 0;
+    }
+    __syncthreads();
+    // pass 2: write
+    if (valid) {
+      for (int dd = 0; dd < nd; ++dd) {
+        float tmin;
+        if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
+        uint32_t slot = lb[kind * nd + dd] + atomicAdd(&lc[kind * nd + dd], 1u);
+        uint32_t dst = P.off[kind * (nd + 1) + dd] + slot;
+        if (dst < P.q_cap[kind]) P.q[kind][dst] = QEntry{r, tmin};
+      }
+      if (kind == 1 && spawn >= 0 && !spawn_listed) {
+        uint32_t slot = lb[nd + spawn] + atomicAdd(&lc[nd + spawn], 1u);
+        uint32_t dst = P.off[(nd + 1) + spawn] + slot;
+        // the spawn domain always tests its own shadow rays (procShads2)
+        if (dst < P.q_cap[1]) P.q[1][dst] = QEntry{r, -kInfF};
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Shading (ooc_shader_ao.h:81-197, ooc_shader_pt.h:93-227) and spawning.
+struct LaneCounts {
+  unsigned long long rvis = 0, svis = 0, shades = 0, pruned = 0, nodes = 0, tris = 0;
+};
+
+__device__ __forceinline__ void writeRay(const RayPool& pool, uint32_t idx, f3 org, int pixid,
+                                         f3 dir, int samid, f3 w, int depth_word,
+                                         const float h[4]) {
+  pool.op[idx] = make_float4(org.x, org.y, org.z, __int_as_float(pixid));
+  pool.ds[idx] = make_float4(dir.x, dir.y, dir.z, __int_as_float(samid));
+  pool.wd[idx] = make_float4(w.x, w.y, w.z, __int_as_float(depth_word));
+  pool.h[idx] = make_float4(h[0], h[1], h[2], h[3]);
+}
+
+// Children's history: parent's history[0..depth) then t, the rest infinity.
+__device__ __forceinline__ void childHistory(const float ph[4], int depth, float t, float h[4]) {
+#pragma unroll
+  for (int k = 0; k < kHistory; ++k) h[k] = (k < depth) ? ph[k] : (k == depth ? t : kInfF);
+}
+
+__device__ __forceinline__ void spawnShadow(const TraceParams& P, bool pred, f3 pos, int pixid,
+                                            f3 wi, int samid, f3 Lr, int depth, int dom,
+                                            const float hist[4], float t, float tmax) {
+  uint32_t idx = waveAppend(pred, &P.ctr->s_count, P.s_cap, &P.ctr->err);
+  if (idx == 0xffffffffu) return;
+  float h[4];
+  childHistory(hist, depth, t, h);
+  writeRay(P.S, idx, pos, pixid, wi, samid, Lr, (depth + 1) | (dom << 8), h);
+  P.Sx.tmax[idx] = tmax;
+  P.Sx.occluded[idx] = 0u;
+}
+
+__device__ __forceinline__ void spawnBounce(const TraceParams& P, bool pred, bool pending, f3 pos,
+                                            int pixid, f3 wi, int samid, f3 Lr, int depth,
+                                            const float hist[4], float t) {
+  // rays whose next virtual depth is 4 go to the pending pool (genR2,
+  // ooc_shader_pt.h:73-90) with their parent's full history
+  uint32_t ir = waveAppend(pred && !pending, &P.ctr->r_count, P.r_cap, &P.ctr->err);
+  uint32_t ip = waveAppend(pred && pending, &P.ctr->p_count, P.p_cap, &P.ctr->err);
+  float h[4];
+  childHistory(hist, depth, t, h);
+  if (ir != 0xffffffffu) writeRay(P.R, ir, pos, pixid, wi, samid, Lr, depth + 1, h);
+  if (ip != 0xffffffffu) writeRay(P.Pn, ip, pos, pixid, wi, samid, Lr, 0, h);
+}
+
+// Light sampling (render/light.h:41-89 and the rect light)
+__device__ __forceinline__ f3 sampleLight(const DevLight& L, uint32_t& rng, f3 pos, f3 nff, f3* wi,
+                                          float* pdf, float* tmax) {
+  if (L.type == kPointLight) {
+    *wi = normalize(sub(mk(L.pos[0], L.pos[1], L.pos[2]), pos));
+    *pdf = 1.0f;
+    *tmax = kInfF;
+    return mk(L.radiance[0], L.radiance[1], L.radiance[2]);
+  }
+  float u1 = rng1D(rng);
+  float u2 = rng1D(rng);
+  if (L.type == kHemisphereLight) {
+    cosineSample(u1, u2, nff, wi, pdf);
+    *tmax = kInfF;
+    return mk(L.radiance[0], L.radiance[1], L.radiance[2]);
+  }
+  f3 p = add(add(mk(L.pos[0], L.pos[1], L.pos[2]), scl(u1, mk(L.e1[0], L.e1[1], L.e1[2]))),
+             scl(u2, mk(L.e2[0], L.e2[1], L.e2[2])));
+  f3 d = sub(p, pos);
+  float dist2 = dot(d, d);
+  float dist = sqrtf(dist2);
+  *wi = scl(d, 1.0f / dist);
+  float cosl = -dot(mk(L.normal[0], L.normal[1], L.normal[2]), *wi);
+  *tmax = dist;
+  *pdf = (cosl > 0.0f) ? dist2 / (L.area * cosl) : 0.0f;
+  return mk(L.radiance[0], L.radiance[1], L.radiance[2]);
+}
+
+__device__ __forceinline__ float frDielectric(float cosI, float etaI, float etaT, f3 wo, f3 n, f3* wt) {
+  float sin2I = fmaxf(0.0f, 1.0f - (cosI * cosI));
+  float eta = etaI / etaT;
+  float sin2T = eta * eta * sin2I;
+  if (sin2T >= 1.0f) return 1.0f;
+  float cosT = sqrtf(1.0f - sin2T);
+  float Rparl = ((etaT * cosI) - (etaI * cosT)) / ((etaT * cosI) + (etaI * cosT));
+  float Rperp = ((etaI * cosI) - (etaT * cosT)) / ((etaI * cosI) + (etaT * cosT));
+  float fr = (Rparl * Rparl + Rperp * Rperp) / 2.0f;
+  *wt = add(scl(eta, neg(wo)), scl(n, eta * cosI - cosT));
+  return fr;
+}
+
+// All lanes of a wave call this together (the spawn appends are ballots);
+// `act` marks lanes whose hit was accepted by the history buffer.
+__device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int dom, float4 op,
+                                           float4 ds, float4 wd, const float hist[4], int depth,
+                                           float t, float u, float v, int prim, int ray_depth) {
+  const DomainDesc& D = P.desc[dom];
+  f3 org = mk(op.x, op.y, op.z), dir = mk(ds.x, ds.y, ds.z);
+  int pixid = __float_as_int(op.w), samid = __float_as_int(ds.w);
+  f3 pos = mk(dir.x * t + org.x, dir.y * t + org.y, dir.z * t + org.z);
+  uint32_t color = 0;
+  f3 normal = mk(0, 0, 1);
+  if (act) {  // render/trimesh_buffer.cc:327-360
+    const uint32_t* fc = P.face + (size_t)(D.face_off + prim) * 3;
+    uint32_t i0 = fc[0], i1 = fc[1], i2 = fc[2];
+    uint32_t c0 = P.rgb[D.vtx_off + i0], c1 = P.rgb[D.vtx_off + i1], c2 = P.rgb[D.vtx_off + i2];
+    float w = 1.f - u - v;
+    uint32_t r = (uint32_t)((float)((c0 >> 16) & 0xff) * w + (float)((c1 >> 16) & 0xff) * u +
+                            (float)((c2 >> 16) & 0xff) * v);
+    uint32_t g = (uint32_t)((float)((c0 >> 8) & 0xff) * w + (float)((c1 >> 8) & 0xff) * u +
+                            (float)((c2 >> 8) & 0xff) * v);
+    uint32_t b = (uint32_t)((float)(c0 & 0xff) * w + (float)(c1 & 0xff) * u + (float)(c2 & 0xff) * v);
+    color = (r << 16) | (g << 8) | b;
+    const float* n0 = P.nrm + (size_t)(D.vtx_off + i0) * 3;
+    const float* n1 = P.nrm + (size_t)(D.vtx_off + i1) * 3;
+    const float* n2 = P.nrm + (size_t)(D.vtx_off + i2) * 3;
+    normal = mk((n0[0] * w) + (n1[0] * u) + (n2[0] * v), (n0[1] * w) + (n1[1] * u) + (n2[1] * v),
+                (n0[2] * w) + (n1[2] * u) + (n2[2] * v));
+  }
+  f3 surf = unpackColor(color);
+  f3 wo = neg(dir);
+  f3 Lin = mk(wd.x, wd.y, wd.z);
+  float cos_theta_i = dot(wo, normal);
+  bool entering = (cos_theta_i > 0.0f);
+  f3 nff = normalize(entering ? normal : neg(normal));
+  const bool delta = (D.mtl != kDiffuse);
+  const int next_virtual = depth + 1;
+  const int next_actual = ray_depth + next_virtual;
+  const bool pending = (next_virtual == kHistory);
+  f3 wi;
+  float pdf, costheta;
+  if (P.ao_mode) {
+    const float ao_weight = 1.0f / (float)P.ao_samples;
+    for (int l = 0; l < P.ao_samples; ++l) {
+      uint32_t rng = rngInit(pixid * (l + 1));
+      float u1 = rng1D(rng), u2 = rng1D(rng);
+      cosineSample(u1, u2, nff, &wi, &pdf);
+      costheta = clamp01(dot(nff, wi));
+      f3 Lr = scl(mul(Lin, surf), ((kOneOverPi * costheta) * ao_weight) / pdf);
+      spawnShadow(P, act && hasPositive(Lr), pos, pixid, wi, samid, Lr, depth, dom, hist, t,
+                  P.ao_max_dist);
+    }
+  } else {
+    bool lit = act && !delta;
+    uint32_t ls = rngInit(samid * next_actual);
+    f3 ks = mk(P.ks[0], P.ks[1], P.ks[2]);
+    for (int l = 0; l < P.nlights; ++l) {
+      const DevLight& L = P.lights[l];
+      int nsamp = (L.type == kPointLight) ? 1 : P.ao_samples;
+      for (int s = 0; s < nsamp; ++s) {
+        float tmax;
+        f3 Li = sampleLight(L, ls, pos, nff, &wi, &pdf, &tmax);
+        bool ok = lit && pdf > 0.0f;
+        f3 Lr = mk(0, 0, 0);
+        if (ok) {
+          costheta = clamp01(dot(nff, wi));
+          // blinnPhong (render/reflection.h:202-214)
+          f3 half = normalize(add(wi, wo));
+          float ndh = clamp01(dot(nff, half));
+          float pw = (float)pow((double)ndh, (double)P.shininess);
+          f3 bp = mul(Li, add(scl(surf, costheta), scl(ks, pw)));
+          float sc = (L.type == kPointLight) ? (1.0f / pdf) : (1.0f / (pdf * (float)nsamp));
+          Lr = scl(mul(Lin, bp), sc);
+        }
+        spawnShadow(P, ok && hasPositive(Lr), pos, pixid, wi, samid, Lr, depth, dom, hist, t, tmax);
+      }
+    }
+  }
+  // bounce (ooc_shader_pt.h:177-226)
+  bool bounce = act && next_actual < P.bounces;
+  wo = normalize(wo);
+  if (delta) {
+    bool refl = false, trans = false;
+    f3 wr = mk(0, 0, 0), wt = mk(0, 0, 0), Lr1 = mk(0, 0, 0), Lr2 = mk(0, 0, 0);
+    if (bounce && cos_theta_i != 0.0f) {
+      float ci = fminf(fmaxf(cos_theta_i, -1.0f), 1.0f);
+      float ac = fabsf(ci);
+      if (!entering) ci = ac;
+      float fr = 1.0f;
+      if (D.mtl == kMirror) {
+        refl = true;
+      } else {
+        float etaI = entering ? D.eta_a : D.eta_b;
+        float etaT = entering ? D.eta_b : D.eta_a;
+        if (D.mtl == kGlass) {
+          fr = frDielectric(ci, etaI, etaT, wo, nff, &wt);
+          refl = (fr != 0.0f);
+          trans = (fr != 1.0f);
+        } else {
+          float sin2I = fmaxf(0.0f, 1.0f - (ci * ci));
+          float eta = etaI / etaT;
+          float sin2T = eta * eta * sin2I;
+          if (sin2T >= 1.0f) {
+            refl = true;
+            fr = 1.0f;
+          } else {
+            float cosT = sqrtf(1.0f - sin2T);
+            wt = add(scl(eta, neg(wo)), scl(nff, eta * ci - cosT));
+            trans = true;
+            fr = 0.0f;
+          }
+        }
+      }
+      if (refl) {
+        wr = normalize(add(neg(wo), scl(2.0f * dot(wo, nff), nff)));
+        Lr1 = scl(Lin, fr / ac);
+      }
+      if (trans) {
+        wt = normalize(wt);
+        Lr2 = scl(Lin, (1.0f - fr) / ac);
+      }
+    }
+    spawnBounce(P, refl && hasPositive(Lr1), pending, pos, pixid, wr, samid, Lr1, depth, hist, t);
+    spawnBounce(P, trans && hasPositive(Lr2), pending, pos, pixid, wt, samid, Lr2, depth, hist, t);
+  } else {
+    f3 Lr = mk(0, 0, 0);
+    bool go = false;
+    if (bounce) {
+      uint32_t rng = rngInit(samid * next_actual);
+      float u1 = rng1D(rng), u2 = rng1D(rng);
+      cosineSample(u1, u2, nff, &wi, &pdf);
+      costheta = clamp01(dot(nff, wi));
+      Lr = dvs(scl(scl(mul(Lin, surf), kOneOverPi), costheta), pdf);
+      go = hasPositive(Lr);
+      if (go && P.rr_start > 0 && next_actual >= P.rr_start) {
+        // Russian roulette (new): survive with p = min(1, max(Lr)); the draw
+        // continues the bounce sampler's stream
+        float p = fmaxf(Lr.x, fmaxf(Lr.y, Lr.z));
+        if (p > 1.0f) p = 1.0f;
+        float uu = rng1D(rng);
+        go = uu < p;
+        if (go) Lr = dvs(Lr, p);
+      }
+    }
+    spawnBounce(P, go, pending, pos, pixid, wi, samid, Lr, depth, hist, t);
+  }
+}
+
+// One radiance queue entry (filterRqs + procRads, ooc_tcontext.inl:28-49,137-150)
+__device__ __forceinline__ void radianceEntry(const TraceParams& P, int dom, bool valid,
+                                              QEntry e, LaneStack& st, LaneCounts& lc,
+                                              int ray_depth) {
+  const DomainDesc& D = P.desc[dom];
+  float4 op = make_float4(0, 0, 0, 0), ds = make_float4(1, 0, 0, 0), wd = make_float4(0, 0, 0, 0);
+  float hist[4] = {kInfF, kInfF, kInfF, kInfF};
+  int depth = 0;
+  bool go = false;
+  if (valid) {
+    op = P.R.op[e.ray];
+    ds = P.R.ds[e.ray];
+    wd = P.R.wd[e.ray];
+    float4 hh = P.R.h[e.ray];
+    hist[0] = hh.x;
+    hist[1] = hh.y;
+    hist[2] = hh.z;
+    hist[3] = hh.w;
+    depth = __float_as_int(wd.w) & 0xff;
+    go = (e.tdom <= hist[depth]) &&
+         vbufCorrect(P.vbuf, __float_as_int(ds.w), hist, depth);
+    if (!go) lc.pruned++;
+  }
+  float t = kInfF, u = 0, v = 0;
+  int prim = -1;
+  if (go) {
+    float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
+    TravCount tc;
+    closestHit<true>(P.nodes + D.node_off, P.tris + D.tri_off, o, d, kRayEps, kInfF,
+                     (uint32_t)D.num_nodes, st, &t, &u, &v, &prim, &tc);
+    lc.rvis++;
+    lc.nodes += tc.nodes;
+    lc.tris += tc.tris;
+  }
+  bool upd = go && prim >= 0 && vbufUpdate(P.vbuf, __float_as_int(ds.w), hist, depth, t);
+  if (upd) {
+    atomicMin((uint32_t*)&P.R.h[e.ray] + depth, __float_as_uint(t));
+    lc.shades++;
+  }
+  if (go && P.log_on) {  // visit log for the predictor: (ray, domain, t)
+    uint32_t i = atomicAdd(&P.ctr->vlog_count, 1u);
+    if (i < P.vlog_cap) {
+      P.vlog[(size_t)i * 3] = e.ray;
+      P.vlog[(size_t)i * 3 + 1] = (uint32_t)dom;
+      P.vlog[(size_t)i * 3 + 2] = __float_as_uint(prim >= 0 ? t : kInfF);
+    }
+  }
+  // all lanes take part in the spawn ballots
+  shadeSpawn(P, upd, dom, op, ds, wd, hist, depth, t, u, v, prim, ray_depth);
+}
+
+// One shadow queue entry (filterSqs + procShads, ooc_tcontext.inl:51-70,152-168)
+__device__ __forceinline__ void shadowEntry(const TraceParams& P, int dom, bool valid, QEntry e,
+                                            LaneStack& st, LaneCounts& lc) {
+  if (!valid) return;
+  const DomainDesc& D = P.desc[dom];
+  if (__hip_atomic_load(&P.Sx.occluded[e.ray], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    lc.pruned++;
+    return;
+  }
+  float4 op = P.S.op[e.ray], ds = P.S.ds[e.ray], wd = P.S.wd[e.ray], hh = P.S.h[e.ray];
+  float hist[4] = {hh.x, hh.y, hh.z, hh.w};
+  int depth = __float_as_int(wd.w) & 0xff;
+  if (!vbufCorrect(P.vbuf, __float_as_int(ds.w), hist, depth)) {
+    lc.pruned++;
+    return;
+  }
+  float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
+  TravCount tc;
+  bool occ = anyHit<true>(P.nodes + D.node_off, P.tris + D.tri_off, o, d, kRayEps,
+                          P.Sx.tmax[e.ray], (uint32_t)D.num_nodes, st, &tc);
+  lc.svis++;
+  lc.nodes += tc.nodes;
+  lc.tris += tc.tris;
+  if (occ)
+    __hip_atomic_store(&P.Sx.occluded[e.ray], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void flushLane(const TraceParams& P, const LaneCounts& lc,
+                                          const LaneStack& st) {
+  DevCounters* c = P.ctr;
+  flushCount(&c->radiance_visits, lc.rvis);
+  flushCount(&c->shadow_visits, lc.svis);
+  flushCount(&c->shades, lc.shades);
+  flushCount(&c->pruned, lc.pruned);
+  flushCount(&c->node_visits, lc.nodes);
+  flushCount(&c->tri_tests, lc.tris);
+  if (st.overflowed) atomicOr(&c->err, 4u);
+}
+
+// Plain form: one launch per scheduled domain, both queues of the domain.
+__global__ void __launch_bounds__(kBlock) kTraceDomain(TraceParams P, int dom, uint32_t nrb) {
+  __shared__ uint32_t ring[kRing * kBlock];
+  LaneStack st;
+  long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  st.init(ring, P.ovf + gtid * kOverflow);
+  LaneCounts lc;
+  const int nd = P.ndomains;
+  const int ray_depth = P.tile_rec[2];
+  if (blockIdx.x < nrb) {
+    uint32_t b = P.off[dom], e = min(P.off[dom + 1], P.q_cap[0]);
+    uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = i < e;
+    QEntry q = valid ? P.q[0][i] : QEntry{0, 0.f};
+    radianceEntry(P, dom, valid, q, st, lc, ray_depth);
+  } else {
+    uint32_t b = P.off[(nd + 1) + dom], e = min(P.off[(nd + 1) + dom + 1], P.q_cap[1]);
+    uint32_t i = b + (blockIdx.x - nrb) * blockDim.x + threadIdx.x;
+    bool valid = i < e;
+    QEntry q = valid ? P.q[1][i] : QEntry{0, 0.f};
+    shadowEntry(P, dom, valid, q, st, lc);
+  }
+  flushLane(P, lc, st);
+}
+
+// Batched form: persistent blocks pull (domain, chunk) work items.
+__global__ void __launch_bounds__(256) kTracePersistent(TraceParams P, int phase) {
+  extern __shared__ uint32_t ring[];
+  __shared__ uint32_t item_sh;
+  DevCounters* c = P.ctr;
+  if (!c->active) return;
+  LaneStack st;
+  long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  st.init(ring, P.ovf + gtid * kOverflow);
+  LaneCounts lc;
+  const uint32_t nitems = c->n_items[0];
+  const int ray_depth = P.tile_rec[2];
+  const WorkItem* items = P.items;
+  while (true) {
+    if (threadIdx.x == 0) item_sh = atomicAdd(&c->item_ctr[phase], 1u);
+    __syncthreads();
+    uint32_t it = item_sh;
+    __syncthreads();
+    if (it >= nitems) break;
+    WorkItem w = items[it];
+    if (phase == 1 && w.kind == 1) continue;  // deferred pass revisits radiance entries only
+    const uint32_t wend = min(w.end, P.q_cap[w.kind]);
+    for (uint32_t base = w.begin; base < wend; base += blockDim.x) {
+      uint32_t i = base + threadIdx.x;
+      bool valid = i < wend;
+      QEntry q = valid ? P.q[w.kind][i] : QEntry{0, 0.f};
+      if (w.kind == 0) {
+        if (phase == 0 && P.learned && valid && P.phase_b[i]) valid = false;  // deferred
+        if (phase == 1 && valid && !P.phase_b[i]) valid = false;
+        radianceEntry(P, w.domain, valid, q, st, lc, ray_depth);
+      } else {
+        shadowEntry(P, w.domain, valid, q, st, lc);
+      }
+    }
+  }
+  flushLane(P, lc, st);
+}
+
+// Retire (ooc_tcontext.inl:123-135): shadow rays tested in this pass.
+__global__ void kRetire(TraceParams P) {
+  DevCounters* c = P.ctr;
+  if (!c->active) return;
+  const uint32_t b = c->s_beg, e = c->s_end;
+  unsigned long long useful = 0;
+  for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
+    float4 hh = P.S.h[i];
+    float hist[4] = {hh.x, hh.y, hh.z, hh.w};
+    float4 wd = P.S.wd[i];
+    int depth = __float_as_int(wd.w) & 0xff;
+    int samid = __float_as_int(P.S.ds[i].w);
+    if (!vbufCorrect(P.vbuf, samid, hist, depth)) continue;
+    ++useful;
+    if (P.Sx.occluded[i]) continue;
+    int pixid = __float_as_int(P.S.op[i].w);
+    float* px = P.image + (size_t)pixid * 4;
+    atomicAdd(px + 0, (float)(P.inv_spp * (double)wd.x));
+    atomicAdd(px + 1, (float)(P.inv_spp * (double)wd.y));
+    atomicAdd(px + 2, (float)(P.inv_spp * (double)wd.z));
+  }
+  flushCount(&c->useful, useful);
+}
+
+// Window end: census of non-stale spawned radiance rays and final hits.
+__global__ void kWindowEnd(TraceParams P) {
+  DevCounters* c = P.ctr;
+  unsigned long long useful = 0, hits = 0;
+  const uint32_t rend = min(c->r_count, P.r_cap);
+  for (uint32_t i = c->r_init + blockIdx.x * blockDim.x + threadIdx.x; i < rend;
+       i += gridDim.x * blockDim.x) {
+    float4 hh = P.R.h[i];
+    float hist[4] = {hh.x, hh.y, hh.z, hh.w};
+    int depth = __float_as_int(P.R.wd[i].w) & 0xff;
+    if (vbufCorrect(P.vbuf, __float_as_int(P.R.ds[i].w), hist, depth)) ++useful;
+  }
+  const long long S = (long long)P.tile_rec[1] * P.image_w * P.spp;
+  for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < S;
+       s += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long* w = P.vbuf + s * kHistory;
+    float prev = 0.f;
+    for (int d = 0; d < kHistory; ++d) {
+      unsigned long long x = w[d];
+      float t = wT(x);
+      if (t == kInfF) break;
+      if (d > 0 && wP(x) != __float_as_uint(prev)) break;
+      ++hits;
+      prev = t;
+    }
+  }
+  flushCount(&c->useful, useful);
+  flushCount(&c->final_hits, hits);
+}
+__global__ void kWindowAdvance(TraceParams P) {
+  int* rec = (int*)P.tile_rec;
+  rec[2] += kHistory;
+}
+
+// Visit log -> predictor samples: (org, dir, domain, label) where the label
+// says whether this domain produced the ray's final closest hit.
+__global__ void kLogFinalize(TraceParams P) {
+  DevCounters* c = P.ctr;
+  if (!P.log_on) return;
+  uint32_t n = min(c->vlog_count, P.vlog_cap);
+  const uint32_t* vl = P.vlog;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t r = vl[(size_t)i * 3];
+    int dom = (int)vl[(size_t)i * 3 + 1];
+    float t = __uint_as_float(vl[(size_t)i * 3 + 2]);
+    float4 wd = P.R.wd[r];
+    int depth = __float_as_int(wd.w) & 0xff;
+    float4 hh = P.R.h[r];
+    float best = depth == 0 ? hh.x : depth == 1 ? hh.y : depth == 2 ? hh.z : hh.w;
+    float label = (t != kInfF && t == best) ? 1.f : 0.f;
+    unsigned long long slot = atomicAdd(&c->log_count, 1ull);
+    if (slot >= P.log_cap) continue;
+    float4 op = P.R.op[r], ds = P.R.ds[r];
+    float* o = P.log_buf + slot * 8;
+    o[0] = op.x;
+    o[1] = op.y;
+    o[2] = op.z;
+    o[3] = ds.x;
+    o[4] = ds.y;
+    o[5] = ds.z;
+    o[6] = (float)dom;
+    o[7] = label;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+struct TracerState {
+  long long S = 0;  // samples of the largest tile
+  uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, items_cap = 0;
+  int ndomains = 0;
+  DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, cursor, score,
+      order, q[2], items, ovf, phase_b, vlog, log;
+  unsigned long long log_cap = 0;
+  bool logging = false;
+  int grid_persistent = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_key[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<uint32_t> h_cnt;
+  std::vector<int> h_order;
+  void releaseGraph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+  }
+};
+
+void tracerRelease(Ctx* c) {
+  if (!c->tracer) return;
+  TracerState* T = c->tracer;
+  T->releaseGraph();
+  for (auto* arr : {T->R, T->Pn, T->Sh})
+    for (int k = 0; k < 4; ++k) arr[k].release();
+  for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
+                    &T->off, &T->cursor, &T->score, &T->order, &T->q[0], &T->q[1], &T->items,
+                    &T->ovf, &T->phase_b, &T->vlog, &T->log})
+    b->release();
+  delete T;
+  c->tracer = nullptr;
+}
+
+static void tileList(const spray_config& cfg, std::vector<int>* ys, std::vector<int>* hs) {
+  long long samples = (long long)cfg.image_w * cfg.image_h * cfg.pixel_samples;
+  long long nt = (samples + cfg.max_samples_per_rank - 1) / cfg.max_samples_per_rank;
+  int tile_h = cfg.image_h / (int)nt;
+  ys->clear();
+  hs->clear();
+  for (int y = 0; y < cfg.image_h; y += tile_h) {
+    ys->push_back(y);
+    hs->push_back(std::min(tile_h, cfg.image_h - y));
+  }
+}
+
+static int spawnPerVertex(const Ctx* c) {
+  if (c->cfg.ao_mode) return c->cfg.ao_samples;
+  int n = 0;
+  for (auto& L : c->scene.lights) n += (L.type == kPointLight) ? 1 : c->cfg.ao_samples;
+  return std::max(n, 1);
+}
+
+// Grows a buffer; a reallocation invalidates the captured graph.
+static void need(TracerState* T, DevBuf& b, size_t bytes) {
+  if (bytes <= b.bytes) return;
+  T->releaseGraph();
+  b.reserve(bytes);
+}
+
+static void ensureState(Ctx* c, double grow) {
+  const spray_config& cfg = c->cfg;
+  std::vector<int> ys, hs;
+  tileList(cfg, &ys, &hs);
+  long long S = (long long)cfg.image_w * (*std::max_element(hs.begin(), hs.end())) * cfg.pixel_samples;
+  if (!c->tracer) c->tracer = new TracerState();
+  TracerState* T = c->tracer;
+  int nd = c->pool.ndomains;
+  if (nd > 1024) throw StatusError(SPRAY_ERR_CAPACITY, "the tracer supports at most 1024 domains");
+  int depths = std::min(cfg.bounces, kHistory);
+  double spec = 2.0 * grow;  // headroom for speculative re-shading
+  auto cap = [](double v) { return (uint32_t)std::min<double>(3.9e9, v + 4096); };
+  uint32_t r_cap = cap(S * (1.0 + spec * depths));
+  uint32_t p_cap = cap(S * spec);
+  uint32_t s_cap = cap((double)S * spawnPerVertex(c) * spec * depths);
+  uint32_t per_ray = (uint32_t)std::min(nd, 12);  // domains a ray crosses, typical bound
+  uint32_t q0 = cap((double)S * spec * per_ray);
+  uint32_t q1 = cap((double)S * spawnPerVertex(c) * spec * per_ray * 0.5);
+  if (T->S != S || T->ndomains != nd) T->releaseGraph();
+  T->S = S;
+  T->ndomains = nd;
+  T->r_cap = std::max(T->r_cap, r_cap);
+  T->s_cap = std::max(T->s_cap, s_cap);
+  T->p_cap = std::max(T->p_cap, p_cap);
+  T->q_cap[0] = std::max(T->q_cap[0], q0);
+  T->q_cap[1] = std::max(T->q_cap[1], q1);
+  for (int k = 0; k < 4; ++k) {
+    need(T, T->R[k], (size_t)T->r_cap * 16);
+    need(T, T->Pn[k], (size_t)T->p_cap * 16);
+    need(T, T->Sh[k], (size_t)T->s_cap * 16);
+  }
+  need(T, T->s_tmax, (size_t)T->s_cap * 4);
+  need(T, T->s_occ, (size_t)T->s_cap * 4);
+  need(T, T->vbuf, (size_t)S * kHistory * 8);
+  need(T, T->ctr, sizeof(DevCounters));
+  need(T, T->tile_rec, 64);
+  need(T, T->cnt, (size_t)2 * nd * 4 + 64);
+  need(T, T->off, (size_t)2 * (nd + 1) * 4 + 64);
+  need(T, T->cursor, (size_t)2 * nd * 4 + 64);
+  need(T, T->score, (size_t)nd * 8 + 64);
+  need(T, T->order, (size_t)nd * 4 + 64);
+  need(T, T->q[0], (size_t)T->q_cap[0] * sizeof(QEntry));
+  need(T, T->q[1], (size_t)T->q_cap[1] * sizeof(QEntry));
+  need(T, T->phase_b, (size_t)T->q_cap[0] + 64);
+  T->h_cnt.resize(2 * nd);
+  T->h_order.resize(nd);
+  int chunk = std::max(64, c->batch_chunk);
+  uint32_t items_cap = (uint32_t)((T->q_cap[0] + T->q_cap[1]) / chunk + 2 * nd + 64);
+  if (T->items_cap < items_cap) {
+    T->items_cap = items_cap;
+    need(T, T->items, (size_t)items_cap * sizeof(WorkItem));
+  }
+  need(T, T->image, (size_t)cfg.image_w * cfg.image_h * 16);
+  hipDeviceProp_t prop;
+  SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
+  int block = 64 * std::max(1, c->batch_tile);
+  int per_cu = std::max(1, 2048 / block);
+  T->grid_persistent = prop.multiProcessorCount * per_cu;
+}
+
+static TraceParams makeParams(Ctx* c) {
+  TracerState* T = c->tracer;
+  const spray_config& cfg = c->cfg;
+  TraceParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.cam = c->cam;
+  P.image_w = cfg.image_w;
+  P.spp = cfg.pixel_samples;
+  P.ao_mode = cfg.ao_mode;
+  P.ao_samples = cfg.ao_samples;
+  P.bounces = cfg.bounces;
+  P.rr_start = cfg.rr_start;
+  P.ao_max_dist = cfg.ao_max_dist;
+  P.shininess = cfg.shininess;
+  for (int k = 0; k < 3; ++k) P.ks[k] = cfg.ks[k];
+  P.inv_spp = 1.0 / (double)cfg.pixel_samples;
+  P.inv_spp_f = (float)P.inv_spp;
+  P.ndomains = c->pool.ndomains;
+  int nl = cfg.ao_mode ? 0 : (int)c->scene.lights.size();
+  if (nl > kMaxLights) throw StatusError(SPRAY_ERR_CAPACITY, "more than 8 lights");
+  P.nlights = nl;
+  for (int l = 0; l < nl; ++l) {
+    const HostLight& L = c->scene.lights[l];
+    DevLight& D = P.lights[l];
+    D.type = L.type;
+    for (int k = 0; k < 3; ++k) {
+      D.pos[k] = L.pos[k];
+      D.e1[k] = L.e1[k];
+      D.e2[k] = L.e2[k];
+      D.radiance[k] = L.radiance[k];
+      D.normal[k] = L.normal[k];
+    }
+    D.area = L.area;
+  }
+  const DomainPool& pool = c->pool;
+  P.desc = pool.d_desc.as<DomainDesc>();
+  P.nodes = pool.nodes.as<BNode4>();
+  P.tris = pool.tris.as<LTri>();
+  P.vtx = pool.vtx.as<float>();
+  P.face = pool.face.as<uint32_t>();
+  P.rgb = pool.rgb.as<uint32_t>();
+  P.nrm = pool.nrm.as<float>();
+  auto pool4 = [](DevBuf* b) {
+    RayPool r;
+    r.op = b[0].as<float4>();
+    r.ds = b[1].as<float4>();
+    r.wd = b[2].as<float4>();
+    r.h = b[3].as<float4>();
+    return r;
+  };
+  P.R = pool4(T->R);
+  P.Pn = pool4(T->Pn);
+  P.S = pool4(T->Sh);
+  P.Sx.tmax = T->s_tmax.as<float>();
+  P.Sx.occluded = T->s_occ.as<uint32_t>();
+  P.r_cap = T->r_cap;
+  P.s_cap = T->s_cap;
+  P.p_cap = T->p_cap;
+  P.vbuf = T->vbuf.as<unsigned long long>();
+  P.image = T->image.as<float>();
+  P.ctr = T->ctr.as<DevCounters>();
+  P.tile_rec = T->tile_rec.as<int>();
+  int nd = P.ndomains;
+  P.cnt = T->cnt.as<uint32_t>();
+  P.off = T->off.as<uint32_t>();
+  P.cursor = T->cursor.as<uint32_t>();
+  P.score = T->score.as<unsigned long long>();
+  P.order = T->order.as<int>();
+  P.q[0] = T->q[0].as<QEntry>();
+  P.q[1] = T->q[1].as<QEntry>();
+  P.q_cap[0] = T->q_cap[0];
+  P.q_cap[1] = T->q_cap[1];
+  P.items = T->items.as<WorkItem>();
+  P.items_cap = T->items_cap;
+  P.chunk = std::max(64, c->batch_chunk);
+  P.phase_b = T->phase_b.as<uint8_t>();
+  P.learned = (cfg.schedule == SPRAY_SCHEDULE_LEARNED) ? 1 : 0;
+  P.log_on = T->logging ? 1 : 0;
+  P.vlog = T->vlog.as<uint32_t>();
+  P.vlog_cap = T->logging ? (uint32_t)std::min<size_t>(T->vlog.bytes / 12, 0xffffffffu) : 0u;
+  P.log_buf = T->log.as<float>();
+  P.log_cap = T->logging ? T->log_cap : 0ull;
+  (void)nd;
+  return P;
+}
+
+static size_t countShmem(int nd) { return (size_t)nd * (6 + 3) * 4; }
+static size_t scatterShmem(int nd) { return (size_t)nd * (6 + 4) * 4; }
+
+static void ensureOverflow(TracerState* T, size_t lanes) { need(T, T->ovf, lanes * kOverflow * 4); }
+
+// One pass of the plain schedule: host reads the per-domain counts.
+static bool passPlain(Ctx* c, const TraceParams& P) {
+  TracerState* T = c->tracer;
+  hipStream_t s = c->stream;
+  int nd = P.ndomains;
+  kPassBegin<<<1, 1, 0, s>>>(P);
+  kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
+  kCount<<<1024, kBlock, countShmem(nd), s>>>(P);
+  kSchedule<<<1, 1024, 0, s>>>(P);
+  kScatter<<<1024, kBlock, scatterShmem(nd), s>>>(P);
+  DevCounters h;
+  SPRAY_HIP(hipMemcpyAsync(&h, P.ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+  SPRAY_HIP(hipMemcpyAsync(T->h_cnt.data(), P.cnt, 2 * nd * 4, hipMemcpyDeviceToHost, s));
+  SPRAY_HIP(hipMemcpyAsync(T->h_order.data(), P.order, nd * 4, hipMemcpyDeviceToHost, s));
+  SPRAY_HIP(hipStreamSynchronize(s));
+  if (!h.active) return false;
+  if (h.err) return true;  // reported after the frame
+  size_t max_blocks = 0;
+  for (int k = 0; k < nd; ++k) {
+    int d = T->h_order[k];
+    uint32_t nr = T->h_cnt[d], ns = T->h_cnt[nd + d];
+    uint32_t br = (nr + kBlock - 1) / kBlock, bs = (ns + kBlock - 1) / kBlock;
+    max_blocks = std::max<size_t>(max_blocks, br + bs);
+  }
+  ensureOverflow(T, max_blocks * kBlock);
+  TraceParams Q = P;
+  Q.ovf = T->ovf.as<uint32_t>();
+  for (int k = 0; k < nd; ++k) {
+    int d = T->h_order[k];
+    uint32_t nr = T->h_cnt[d], ns = T->h_cnt[nd + d];
+    if (nr + ns == 0) continue;
+    uint32_t br = (nr + kBlock - 1) / kBlock, bs = (ns + kBlock - 1) / kBlock;
+    kTraceDomain<<<br + bs, kBlock, 0, s>>>(Q, d, br);
+  }
+  kRetire<<<1024, 256, 0, s>>>(P);
+  if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
+  return true;
+}
+
+// One pass of the batched schedule: sizes stay on the device.
+static void passBatched(Ctx* c, const TraceParams& P) {
+  TracerState* T = c->tracer;
+  hipStream_t s = c->stream;
+  int nd = P.ndomains;
+  int block = 64 * std::max(1, c->batch_tile);
+  kPassBegin<<<1, 1, 0, s>>>(P);
+  kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
+  kCount<<<1024, kBlock, countShmem(nd), s>>>(P);
+  kSchedule<<<1, 1024, 0, s>>>(P);
+  kScatter<<<1024, kBlock, scatterShmem(nd), s>>>(P);
+  TraceParams Q = P;
+  Q.ovf = T->ovf.as<uint32_t>();
+  size_t shm = (size_t)kRing * block * 4;
+  kTracePersistent<<<T->grid_persistent, block, shm, s>>>(Q, 0);
+  if (P.learned) kTracePersistent<<<T->grid_persistent, block, shm, s>>>(Q, 1);
+  kRetire<<<1024, 256, 0, s>>>(P);
+  if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
+}
+
+static void tileBody(Ctx* c, const TraceParams& P, bool batched, int windows, int* passes_run) {
+  hipStream_t s = c->stream;
+  const long long S = c->tracer->S;
+  int eye_grid = (int)std::min<long long>(65535, (S + 255) / 256);
+  kTileStart<<<1, 1, 0, s>>>(P);
+  kGenEyes<<<eye_grid, 256, 0, s>>>(P);
+  for (int w = 0; w < windows; ++w) {
+    if (w > 0) {
+      kWindowStart<<<1, 1, 0, s>>>(P);
+      kRestartPending<<<1024, 256, 0, s>>>(P);
+      kRestartDone<<<1, 1, 0, s>>>(P);
+    }
+    kVbufReset<<<eye_grid, 256, 0, s>>>(P);
+    for (int p = 0; p <= kHistory; ++p) {
+      if (batched) {
+        passBatched(c, P);
+      } else {
+        if (!passPlain(c, P)) break;
+      }
+      if (passes_run) ++*passes_run;
+    }
+    kWindowEnd<<<1024, 256, 0, s>>>(P);
+    kWindowAdvance<<<1, 1, 0, s>>>(P);
+  }
+}
+
+void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
+  const spray_config& cfg = c->cfg;
+  std::vector<int> ys, hs;
+  tileList(cfg, &ys, &hs);
+  double grow = 1.0;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    ensureState(c, grow);
+    TracerState* T = c->tracer;
+    hipStream_t s = c->stream;
+    bool batched = cfg.schedule != SPRAY_SCHEDULE_PLAIN;
+    int block = 64 * std::max(1, c->batch_tile);
+    ensureOverflow(T, std::max<size_t>((size_t)T->grid_persistent * block, 4096));
+    TraceParams P = makeParams(c);
+    P.ovf = T->ovf.as<uint32_t>();
+    SPRAY_HIP(hipMemsetAsync(P.image, 0, (size_t)cfg.image_w * cfg.image_h * 16, s));
+    SPRAY_HIP(hipMemsetAsync(P.ctr, 0, sizeof(DevCounters), s));
+    int windows = (cfg.bounces + kHistory - 1) / kHistory;
+    hipEvent_t e0, e1;
+    SPRAY_HIP(hipEventCreate(&e0));
+    SPRAY_HIP(hipEventCreate(&e1));
+    SPRAY_HIP(hipEventRecord(e0, s));
+    int key[6] = {(int)batched, windows, (int)P.learned, c->batch_chunk, c->batch_tile,
+                  P.log_on + 2 * c->param_version};
+    if (batched && (T->graph == nullptr || std::memcmp(key, T->graph_key, sizeof(key)) != 0)) {
+      T->releaseGraph();
+      hipGraph_t g;
+      SPRAY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      tileBody(c, P, true, windows, nullptr);
+      SPRAY_HIP(hipStreamEndCapture(s, &g));
+      SPRAY_HIP(hipGraphInstantiate(&T->graph, g, nullptr, nullptr, 0));
+      SPRAY_HIP(hipGraphDestroy(g));
+      std::memcpy(T->graph_key, key, sizeof(key));
+    }
+    for (size_t ti = 0; ti < ys.size(); ++ti) {
+      int rec[4] = {ys[ti], hs[ti], 0, 0};
+      SPRAY_HIP(hipMemcpyAsync(T->tile_rec.p, rec, sizeof(rec), hipMemcpyHostToDevice, s));
+      if (batched)
+        SPRAY_HIP(hipGraphLaunch(T->graph, s));
+      else
+        tileBody(c, P, false, windows, nullptr);
+    }
+    SPRAY_HIP(hipEventRecord(e1, s));
+    DevCounters h;
+    SPRAY_HIP(hipMemcpyAsync(&h, P.ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+    SPRAY_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    SPRAY_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (h.err & 4u) throw StatusError(SPRAY_ERR_CAPACITY, "traversal stack overflow");
+    if (h.err & 1u) {  // a pool or queue bound was hit: grow and render again
+      grow *= 2.0;
+      continue;
+    }
+    if (rgba_out)
+      SPRAY_HIP(hipMemcpy(rgba_out, P.image, (size_t)cfg.image_w * cfg.image_h * 16,
+                          hipMemcpyDeviceToHost));
+    st->useful_rays = h.useful;
+    st->eye_rays = (unsigned long long)cfg.image_w * cfg.image_h * cfg.pixel_samples;
+    st->radiance_visits = h.radiance_visits;
+    st->shadow_visits = h.shadow_visits;
+    st->shades = h.shades;
+    st->final_hits = h.final_hits;
+    st->passes = h.passes;
+    st->windows = (unsigned long long)windows * ys.size();
+    st->tiles = ys.size();
+    st->queue_entries = h.queue_entries;
+    st->deferred_entries = h.deferred;
+    st->pruned_entries = h.pruned;
+    st->trace_ms = ms;
+    st->node_visits = h.node_visits;
+    st->tri_tests = h.tri_tests;
+    return;
+  }
+  throw StatusError(SPRAY_ERR_CAPACITY, "ray pools exceeded after growing 8x");
+}
+
+void tracerAutotune(Ctx* c, int* chunk_out, int* tile_out) {
+  int saved_sched = c->cfg.schedule;
+  if (c->cfg.schedule == SPRAY_SCHEDULE_PLAIN) c->cfg.schedule = SPRAY_SCHEDULE_BATCHED;
+  const int chunks[] = {256, 1024, 4096};
+  const int tiles[] = {1, 2, 4};
+  double best = 1e30;
+  int bc = c->batch_chunk, bt = c->batch_tile;
+  for (int ch : chunks)
+    for (int tl : tiles) {
+      c->batch_chunk = ch;
+      c->batch_tile = tl;
+      spray_stats st;
+      std::memset(&st, 0, sizeof(st));
+      tracerTraceFrame(c, nullptr, &st);  // warm (graph capture)
+      tracerTraceFrame(c, nullptr, &st);
+      if (st.trace_ms < best) {
+        best = st.trace_ms;
+        bc = ch;
+        bt = tl;
+      }
+    }
+  c->batch_chunk = bc;
+  c->batch_tile = bt;
+  c->cfg.schedule = saved_sched;
+  if (c->tracer) c->tracer->releaseGraph();
+  *chunk_out = bc;
+  *tile_out = bt;
+}
+
+}  // namespace spray

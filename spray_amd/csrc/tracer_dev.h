@@ -1,0 +1,164 @@
+// Device-side state and helpers of the speculative tracer.
+#pragma once
+
+#include "dmath.h"
+#include "kernels_common.h"
+#include "spray_internal.h"
+#include "traverse.h"
+
+namespace spray {
+
+constexpr int kMaxLights = 8;
+constexpr int kTopDepths = kListSize;  // list positions that carry score weight
+
+// Ray pools are SoA streams of float4 (16-byte loads per lane):
+//   op = (org.xyz, pixid)  ds = (dir.xyz, samid)
+//   wd = (w.xyz, depth | spawn_domain << 8)  h = history[4]
+struct RayPool {
+  float4 *op, *ds, *wd, *h;
+};
+struct ShadowExtra {
+  float* tmax;
+  uint32_t* occluded;
+};
+
+// Queue entry: ray index and the distance at which the ray enters the domain
+struct QEntry {
+  uint32_t ray;
+  float tdom;
+};
+
+// Work item of the batched form: a chunk of one domain's queue
+struct WorkItem {
+  int domain;
+  int kind;  // 0 radiance, 1 shadow
+  uint32_t begin, end;
+};
+
+struct DevCounters {
+  uint32_t r_count, s_count, p_count;  // pool fill levels (atomic append)
+  uint32_t r_beg, r_end, s_beg, s_end;  // rays to enqueue / trace in this pass
+  uint32_t r_init;                      // initial rays of the window (eyes / restarts)
+  uint32_t active;                      // pass has work
+  uint32_t err;                         // bit 0 pool overflow, 1 list overflow, 2 stack overflow
+  uint32_t n_items[2];                  // work items (phase A, phase B)
+  uint32_t item_ctr[2];                 // persistent-kernel work counters
+  uint32_t r_mark, s_mark;              // end of the last enqueued ranges
+  uint32_t q_total[2];                  // entries in this pass's queues
+  uint32_t vlog_count;                  // visit-log entries of this pass
+  unsigned long long useful, radiance_visits, shadow_visits, shades, final_hits, queue_entries,
+      pruned, deferred, passes, node_visits, tri_tests, log_count;
+};
+
+struct TraceParams {
+  CameraDev cam;
+  int image_w, spp, tile_y, tile_w, tile_h;  // tile_* come from the device tile record
+  int ao_mode, ao_samples, bounces, rr_start, nlights;
+  float ao_max_dist, shininess, ks[3];
+  float inv_spp_f;
+  double inv_spp;
+  int ndomains;
+  DevLight lights[kMaxLights];
+  // pool
+  const DomainDesc* desc;
+  const BNode4* nodes;
+  const LTri* tris;
+  const float* vtx;
+  const uint32_t* face;
+  const uint32_t* rgb;
+  const float* nrm;
+  // tracer state
+  RayPool R, Pn;  // radiance (window) and pending pools
+  RayPool S;
+  ShadowExtra Sx;
+  uint32_t r_cap, s_cap, p_cap;
+  unsigned long long* vbuf;  // 4 words per sample: t bits << 32 | parent bits
+  float* image;              // rgba per pixel
+  DevCounters* ctr;
+  const int* tile_rec;       // [tile_y, tile_h, ray_depth(window offset)]
+  // queues
+  uint32_t* cnt;      // [2][nd] entries per domain (radiance, shadow)
+  uint32_t* off;      // [2][nd + 1] exclusive offsets
+  uint32_t* cursor;   // [2][nd]
+  unsigned long long* score;  // [nd]
+  int* order;         // [nd] schedule
+  QEntry* q[2];       // queues (radiance, shadow)
+  uint32_t q_cap[2];
+  WorkItem* items;    // [2 phases][max items]
+  uint32_t items_cap;
+  int chunk;
+  uint32_t* ovf;      // traversal stack overflow areas
+  // learned form
+  uint8_t* phase_b;   // per radiance queue entry: 1 = deferred
+  int learned;
+  // predictor logging: visit log of the pass (ray, domain, t) and samples
+  int log_on;
+  uint32_t* vlog;
+  uint32_t vlog_cap;           // entries of 3 words
+  float* log_buf;              // 8 floats per sample
+  unsigned long long log_cap;  // samples
+};
+
+// ---- history buffer (VBuf, ooc/ooc_vbuf.cc:41-112) -------------------------
+// Each of the 4 depth slots of a sample is one 64-bit word: the accepted t in
+// the high half and the t of the parent slot in the low half.  A slot whose
+// parent field differs from the t of the slot above it is stale and reads as
+// infinity, which stands in for the reference's "reset deeper entries".
+__device__ __forceinline__ float wT(unsigned long long w) { return __uint_as_float((uint32_t)(w >> 32)); }
+__device__ __forceinline__ uint32_t wP(unsigned long long w) { return (uint32_t)w; }
+__device__ __forceinline__ unsigned long long mkW(float t, uint32_t parent) {
+  return ((unsigned long long)__float_as_uint(t) << 32) | parent;
+}
+__device__ __forceinline__ unsigned long long ldW(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// VBuf::correct: history[0..n) equals the buffer's chain
+__device__ __forceinline__ bool vbufCorrect(const unsigned long long* vb, int samid,
+                                            const float h[4], int n) {
+  const unsigned long long* w = vb + (size_t)samid * kHistory;
+  for (int i = 0; i < n; ++i) {
+    unsigned long long x = ldW(&w[i]);
+    if (wT(x) != h[i]) return false;
+    if (i > 0 && wP(x) != __float_as_uint(h[i - 1])) return false;
+  }
+  return true;
+}
+
+// VBuf::update for a ray at `depth` with hit distance t: true when the
+// candidate (history[0..depth), t) became the buffer's lexicographic minimum.
+__device__ __forceinline__ bool vbufUpdate(unsigned long long* vb, int samid, const float h[4],
+                                           int depth, float t) {
+  unsigned long long* w = vb + (size_t)samid * kHistory + depth;
+  uint32_t parent = depth > 0 ? __float_as_uint(h[depth - 1]) : 0u;
+  unsigned long long want = mkW(t, parent);
+  while (true) {
+    unsigned long long old = ldW(w);
+    if (!vbufCorrect(vb, samid, h, depth)) return false;
+    if (wP(old) == parent && !(t < wT(old))) return false;
+    if (__hip_atomic_compare_exchange_strong(w, &old, want, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return true;
+  }
+}
+
+// Wave-aggregated append: one atomic per wave, lane order preserved.
+__device__ __forceinline__ uint32_t waveAppend(bool pred, uint32_t* counter, uint32_t cap,
+                                               uint32_t* err) {
+  unsigned long long m = __ballot(pred);
+  if (m == 0ull) return 0xffffffffu;
+  int lane = __lane_id();
+  int leader = __builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  if (!pred) return 0xffffffffu;
+  if (idx >= cap) {
+    atomicOr(err, 1u);
+    return 0xffffffffu;
+  }
+  return idx;
+}
+
+}  // namespace spray

@@ -1,0 +1,162 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the same
+inputs (closest hit, any hit, domain lists, normals, whole frames)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import orc
+
+pytestmark = pytest.mark.gpu
+
+C1_CAM = dict(pos=[2.0, 2.0, 2.0 + 2.0 * math.sqrt(3.0)], lookat=[2.0, 2.0, 2.0], up=[0.0, 1.0, 0.0])
+RECT_LIGHT = "light rect 0.5 4.5 0.5 3 0 0 0 0 3 4 4 4"
+
+
+@pytest.fixture(scope="module")
+def c1(tmp_path_factory):
+    import spray_amd as sa
+    d = str(tmp_path_factory.mktemp("c1"))
+    sa.scenegen_write(d, "c1", 4, 4, 4, 64, rgb=0xFFFFFF)
+    sa.scenegen_write(d, "c2", 4, 4, 4, 64, rgb=0xB3B3B3, extra_lines=RECT_LIGHT)
+    return d
+
+
+@pytest.fixture(scope="module")
+def ctx_c1(c1):
+    import spray_amd as sa
+    cfg = sa.default_config(image_w=512, image_h=512, pixel_samples=1, ao_mode=1, ao_samples=8,
+                            bounces=1, ao_max_dist=0.5, has_camera=1, camera_pos=C1_CAM["pos"],
+                            camera_lookat=C1_CAM["lookat"], camera_up=C1_CAM["up"], nthreads=8)
+    ctx = sa.Context(cfg)
+    ctx.load_scene(os.path.join(c1, "c1.spray"), c1)
+    yield ctx
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def ref_c1(c1):
+    return orc.Scene(os.path.join(c1, "c1.spray"), c1)
+
+
+def eye_rays(w, h, spp=1):
+    ys, xs = np.mgrid[0:h, 0:w]
+    xs = xs.ravel().astype(np.float32)
+    ys = ys.ravel().astype(np.float32)
+    dirs = np.zeros((w * h, 3), np.float32)
+    orc.lib().orc_camera_dirs(orc.p(np.array(C1_CAM["pos"], np.float32)),
+                              orc.p(np.array(C1_CAM["lookat"], np.float32)),
+                              orc.p(np.array(C1_CAM["up"], np.float32)), 90.0, w, h, w * h,
+                              orc.p(xs), orc.p(ys), orc.p(dirs))
+    org = np.tile(np.array(C1_CAM["pos"], np.float32), (w * h, 1))
+    return org, dirs
+
+
+def test_closest_hit_c1_domain(ctx_c1, ref_c1):
+    """All 262,144 C1 eye rays against single domains: primID bit-exact on
+    >= 99.99% of rays, t within 1e-5 relative."""
+    org, dirs = eye_rays(512, 512)
+    for dom in (21, 42, 0):
+        t, u, v, p = ctx_c1.trace_rays(org, dirs, dom)
+        rt, ru, rv, rp = ref_c1.closest(dom, org, dirs)
+        agree = (p == rp)
+        print(f"domain {dom}: hits={int((rp >= 0).sum())} prim agreement {agree.mean():.6f}")
+        assert agree.mean() >= 0.9999
+        both = (p >= 0) & (rp >= 0) & agree
+        assert np.all(np.abs(t[both] - rt[both]) <= 1e-5 * np.abs(rt[both]))
+        # identical arithmetic: expected bit-exact
+        assert np.array_equal(t[both], rt[both]) and np.array_equal(u[both], ru[both])
+
+
+def test_closest_hit_random_rays(ctx_c1, ref_c1):
+    rng = np.random.default_rng(7)
+    n = 200_000
+    org = rng.uniform(-0.5, 4.5, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dirs = d.astype(np.float32)
+    for dom in (5, 63):
+        t, u, v, p = ctx_c1.trace_rays(org, dirs, dom)
+        rt, ru, rv, rp = ref_c1.closest(dom, org, dirs)
+        assert (p == rp).mean() >= 0.9999
+        both = (p >= 0) & (p == rp)
+        assert np.array_equal(t[both], rt[both])
+
+
+def test_any_hit(ctx_c1, ref_c1):
+    rng = np.random.default_rng(3)
+    n = 200_000
+    org = rng.uniform(0.0, 4.0, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    dirs = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    tfar = rng.uniform(0.01, 2.0, n).astype(np.float32)
+    for dom in (21, 22):
+        occ = ctx_c1.trace_rays(org, dirs, dom, mode=1, tfar=tfar)
+        rocc = ref_c1.any_hit(dom, org, dirs, tfar)
+        assert np.array_equal(occ, rocc)
+
+
+def test_domain_list(ctx_c1, ref_c1):
+    rng = np.random.default_rng(11)
+    n = 100_000
+    org = rng.uniform(-1.0, 5.0, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    dirs = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    cnt, ids, ts = ctx_c1.domain_list(org, dirs, maxk=16)
+    rc, rids, rts = ref_c1.domain_list(org, dirs, maxk=16)
+    assert np.array_equal(cnt, rc)
+    for i in range(n):
+        k = min(cnt[i], 16)
+        assert np.array_equal(ids[i, :k], rids[i, :k])
+        assert np.array_equal(ts[i, :k], rts[i, :k])
+
+
+def test_normals_bit_exact(ctx_c1, ref_c1):
+    for dom in (0, 21, 63):
+        n = ctx_c1.normals(dom)
+        rn = ref_c1.mesh(dom)[3]
+        assert np.array_equal(n, rn)
+
+
+def _img_close(a, b):
+    err = np.abs(a[..., :3] - b[..., :3])
+    return float(np.sqrt((err ** 2).mean())), float(err.max())
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_ao_frame_c1(ctx_c1, ref_c1, schedule):
+    """C1 AO frame (512x512, 1 spp, 8 AO rays, max dist 0.5) vs the oracle."""
+    ctx_c1.set_schedule(schedule)
+    img, st = ctx_c1.trace_frame()
+    rimg, rst = ref_c1.render(image_w=512, image_h=512, pixel_samples=1, ao_mode=1, ao_samples=8,
+                              bounces=1, ao_max_dist=0.5, nthreads=8, cam_pos=C1_CAM["pos"],
+                              cam_lookat=C1_CAM["lookat"], cam_up=C1_CAM["up"])
+    rmse, mx = _img_close(img, rimg)
+    print(f"schedule {schedule}: rmse {rmse:.3g} max {mx:.3g} useful {st['useful_rays']} "
+          f"oracle {rst['useful_rays']} trace_ms {st['trace_ms']:.2f}")
+    assert st["useful_rays"] == rst["useful_rays"]
+    assert st["final_hits"] == rst["final_hits"]
+    assert rmse <= 1e-4 and mx <= 1e-3
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_pt_frame_small(c1, schedule):
+    """Path tracing with the rect light, depth 5, RR from bounce 3, several
+    blocking tiles and two speculation windows, vs the oracle."""
+    import spray_amd as sa
+    kw = dict(image_w=128, image_h=128, pixel_samples=4, ao_mode=0, ao_samples=1, bounces=5,
+              rr_start=3, max_samples_per_rank=16384)
+    cfg = sa.default_config(has_camera=1, camera_pos=C1_CAM["pos"], camera_lookat=C1_CAM["lookat"],
+                            camera_up=C1_CAM["up"], nthreads=8, schedule=schedule, **kw)
+    with sa.Context(cfg) as ctx:
+        ctx.load_scene(os.path.join(c1, "c2.spray"), c1)
+        img, st = ctx.trace_frame()
+    ref = orc.Scene(os.path.join(c1, "c2.spray"), c1)
+    rimg, rst = ref.render(nthreads=8, cam_pos=C1_CAM["pos"], cam_lookat=C1_CAM["lookat"],
+                           cam_up=C1_CAM["up"], **kw)
+    rmse, mx = _img_close(img, rimg)
+    print(f"PT schedule {schedule}: rmse {rmse:.3g} max {mx:.3g} useful {st['useful_rays']} "
+          f"oracle {rst['useful_rays']}")
+    assert st["useful_rays"] == rst["useful_rays"]
+    assert rmse <= 1e-4 and mx <= 1e-3
