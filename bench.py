@@ -1,0 +1,212 @@
+"""Benchmark: SpRay per-rank tracing on one MI355X.
+
+Workload (the second configuration of the benchmark set): 64 domains in a
+4x4x4 grid of unit cubes, each a marching-cubes isosurface of
+f = sin(6x)sin(6y)sin(6z) + 0.3 sin(17(x+y+z)) on a 64^3 grid (~91k
+triangles per domain, ~5.8M total), path tracing with albedo 0.7 (vertex
+colour 179), one rectangular area light, 1024x1024, 8 spp, depth 5, Russian
+roulette from bounce 3, fp32 rays.  A step is one frame (8 blocking tiles of
+1,048,576 samples).  Synthetic geometry generated in-process.
+
+Prints one JSON line.  `value` is Mrays/s of useful ray queries (eye +
+non-stale bounce + non-stale shadow rays, identical for every schedule) over
+the wall time of the timed frames, with the scene resident in HBM.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+C2_CAM = dict(pos=[2.0, 2.0, 2.0 + 2.0 * math.sqrt(3.0)], lookat=[2.0, 2.0, 2.0], up=[0.0, 1.0, 0.0])
+RECT_LIGHT = "light rect 0.5 4.5 0.5 3 0 0 0 0 3 4 4 4"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--spp", type=int, default=8)
+    ap.add_argument("--schedules", default="plain,batched,learned")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--cpu-tiles", type=int, default=8, help="blocking tiles timed on the CPU")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default batch parameters")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every pass directly (kernel-level profilers)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (rocprof counter calibration)")
+    return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    a = torch.ones(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "launches": 3}))
+
+
+def algorithmic_bytes(st):
+    """Bytes one frame's trace kernels must move: per (ray, domain) visit a
+    32 B ray record, 16 B hit record and 8 B queue entry; per BVH node fetch
+    128 B; per triangle test 48 B; per accepted shading vertex 80 B ray write
+    + 16 B queue + 16 B history."""
+    visits = st["radiance_visits"] + st["shadow_visits"]
+    return 56 * visits + 128 * st["node_visits"] + 48 * st["tri_tests"] + 112 * st["shades"]
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU image-stripe "
+              "sharding is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    import numpy as np
+    import spray_amd as sa
+
+    tmp = tempfile.mkdtemp(prefix="spray_c2_")
+    t0 = time.time()
+    tris = sa.scenegen_write(tmp, "c2", 4, 4, 4, 64, rgb=0xB3B3B3, extra_lines=RECT_LIGHT)
+    gen_s = time.time() - t0
+    W = H = args.size
+    kw = dict(image_w=W, image_h=H, pixel_samples=args.spp, ao_mode=0, ao_samples=1, bounces=5,
+              rr_start=3)
+    cfg = sa.default_config(has_camera=1, camera_pos=C2_CAM["pos"], camera_lookat=C2_CAM["lookat"],
+                            camera_up=C2_CAM["up"], nthreads=min(16, os.cpu_count() or 1), **kw)
+    ctx = sa.Context(cfg)
+    t0 = time.time()
+    ctx.load_scene(os.path.join(tmp, "c2.spray"), tmp)
+    load_s = time.time() - t0
+
+    schedules = {"plain": 0, "batched": 1, "learned": 2}
+    wanted = [s for s in args.schedules.split(",") if s]
+    results = {}
+    images = {}
+    for name in wanted:
+        sched = schedules[name]
+        try:
+            ctx.set_schedule(sched)
+            if name == "learned":
+                # train the predictor on hit logs of plain-schedule frames
+                ctx.set_schedule(0)
+                ctx.predictor_log(True, 50_000_000)
+                ctx.trace_frame(want_image=False)
+                ctx.predictor_log(False)
+                ctx.predictor_train(2, 4096, 2e-3)
+                ctx.set_schedule(sched)
+            if name != "plain" and not args.no_autotune:
+                ctx.autotune()
+            ctx.profile_kernels(args.no_graph)
+        except sa.SprayError as e:
+            results[name] = {"error": str(e)}
+            continue
+        for _ in range(args.warmup):
+            ctx.trace_frame(want_image=False)
+        frame_ms, walls = [], []
+        st = None
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            tw = time.perf_counter()
+            img, st = ctx.trace_frame(want_image=False)
+            walls.append(time.perf_counter() - tw)
+            frame_ms.append(st["trace_ms"])
+        total_s = time.perf_counter() - t_start
+        img, st_img = ctx.trace_frame(want_image=True)
+        images[name] = img
+        # kernel timing of the dominant (trace) kernel, on the tracer's stream
+        ctx.profile_kernels(True)
+        ctx.kernel_time(reset=True)
+        _, st_prof = ctx.trace_frame(want_image=False)
+        launches, kern_ms = ctx.kernel_time(reset=True)
+        ctx.profile_kernels(args.no_graph)
+        results[name] = dict(
+            mrays_s=st["useful_rays"] * args.steps / total_s / 1e6,
+            ms_per_frame=1000.0 * total_s / args.steps,
+            device_ms_median=statistics.median(frame_ms),
+            useful_rays=st["useful_rays"], radiance_visits=st["radiance_visits"],
+            shadow_visits=st["shadow_visits"], shades=st["shades"], final_hits=st["final_hits"],
+            queue_entries=st["queue_entries"], pruned=st["pruned_entries"],
+            deferred=st["deferred_entries"], passes=st["passes"],
+            node_visits=st["node_visits"], tri_tests=st["tri_tests"],
+            trace_kernel_launches=launches, trace_kernel_ms=kern_ms,
+            trace_kernel_bytes=algorithmic_bytes(st_prof),
+        )
+    ref_name = "plain" if "plain" in images else next(iter(images), None)
+    for name, img in images.items():
+        if ref_name and name != ref_name:
+            diff = np.abs(img[..., :3] - images[ref_name][..., :3])
+            results[name]["image_rmse_vs_plain"] = float(np.sqrt((diff ** 2).mean()))
+            results[name]["image_max_vs_plain"] = float(diff.max())
+
+    ok = {k: v for k, v in results.items() if "mrays_s" in v}
+    best = max(ok, key=lambda k: ok[k]["mrays_s"]) if ok else None
+    # roofline of the dominant kernel (the trace kernel) of the reported schedule
+    roof = None
+    if best:
+        r = ok[best]
+        avg_ms = r["trace_kernel_ms"] / max(1, r["trace_kernel_launches"])
+        bytes_per_launch = r["trace_kernel_bytes"] / max(1, r["trace_kernel_launches"])
+        achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "kTracePersistent" if best != "plain" else "kTraceDomain",
+                "avg_launch_ms": round(avg_ms, 4), "launches_per_frame": r["trace_kernel_launches"],
+                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(tmp, kw, args.cpu_tiles)
+
+    line = {
+        "metric": "Mrays/s (AO + path) on 1 MI355X; learned vs plain schedule at equal image",
+        "value": round(ok[best]["mrays_s"], 3) if best else None,
+        "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ok[best]["ms_per_frame"], 3) if best else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"C2 path tracing: 64 domains (4x4x4 MC isosurfaces, {tris} tris), "
+                               f"{W}x{H}, {args.spp} spp, depth 5, RR>=3, rect light",
+                   "schedule": best, "triangles": tris, "scene_gen_s": round(gen_s, 2),
+                   "load_build_s": round(load_s, 2)},
+        "schedules": results,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+def cpu_baseline(scene_dir, kw, ntiles):
+    """The oracle (the CPU restatement of the reference's OpenMP tracer) on
+    the first `ntiles` blocking tiles of the same frame, on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    threads = min(16, os.cpu_count() or 1)
+    ref = orc.Scene(os.path.join(scene_dir, "c2.spray"), scene_dir, nthreads=threads)
+    _, st = ref.render(nthreads=threads, cam_pos=C2_CAM["pos"], cam_lookat=C2_CAM["lookat"],
+                       cam_up=C2_CAM["up"], tile_begin=0, tile_end=ntiles, **kw)
+    return {"value": round(st["useful_rays"] / st["seconds"] / 1e6, 4), "unit": "Mrays/s",
+            "cores": threads, "kind": "port",
+            "sample": f"first {ntiles} of {orc.num_tiles(**kw)} blocking tiles of the same frame "
+                      f"({st['useful_rays']} useful rays, {st['seconds']:.1f} s, speculative "
+                      f"OpenMP restatement with its own SAH BVH)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -154,6 +154,14 @@ int spray_set_schedule(spray_ctx* ctx, int schedule);
 int spray_autotune(spray_ctx* ctx, int* chunk_out, int* tile_out);
 int spray_set_batch_params(spray_ctx* ctx, int chunk, int tile);
 
+/* Kernel timing (new): when enabled, every trace-kernel launch (the
+ * per-domain kernel of the plain form, the persistent kernel of the batched
+ * forms) is bracketed by hipEvents on the context's stream (the batched
+ * form then runs without its graph).  spray_kernel_time returns the launch
+ * count and summed device time since the last reset. */
+int spray_profile_kernels(spray_ctx* ctx, int enable);
+int spray_kernel_time(spray_ctx* ctx, long long* launches, double* total_ms, int reset);
+
 /* Learned form (new): domain-hit predictor.  Logging collects
  * (origin, direction, domain, hit) samples from plain-schedule frames;
  * training runs the MLP's own forward/backward/Adam kernels on the GPU. */

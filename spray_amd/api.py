@@ -161,6 +161,14 @@ class Context:
         self._check(self.lib.spray_domain_list(self.h, _fp(org), _fp(dirs), n, maxk, _ip(cnt), _ip(ids), _fp(ts)))
         return cnt, ids, ts
 
+    def profile_kernels(self, enable):
+        self._check(self.lib.spray_profile_kernels(self.h, int(enable)))
+
+    def kernel_time(self, reset=True):
+        n, ms = C.c_longlong(), C.c_double()
+        self._check(self.lib.spray_kernel_time(self.h, C.byref(n), C.byref(ms), int(reset)))
+        return n.value, ms.value
+
     # learned schedule
     def predictor_log(self, enable, max_samples=0):
         self._check(self.lib.spray_predictor_log(self.h, int(enable), int(max_samples)))

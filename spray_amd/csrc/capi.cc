@@ -392,6 +392,18 @@ int spray_set_batch_params(spray_ctx* ctx, int chunk, int tile) {
   });
 }
 
+int spray_profile_kernels(spray_ctx* ctx, int enable) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  ctx->c.profile = enable != 0;
+  return SPRAY_OK;
+}
+
+int spray_kernel_time(spray_ctx* ctx, long long* launches, double* total_ms, int reset) {
+  if (!ctx || !launches || !total_ms) return SPRAY_ERR_INVALID;
+  tracerKernelTime(&ctx->c, launches, total_ms, reset);
+  return SPRAY_OK;
+}
+
 int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const char* name,
                          const char* extra_lines, long long* triangles_out) {
   if (!p || !dir || !name) return SPRAY_ERR_INVALID;

@@ -99,6 +99,7 @@ struct Ctx {
   int batch_chunk = 1024;
   int batch_tile = 4;      // waves per block of the persistent kernel (block = 64 * tile)
   int param_version = 0;   // bumped when camera / config change (graph re-capture)
+  bool profile = false;    // time every trace-kernel launch with hipEvents
   std::string err;
   // scratch for spray_trace_rays / spray_domain_list
   DevBuf q_in, q_out;
@@ -115,6 +116,7 @@ void domainListDevice(Ctx* c, const float* d_org, const float* d_dir, size_t n, 
 void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st);
 void tracerRelease(Ctx* c);
 void tracerAutotune(Ctx* c, int* chunk, int* tile);
+void tracerKernelTime(Ctx* c, long long* launches, double* ms, int reset);
 // mlp.hip
 void predictorRelease(Ctx* c);
 

@@ -1,5 +1,6 @@
 // Domain-hit predictor (learned schedule): placeholder until the MLP kernels land.
 #include "ctx.h"
+#include "tracer_dev.h"
 
 namespace spray {
 struct Predictor {};
@@ -7,6 +8,12 @@ void predictorRelease(Ctx* c) {
   delete c->pred;
   c->pred = nullptr;
 }
+__global__ void kPhaseNone(TraceParams P) {
+  uint32_t n = P.ctr->q_total[0];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    P.phase_b[i] = 0;
+}
+void predictorPhase(Ctx* c, const TraceParams& P) { kPhaseNone<<<1024, 256, 0, c->stream>>>(P); }
 }  // namespace spray
 
 using namespace spray;

@@ -135,11 +135,8 @@ __global__ void kPassBegin(TraceParams P) {
   c->s_mark = c->s_end;
   c->active = (c->r_end > c->r_beg || c->s_end > c->s_beg) ? 1u : 0u;
   if (c->active) c->passes++;
-  c->item_ctr[0] = 0;
-  c->item_ctr[1] = 0;
-  c->n_items[0] = 0;
-  c->n_items[1] = 0;
   c->vlog_count = 0;
+  c->shade_count = 0;
 }
 
 __global__ void kClearQ(TraceParams P) {
@@ -151,7 +148,9 @@ __global__ void kClearQ(TraceParams P) {
     P.cursor[i] = 0;
     P.cursor[nd + i] = 0;
     P.score[i] = 0;
+    P.wctr[i] = 0;
   }
+  if (blockIdx.x == 0 && threadIdx.x < 2) P.wctr[nd + threadIdx.x] = 0;
 }
 
 // Ray of the current pass: index into the radiance range, then the shadow range.
@@ -252,17 +251,38 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   }
 }
 
-// Offsets, schedule (DomainStats::schedule: descending score, ties by id)
-// and the batched work list, in one block.
+// Schedule (DomainStats::schedule, ooc_domain_stats.cc:60-111: descending
+// score, ties by id) and queue offsets laid out in schedule order, in one block.
+__device__ __forceinline__ uint32_t blockScanExcl(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  uint32_t x = v;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      uint32_t t = wsum[i];
+      wsum[i] = acc;
+      acc += t;
+    }
+    wsum[16] = acc;
+  }
+  __syncthreads();
+  uint32_t r = wsum[w] + x - v;
+  *total = wsum[16];
+  __syncthreads();
+  return r;
+}
+
 __global__ void __launch_bounds__(1024) kSchedule(TraceParams P) {
   DevCounters* c = P.ctr;
   if (!c->active) return;
   const int nd = P.ndomains;
-  __shared__ uint32_t carry[3];
   __shared__ uint32_t wsum[16 + 1];
-  if (threadIdx.x == 0) carry[0] = carry[1] = carry[2] = 0;
-  __syncthreads();
-  // rank by (score desc, id asc)
   for (int d = threadIdx.x; d < nd; d += blockDim.x) {
     unsigned long long sd = P.score[d];
     int rank = 0;
@@ -273,90 +293,28 @@ __global__ void __launch_bounds__(1024) kSchedule(TraceParams P) {
     P.order[rank] = d;
   }
   __syncthreads();
-  // exclusive offsets per kind in domain-id order
   for (int kind = 0; kind < 2; ++kind) {
+    uint32_t carry = 0;
     for (int base = 0; base < nd; base += blockDim.x) {
-      int d = base + threadIdx.x;
-      uint32_t v = d < nd ? P.cnt[kind * nd + d] : 0u;
-      // block exclusive scan (wave shuffles + LDS)
-      uint32_t x = v;
-      int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
+      int k = base + threadIdx.x;
+      int d = k < nd ? P.order[k] : 0;
+      uint32_t v = k < nd ? P.cnt[kind * nd + d] : 0u;
+      uint32_t total;
+      uint32_t pre = blockScanExcl(v, wsum, &total);
+      if (k < nd) {
+        P.off[kind * (nd + 1) + k] = carry + pre;
+        P.dbase[kind * nd + d] = carry + pre;
       }
-      if (lane == 63) wsum[w] = x;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
-          uint32_t t = wsum[i];
-          wsum[i] = acc;
-          acc += t;
-        }
-        wsum[16] = acc;
-      }
-      __syncthreads();
-      if (d < nd) P.off[kind * (nd + 1) + d] = carry[kind] + wsum[w] + x - v;
-      __syncthreads();
-      if (threadIdx.x == 0) carry[kind] += wsum[16];
-      __syncthreads();
+      carry += total;
     }
     if (threadIdx.x == 0) {
-      P.off[kind * (nd + 1) + nd] = carry[kind];
-      c->q_total[kind] = carry[kind];
-      c->queue_entries += carry[kind];
-      if (carry[kind] > P.q_cap[kind]) atomicOr(&c->err, 1u);
+      P.off[kind * (nd + 1) + nd] = carry;
+      c->q_total[kind] = min(carry, P.q_cap[kind]);
+      c->queue_entries += carry;
+      if (carry > P.q_cap[kind]) atomicOr(&c->err, 1u);
     }
     __syncthreads();
   }
-  // work items in schedule order: per domain, radiance chunks then shadow chunks
-  for (int base = 0; base < nd; base += blockDim.x) {
-    int k = base + threadIdx.x;
-    int d = k < nd ? P.order[k] : 0;
-    uint32_t nr = k < nd ? (P.cnt[d] + P.chunk - 1) / P.chunk : 0u;
-    uint32_t ns = k < nd ? (P.cnt[nd + d] + P.chunk - 1) / P.chunk : 0u;
-    uint32_t v = nr + ns;
-    uint32_t x = v;
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int o = 1; o < 64; o <<= 1) {
-      uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t acc = 0;
-      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
-        uint32_t t = wsum[i];
-        wsum[i] = acc;
-        acc += t;
-      }
-      wsum[16] = acc;
-    }
-    __syncthreads();
-    uint32_t pos = carry[2] + wsum[w] + x - v;
-    if (k < nd && pos + v <= P.items_cap) {
-      uint32_t ob = P.off[d];
-      for (uint32_t j = 0; j < nr; ++j) {
-        uint32_t b = j * P.chunk;
-        uint32_t e = min(b + (uint32_t)P.chunk, P.cnt[d]);
-        P.items[pos++] = WorkItem{d, 0, ob + b, ob + e};
-      }
-      uint32_t os = P.off[(nd + 1) + d];
-      for (uint32_t j = 0; j < ns; ++j) {
-        uint32_t b = j * P.chunk;
-        uint32_t e = min(b + (uint32_t)P.chunk, P.cnt[nd + d]);
-        P.items[pos++] = WorkItem{d, 1, os + b, os + e};
-      }
-    } else if (k < nd && v) {
-      atomicOr(&c->err, 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) carry[2] += wsum[16];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) c->n_items[0] = min(carry[2], P.items_cap);
 }
 
 // Scatter the queue entries (block-aggregated slot reservation).
@@ -419,12 +377,12 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
         float tmin;
         if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
         uint32_t slot = lb[kind * nd + dd] + atomicAdd(&lc[kind * nd + dd], 1u);
-        uint32_t dst = P.off[kind * (nd + 1) + dd] + slot;
+        uint32_t dst = P.dbase[kind * nd + dd] + slot;
         if (dst < P.q_cap[kind]) P.q[kind][dst] = QEntry{r, tmin};
       }
       if (kind == 1 && spawn >= 0 && !spawn_listed) {
         uint32_t slot = lb[nd + spawn] + atomicAdd(&lc[nd + spawn], 1u);
-        uint32_t dst = P.off[(nd + 1) + spawn] + slot;
+        uint32_t dst = P.dbase[nd + spawn] + slot;
         // the spawn domain always tests its own shadow rays (procShads2)
         if (dst < P.q_cap[1]) P.q[1][dst] = QEntry{r, -kInfF};
       }
@@ -435,10 +393,6 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
 
 // ---------------------------------------------------------------------------
 // Shading (ooc_shader_ao.h:81-197, ooc_shader_pt.h:93-227) and spawning.
-struct LaneCounts {
-  unsigned long long rvis = 0, svis = 0, shades = 0, pruned = 0, nodes = 0, tris = 0;
-};
-
 __device__ __forceinline__ void writeRay(const RayPool& pool, uint32_t idx, f3 org, int pixid,
                                          f3 dir, int samid, f3 w, int depth_word,
                                          const float h[4]) {
@@ -667,157 +621,204 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
   }
 }
 
-// One radiance queue entry (filterRqs + procRads, ooc_tcontext.inl:28-49,137-150)
-__device__ __forceinline__ void radianceEntry(const TraceParams& P, int dom, bool valid,
-                                              QEntry e, LaneStack& st, LaneCounts& lc,
-                                              int ray_depth) {
-  const DomainDesc& D = P.desc[dom];
-  float4 op = make_float4(0, 0, 0, 0), ds = make_float4(1, 0, 0, 0), wd = make_float4(0, 0, 0, 0);
-  float hist[4] = {kInfF, kInfF, kInfF, kInfF};
-  int depth = 0;
-  bool go = false;
-  if (valid) {
-    op = P.R.op[e.ray];
-    ds = P.R.ds[e.ray];
-    wd = P.R.wd[e.ray];
-    float4 hh = P.R.h[e.ray];
-    hist[0] = hh.x;
-    hist[1] = hh.y;
-    hist[2] = hh.z;
-    hist[3] = hh.w;
-    depth = __float_as_int(wd.w) & 0xff;
-    go = (e.tdom <= hist[depth]) &&
-         vbufCorrect(P.vbuf, __float_as_int(ds.w), hist, depth);
-    if (!go) lc.pruned++;
+// ---------------------------------------------------------------------------
+// Trace step of a pass.  Every lane owns one queue entry at a time:
+//   radiance (filterRqs + procRads, ooc_tcontext.inl:28-49,137-150): keep the
+//     entry if tdom <= history[depth] and the ray's history is still the
+//     buffer's chain; closest hit; history-buffer update; accepted hits
+//     become shade records;
+//   shadow (filterSqs + procShads, ooc_tcontext.inl:51-70,152-168): keep if
+//     not yet occluded and still on the chain; any hit sets the flag.
+// A wave refetches entries for its idle lanes with one atomic, so long
+// traversals do not hold up the rest of the wave.
+struct LaneCounts {
+  unsigned long long rvis = 0, svis = 0, shades = 0, pruned = 0, nodes = 0, tris = 0;
+};
+
+constexpr int kTravBlock = 256;
+
+__device__ __forceinline__ uint32_t findDomain(const TraceParams& P, int kind, uint32_t q) {
+  // largest k with off[k] <= q, over the schedule-ordered offsets
+  const uint32_t* off = P.off + kind * (P.ndomains + 1);
+  int lo = 0, hi = P.ndomains - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= q)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
-  float t = kInfF, u = 0, v = 0;
-  int prim = -1;
-  if (go) {
-    float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
-    TravCount tc;
-    closestHit<true>(P.nodes + D.node_off, P.tris + D.tri_off, o, d, kRayEps, kInfF,
-                     (uint32_t)D.num_nodes, st, &t, &u, &v, &prim, &tc);
-    lc.rvis++;
-    lc.nodes += tc.nodes;
-    lc.tris += tc.tris;
-  }
-  bool upd = go && prim >= 0 && vbufUpdate(P.vbuf, __float_as_int(ds.w), hist, depth, t);
-  if (upd) {
-    atomicMin((uint32_t*)&P.R.h[e.ray] + depth, __float_as_uint(t));
-    lc.shades++;
-  }
-  if (go && P.log_on) {  // visit log for the predictor: (ray, domain, t)
-    uint32_t i = atomicAdd(&P.ctr->vlog_count, 1u);
-    if (i < P.vlog_cap) {
-      P.vlog[(size_t)i * 3] = e.ray;
-      P.vlog[(size_t)i * 3 + 1] = (uint32_t)dom;
-      P.vlog[(size_t)i * 3 + 2] = __float_as_uint(prim >= 0 ? t : kInfF);
-    }
-  }
-  // all lanes take part in the spawn ballots
-  shadeSpawn(P, upd, dom, op, ds, wd, hist, depth, t, u, v, prim, ray_depth);
+  return (uint32_t)P.order[lo];
 }
 
-// One shadow queue entry (filterSqs + procShads, ooc_tcontext.inl:51-70,152-168)
-__device__ __forceinline__ void shadowEntry(const TraceParams& P, int dom, bool valid, QEntry e,
-                                            LaneStack& st, LaneCounts& lc) {
-  if (!valid) return;
-  const DomainDesc& D = P.desc[dom];
-  if (__hip_atomic_load(&P.Sx.occluded[e.ray], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-    lc.pruned++;
-    return;
-  }
-  float4 op = P.S.op[e.ray], ds = P.S.ds[e.ray], wd = P.S.wd[e.ray], hh = P.S.h[e.ray];
-  float hist[4] = {hh.x, hh.y, hh.z, hh.w};
-  int depth = __float_as_int(wd.w) & 0xff;
-  if (!vbufCorrect(P.vbuf, __float_as_int(ds.w), hist, depth)) {
-    lc.pruned++;
-    return;
-  }
-  float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
-  TravCount tc;
-  bool occ = anyHit<true>(P.nodes + D.node_off, P.tris + D.tri_off, o, d, kRayEps,
-                          P.Sx.tmax[e.ray], (uint32_t)D.num_nodes, st, &tc);
-  lc.svis++;
-  lc.nodes += tc.nodes;
-  lc.tris += tc.tris;
-  if (occ)
-    __hip_atomic_store(&P.Sx.occluded[e.ray], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void flushLane(const TraceParams& P, const LaneCounts& lc,
-                                          const LaneStack& st) {
-  DevCounters* c = P.ctr;
-  flushCount(&c->radiance_visits, lc.rvis);
-  flushCount(&c->shadow_visits, lc.svis);
-  flushCount(&c->shades, lc.shades);
-  flushCount(&c->pruned, lc.pruned);
-  flushCount(&c->node_visits, lc.nodes);
-  flushCount(&c->tri_tests, lc.tris);
-  if (st.overflowed) atomicOr(&c->err, 4u);
-}
-
-// Plain form: one launch per scheduled domain, both queues of the domain.
-__global__ void __launch_bounds__(kBlock) kTraceDomain(TraceParams P, int dom, uint32_t nrb) {
-  __shared__ uint32_t ring[kRing * kBlock];
-  LaneStack st;
-  long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  st.init(ring, P.ovf + gtid * kOverflow);
-  LaneCounts lc;
-  const int nd = P.ndomains;
-  const int ray_depth = P.tile_rec[2];
-  if (blockIdx.x < nrb) {
-    uint32_t b = P.off[dom], e = min(P.off[dom + 1], P.q_cap[0]);
-    uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < e;
-    QEntry q = valid ? P.q[0][i] : QEntry{0, 0.f};
-    radianceEntry(P, dom, valid, q, st, lc, ray_depth);
-  } else {
-    uint32_t b = P.off[(nd + 1) + dom], e = min(P.off[(nd + 1) + dom + 1], P.q_cap[1]);
-    uint32_t i = b + (blockIdx.x - nrb) * blockDim.x + threadIdx.x;
-    bool valid = i < e;
-    QEntry q = valid ? P.q[1][i] : QEntry{0, 0.f};
-    shadowEntry(P, dom, valid, q, st, lc);
-  }
-  flushLane(P, lc, st);
-}
-
-// Batched form: persistent blocks pull (domain, chunk) work items.
-__global__ void __launch_bounds__(256) kTracePersistent(TraceParams P, int phase) {
+__global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain_dom, int phase) {
   extern __shared__ uint32_t ring[];
-  __shared__ uint32_t item_sh;
   DevCounters* c = P.ctr;
   if (!c->active) return;
+  const int nd = P.ndomains;
+  uint32_t nr, ns, rbase, sbase;
+  uint32_t* wc;
+  if (plain_dom >= 0) {
+    rbase = P.dbase[plain_dom];
+    sbase = P.dbase[nd + plain_dom];
+    nr = P.cnt[plain_dom];
+    ns = P.cnt[nd + plain_dom];
+    wc = &P.wctr[plain_dom];
+  } else {
+    rbase = 0;
+    sbase = 0;
+    nr = c->q_total[0];
+    ns = phase == 0 ? c->q_total[1] : 0u;
+    wc = &P.wctr[nd + phase];
+  }
+  // bounded by the queue capacities even after an overflow
+  nr = rbase < P.q_cap[0] ? min(nr, P.q_cap[0] - rbase) : 0u;
+  ns = sbase < P.q_cap[1] ? min(ns, P.q_cap[1] - sbase) : 0u;
+  const uint32_t total = nr + ns;
   LaneStack st;
   long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   st.init(ring, P.ovf + gtid * kOverflow);
   LaneCounts lc;
-  const uint32_t nitems = c->n_items[0];
-  const int ray_depth = P.tile_rec[2];
-  const WorkItem* items = P.items;
+  TravCount tc;
+  TravState s;
+  const int lane = __lane_id();
+  bool active = false, exhausted = false;
+  int kind = 0, dom = 0, depth = 0, samid = 0;
+  uint32_t ray = 0;
+  float hist[4] = {kInfF, kInfF, kInfF, kInfF};
   while (true) {
-    if (threadIdx.x == 0) item_sh = atomicAdd(&c->item_ctr[phase], 1u);
-    __syncthreads();
-    uint32_t it = item_sh;
-    __syncthreads();
-    if (it >= nitems) break;
-    WorkItem w = items[it];
-    if (phase == 1 && w.kind == 1) continue;  // deferred pass revisits radiance entries only
-    const uint32_t wend = min(w.end, P.q_cap[w.kind]);
-    for (uint32_t base = w.begin; base < wend; base += blockDim.x) {
-      uint32_t i = base + threadIdx.x;
-      bool valid = i < wend;
-      QEntry q = valid ? P.q[w.kind][i] : QEntry{0, 0.f};
-      if (w.kind == 0) {
-        if (phase == 0 && P.learned && valid && P.phase_b[i]) valid = false;  // deferred
-        if (phase == 1 && valid && !P.phase_b[i]) valid = false;
-        radianceEntry(P, w.domain, valid, q, st, lc, ray_depth);
-      } else {
-        shadowEntry(P, w.domain, valid, q, st, lc);
+    unsigned long long idle = __ballot(!active);
+    int nidle = __popcll(idle);
+    if (!exhausted && (nidle >= P.refill || nidle == 64)) {
+      int leader = __builtin_ctzll(idle);
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(wc, (uint32_t)nidle);
+      base = __shfl(base, leader, 64);
+      if (base + (uint32_t)nidle >= total) exhausted = true;
+      if (!active) {
+        uint32_t e = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (e < total) {
+          kind = e < nr ? 0 : 1;
+          uint32_t q = kind == 0 ? rbase + e : sbase + (e - nr);
+          QEntry qe = P.q[kind][q];
+          dom = plain_dom >= 0 ? plain_dom : (int)findDomain(P, kind, q);
+          ray = qe.ray;
+          const RayPool& pool = kind ? P.S : P.R;
+          float4 op = pool.op[ray], ds = pool.ds[ray], wd = pool.wd[ray], hh = pool.h[ray];
+          hist[0] = hh.x;
+          hist[1] = hh.y;
+          hist[2] = hh.z;
+          hist[3] = hh.w;
+          depth = __float_as_int(wd.w) & 0xff;
+          samid = __float_as_int(ds.w);
+          bool go;
+          float tfar = kInfF;
+          if (kind == 0) {
+            bool deferred = P.learned && P.phase_b[q];
+            go = (phase == 0 ? !deferred : deferred) && qe.tdom <= pick4(hist, depth) &&
+                 vbufCorrect(P.vbuf, samid, hist, depth);
+          } else {
+            go = !__hip_atomic_load(&P.Sx.occluded[ray], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) &&
+                 vbufCorrect(P.vbuf, samid, hist, depth);
+            tfar = P.Sx.tmax[ray];
+          }
+          if (go) {
+            const DomainDesc& D = P.desc[dom];
+            float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
+            travInit(s, P.nodes + D.node_off, P.tris + D.tri_off, (uint32_t)D.num_nodes, o, d,
+                     kRayEps, tfar, kind == 1, st);
+            active = true;
+            if (kind == 0)
+              lc.rvis++;
+            else
+              lc.svis++;
+          } else {
+            lc.pruned++;
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0ull) {
+      if (exhausted) break;
+      continue;
+    }
+    if (active && travStep(s, st, tc)) {
+      active = false;
+      if (kind == 0) {
+        bool upd = s.prim >= 0 && vbufUpdate(P.vbuf, samid, hist, depth, s.tlim);
+        if (P.log_on) {  // visit log for the predictor: (ray, domain, t)
+          uint32_t i = atomicAdd(&c->vlog_count, 1u);
+          if (i < P.vlog_cap) {
+            P.vlog[(size_t)i * 3] = ray;
+            P.vlog[(size_t)i * 3 + 1] = (uint32_t)dom;
+            P.vlog[(size_t)i * 3 + 2] = __float_as_uint(s.prim >= 0 ? s.tlim : kInfF);
+          }
+        }
+        if (upd) atomicMin((uint32_t*)&P.R.h[ray] + depth, __float_as_uint(s.tlim));
+        uint32_t idx = waveAppend(upd, &c->shade_count, P.sh_cap, &c->err);
+        if (idx != 0xffffffffu) {
+          P.sh_ray[idx] = ray;
+          P.sh_dom[idx] = (uint32_t)dom;
+          P.sh_tuv[(size_t)idx * 3] = s.tlim;
+          P.sh_tuv[(size_t)idx * 3 + 1] = s.u;
+          P.sh_tuv[(size_t)idx * 3 + 2] = s.v;
+          P.sh_prim[idx] = s.prim;
+        }
+      } else if (s.prim > 0) {
+        __hip_atomic_store(&P.Sx.occluded[ray], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }
-  flushLane(P, lc, st);
+  flushCount(&c->radiance_visits, lc.rvis);
+  flushCount(&c->shadow_visits, lc.svis);
+  flushCount(&c->pruned, lc.pruned);
+  flushCount(&c->node_visits, tc.nodes);
+  flushCount(&c->tri_tests, tc.tris);
+  if (st.overflowed) atomicOr(&c->err, 4u);
+}
+
+// Shading after the trace step: every hit whose history entry is still the
+// buffer's final entry for this pass is shaded once (stale speculative hits
+// would only spawn rays that can never retire, so they are skipped).
+__global__ void __launch_bounds__(kBlock) kShade(TraceParams P) {
+  DevCounters* c = P.ctr;
+  if (!c->active) return;
+  const uint32_t n = min(c->shade_count, P.sh_cap);
+  const int ray_depth = P.tile_rec[2];
+  unsigned long long shades = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool valid = i < n;
+    uint32_t ray = valid ? P.sh_ray[i] : 0u;
+    int dom = valid ? (int)P.sh_dom[i] : 0;
+    float t = valid ? P.sh_tuv[(size_t)i * 3] : kInfF;
+    float u = valid ? P.sh_tuv[(size_t)i * 3 + 1] : 0.f;
+    float v = valid ? P.sh_tuv[(size_t)i * 3 + 2] : 0.f;
+    int prim = valid ? P.sh_prim[i] : -1;
+    float4 op = make_float4(0, 0, 0, 0), ds = make_float4(1, 0, 0, 0), wd = make_float4(0, 0, 0, 0);
+    float hist[4] = {kInfF, kInfF, kInfF, kInfF};
+    int depth = 0;
+    bool act = false;
+    if (valid) {
+      op = P.R.op[ray];
+      ds = P.R.ds[ray];
+      wd = P.R.wd[ray];
+      float4 hh = P.R.h[ray];
+      hist[0] = hh.x;
+      hist[1] = hh.y;
+      hist[2] = hh.z;
+      hist[3] = hh.w;
+      depth = __float_as_int(wd.w) & 0xff;
+      int samid = __float_as_int(ds.w);
+      uint32_t parent = depth > 0 ? __float_as_uint(pick4(hist, depth - 1)) : 0u;
+      act = vbufCorrect(P.vbuf, samid, hist, depth) &&
+            ldW(P.vbuf + (size_t)samid * kHistory + depth) == mkW(t, parent);
+    }
+    shades += act ? 1 : 0;
+    shadeSpawn(P, act, dom, op, ds, wd, hist, depth, t, u, v, prim, ray_depth);
+  }
+  flushCount(&c->shades, shades);
 }
 
 // Retire (ooc_tcontext.inl:123-135): shadow rays tested in this pass.
@@ -913,10 +914,10 @@ __global__ void kLogFinalize(TraceParams P) {
 // Host side
 struct TracerState {
   long long S = 0;  // samples of the largest tile
-  uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, items_cap = 0;
+  uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
-  DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, cursor, score,
-      order, q[2], items, ovf, phase_b, vlog, log;
+  DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, cursor,
+      wctr, score, order, q[2], ovf, phase_b, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim;
   unsigned long long log_cap = 0;
   bool logging = false;
   int grid_persistent = 0;
@@ -924,6 +925,33 @@ struct TracerState {
   int graph_key[6] = {0, 0, 0, 0, 0, 0};
   std::vector<uint32_t> h_cnt;
   std::vector<int> h_order;
+  // kernel timing of the trace launches (hipEvents on the tracer's stream)
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  double prof_ms = 0;
+  long long prof_launches = 0;
+  int evBegin(hipStream_t s) {
+    if (ev_used + 2 > ev.size()) {
+      for (int k = 0; k < 256; ++k) {
+        hipEvent_t e;
+        SPRAY_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+      }
+    }
+    SPRAY_HIP(hipEventRecord(ev[ev_used], s));
+    ev_used += 2;
+    return (int)ev_used - 2;
+  }
+  void evEnd(int i, hipStream_t s) { SPRAY_HIP(hipEventRecord(ev[i + 1], s)); }
+  void evCollect() {
+    for (size_t i = 0; i + 1 < ev_used; i += 2) {
+      float ms = 0;
+      SPRAY_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      prof_ms += ms;
+      prof_launches++;
+    }
+    ev_used = 0;
+  }
   void releaseGraph() {
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
@@ -934,11 +962,13 @@ void tracerRelease(Ctx* c) {
   if (!c->tracer) return;
   TracerState* T = c->tracer;
   T->releaseGraph();
+  for (hipEvent_t e : T->ev) (void)hipEventDestroy(e);
   for (auto* arr : {T->R, T->Pn, T->Sh})
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
-                    &T->off, &T->cursor, &T->score, &T->order, &T->q[0], &T->q[1], &T->items,
-                    &T->ovf, &T->phase_b, &T->vlog, &T->log})
+                    &T->off, &T->dbase, &T->cursor, &T->wctr, &T->score, &T->order, &T->q[0],
+                    &T->q[1], &T->ovf, &T->phase_b, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
+                    &T->sh_tuv, &T->sh_prim})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1008,7 +1038,13 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->tile_rec, 64);
   need(T, T->cnt, (size_t)2 * nd * 4 + 64);
   need(T, T->off, (size_t)2 * (nd + 1) * 4 + 64);
+  need(T, T->dbase, (size_t)2 * nd * 4 + 64);
   need(T, T->cursor, (size_t)2 * nd * 4 + 64);
+  need(T, T->wctr, (size_t)(nd + 2) * 4 + 64);
+  need(T, T->sh_ray, (size_t)T->q_cap[0] * 4);
+  need(T, T->sh_dom, (size_t)T->q_cap[0] * 4);
+  need(T, T->sh_tuv, (size_t)T->q_cap[0] * 12);
+  need(T, T->sh_prim, (size_t)T->q_cap[0] * 4);
   need(T, T->score, (size_t)nd * 8 + 64);
   need(T, T->order, (size_t)nd * 4 + 64);
   need(T, T->q[0], (size_t)T->q_cap[0] * sizeof(QEntry));
@@ -1016,17 +1052,12 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->phase_b, (size_t)T->q_cap[0] + 64);
   T->h_cnt.resize(2 * nd);
   T->h_order.resize(nd);
-  int chunk = std::max(64, c->batch_chunk);
-  uint32_t items_cap = (uint32_t)((T->q_cap[0] + T->q_cap[1]) / chunk + 2 * nd + 64);
-  if (T->items_cap < items_cap) {
-    T->items_cap = items_cap;
-    need(T, T->items, (size_t)items_cap * sizeof(WorkItem));
-  }
   need(T, T->image, (size_t)cfg.image_w * cfg.image_h * 16);
   hipDeviceProp_t prop;
   SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
+  // resident blocks: enough waves per CU to hide traversal latency
   int block = 64 * std::max(1, c->batch_tile);
-  int per_cu = std::max(1, 2048 / block);
+  int per_cu = std::max(1, 1536 / block);
   T->grid_persistent = prop.multiProcessorCount * per_cu;
 }
 
@@ -1095,16 +1126,21 @@ static TraceParams makeParams(Ctx* c) {
   int nd = P.ndomains;
   P.cnt = T->cnt.as<uint32_t>();
   P.off = T->off.as<uint32_t>();
+  P.dbase = T->dbase.as<uint32_t>();
   P.cursor = T->cursor.as<uint32_t>();
+  P.wctr = T->wctr.as<uint32_t>();
   P.score = T->score.as<unsigned long long>();
   P.order = T->order.as<int>();
   P.q[0] = T->q[0].as<QEntry>();
   P.q[1] = T->q[1].as<QEntry>();
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
-  P.items = T->items.as<WorkItem>();
-  P.items_cap = T->items_cap;
-  P.chunk = std::max(64, c->batch_chunk);
+  P.refill = std::min(64, std::max(1, c->batch_chunk));
+  P.sh_ray = T->sh_ray.as<uint32_t>();
+  P.sh_dom = T->sh_dom.as<uint32_t>();
+  P.sh_tuv = T->sh_tuv.as<float>();
+  P.sh_prim = T->sh_prim.as<int>();
+  P.sh_cap = T->q_cap[0];
   P.phase_b = T->phase_b.as<uint8_t>();
   P.learned = (cfg.schedule == SPRAY_SCHEDULE_LEARNED) ? 1 : 0;
   P.log_on = T->logging ? 1 : 0;
@@ -1121,9 +1157,9 @@ static size_t scatterShmem(int nd) { return (size_t)nd * (6 + 4) * 4; }
 
 static void ensureOverflow(TracerState* T, size_t lanes) { need(T, T->ovf, lanes * kOverflow * 4); }
 
-// One pass of the plain schedule: host reads the per-domain counts.
-static bool passPlain(Ctx* c, const TraceParams& P) {
-  TracerState* T = c->tracer;
+static size_t travShmem(int block) { return (size_t)kRing * block * 4; }
+
+static void passQueues(Ctx* c, const TraceParams& P) {
   hipStream_t s = c->stream;
   int nd = P.ndomains;
   kPassBegin<<<1, 1, 0, s>>>(P);
@@ -1131,53 +1167,68 @@ static bool passPlain(Ctx* c, const TraceParams& P) {
   kCount<<<1024, kBlock, countShmem(nd), s>>>(P);
   kSchedule<<<1, 1024, 0, s>>>(P);
   kScatter<<<1024, kBlock, scatterShmem(nd), s>>>(P);
+}
+
+static void passFinish(Ctx* c, const TraceParams& P) {
+  hipStream_t s = c->stream;
+  kShade<<<1024, kBlock, 0, s>>>(P);
+  kRetire<<<1024, 256, 0, s>>>(P);
+  if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
+}
+
+// One pass of the plain schedule: the host reads the per-domain counts and
+// launches one trace kernel per domain in schedule order (the reference's
+// sequential domain sweep, ooc_pcontext.h:127-160).
+static bool passPlain(Ctx* c, const TraceParams& P) {
+  TracerState* T = c->tracer;
+  hipStream_t s = c->stream;
+  int nd = P.ndomains;
+  passQueues(c, P);
   DevCounters h;
   SPRAY_HIP(hipMemcpyAsync(&h, P.ctr, sizeof(h), hipMemcpyDeviceToHost, s));
   SPRAY_HIP(hipMemcpyAsync(T->h_cnt.data(), P.cnt, 2 * nd * 4, hipMemcpyDeviceToHost, s));
   SPRAY_HIP(hipMemcpyAsync(T->h_order.data(), P.order, nd * 4, hipMemcpyDeviceToHost, s));
   SPRAY_HIP(hipStreamSynchronize(s));
   if (!h.active) return false;
-  if (h.err) return true;  // reported after the frame
-  size_t max_blocks = 0;
-  for (int k = 0; k < nd; ++k) {
-    int d = T->h_order[k];
-    uint32_t nr = T->h_cnt[d], ns = T->h_cnt[nd + d];
-    uint32_t br = (nr + kBlock - 1) / kBlock, bs = (ns + kBlock - 1) / kBlock;
-    max_blocks = std::max<size_t>(max_blocks, br + bs);
-  }
-  ensureOverflow(T, max_blocks * kBlock);
+  hipDeviceProp_t prop;
+  SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
+  const int max_grid = prop.multiProcessorCount * (1536 / kTravBlock);
+  ensureOverflow(T, (size_t)max_grid * kTravBlock);
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
   for (int k = 0; k < nd; ++k) {
     int d = T->h_order[k];
-    uint32_t nr = T->h_cnt[d], ns = T->h_cnt[nd + d];
-    if (nr + ns == 0) continue;
-    uint32_t br = (nr + kBlock - 1) / kBlock, bs = (ns + kBlock - 1) / kBlock;
-    kTraceDomain<<<br + bs, kBlock, 0, s>>>(Q, d, br);
+    uint32_t n = T->h_cnt[d] + T->h_cnt[nd + d];
+    if (n == 0) continue;
+    int grid = (int)std::min<uint32_t>(max_grid, (n + kTravBlock - 1) / kTravBlock);
+    int ei = c->profile ? T->evBegin(s) : -1;
+    kTraverse<<<grid, kTravBlock, travShmem(kTravBlock), s>>>(Q, d, 0);
+    if (ei >= 0) T->evEnd(ei, s);
   }
-  kRetire<<<1024, 256, 0, s>>>(P);
-  if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
+  passFinish(c, P);
   return true;
 }
 
-// One pass of the batched schedule: sizes stay on the device.
+// One pass of the batched schedule: every queue of the pass in one launch
+// (two with the learned form: deferred entries run after the others);
+// sizes stay on the device so the pass can be captured in a graph.
 static void passBatched(Ctx* c, const TraceParams& P) {
   TracerState* T = c->tracer;
   hipStream_t s = c->stream;
-  int nd = P.ndomains;
   int block = 64 * std::max(1, c->batch_tile);
-  kPassBegin<<<1, 1, 0, s>>>(P);
-  kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
-  kCount<<<1024, kBlock, countShmem(nd), s>>>(P);
-  kSchedule<<<1, 1024, 0, s>>>(P);
-  kScatter<<<1024, kBlock, scatterShmem(nd), s>>>(P);
+  passQueues(c, P);
+  if (P.learned) predictorPhase(c, P);
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
-  size_t shm = (size_t)kRing * block * 4;
-  kTracePersistent<<<T->grid_persistent, block, shm, s>>>(Q, 0);
-  if (P.learned) kTracePersistent<<<T->grid_persistent, block, shm, s>>>(Q, 1);
-  kRetire<<<1024, 256, 0, s>>>(P);
-  if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
+  int ei = c->profile ? T->evBegin(s) : -1;
+  kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -1, 0);
+  if (ei >= 0) T->evEnd(ei, s);
+  if (P.learned) {
+    ei = c->profile ? T->evBegin(s) : -1;
+    kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -1, 1);
+    if (ei >= 0) T->evEnd(ei, s);
+  }
+  passFinish(c, P);
 }
 
 static void tileBody(Ctx* c, const TraceParams& P, bool batched, int windows, int* passes_run) {
@@ -1229,7 +1280,8 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     SPRAY_HIP(hipEventRecord(e0, s));
     int key[6] = {(int)batched, windows, (int)P.learned, c->batch_chunk, c->batch_tile,
                   P.log_on + 2 * c->param_version};
-    if (batched && (T->graph == nullptr || std::memcmp(key, T->graph_key, sizeof(key)) != 0)) {
+    const bool use_graph = batched && !c->profile;  // profiling times each launch instead
+    if (use_graph && (T->graph == nullptr || std::memcmp(key, T->graph_key, sizeof(key)) != 0)) {
       T->releaseGraph();
       hipGraph_t g;
       SPRAY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1242,10 +1294,10 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     for (size_t ti = 0; ti < ys.size(); ++ti) {
       int rec[4] = {ys[ti], hs[ti], 0, 0};
       SPRAY_HIP(hipMemcpyAsync(T->tile_rec.p, rec, sizeof(rec), hipMemcpyHostToDevice, s));
-      if (batched)
+      if (use_graph)
         SPRAY_HIP(hipGraphLaunch(T->graph, s));
       else
-        tileBody(c, P, false, windows, nullptr);
+        tileBody(c, P, batched, windows, nullptr);
     }
     SPRAY_HIP(hipEventRecord(e1, s));
     DevCounters h;
@@ -1253,6 +1305,7 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     SPRAY_HIP(hipStreamSynchronize(s));
     float ms = 0;
     SPRAY_HIP(hipEventElapsedTime(&ms, e0, e1));
+    T->evCollect();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (h.err & 4u) throw StatusError(SPRAY_ERR_CAPACITY, "traversal stack overflow");
@@ -1283,10 +1336,21 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
   throw StatusError(SPRAY_ERR_CAPACITY, "ray pools exceeded after growing 8x");
 }
 
+void tracerKernelTime(Ctx* c, long long* launches, double* ms, int reset) {
+  TracerState* T = c->tracer;
+  *launches = T ? T->prof_launches : 0;
+  *ms = T ? T->prof_ms : 0.0;
+  if (T && reset) {
+    T->prof_launches = 0;
+    T->prof_ms = 0;
+  }
+}
+
 void tracerAutotune(Ctx* c, int* chunk_out, int* tile_out) {
   int saved_sched = c->cfg.schedule;
   if (c->cfg.schedule == SPRAY_SCHEDULE_PLAIN) c->cfg.schedule = SPRAY_SCHEDULE_BATCHED;
-  const int chunks[] = {256, 1024, 4096};
+  // queue chunk = lanes a wave refills at once; tile = waves per block
+  const int chunks[] = {8, 16, 32, 64};
   const int tiles[] = {1, 2, 4};
   double best = 1e30;
   int bc = c->batch_chunk, bt = c->batch_tile;

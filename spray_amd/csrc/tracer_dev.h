@@ -28,24 +28,16 @@ struct QEntry {
   float tdom;
 };
 
-// Work item of the batched form: a chunk of one domain's queue
-struct WorkItem {
-  int domain;
-  int kind;  // 0 radiance, 1 shadow
-  uint32_t begin, end;
-};
-
 struct DevCounters {
   uint32_t r_count, s_count, p_count;  // pool fill levels (atomic append)
   uint32_t r_beg, r_end, s_beg, s_end;  // rays to enqueue / trace in this pass
   uint32_t r_init;                      // initial rays of the window (eyes / restarts)
   uint32_t active;                      // pass has work
   uint32_t err;                         // bit 0 pool overflow, 1 list overflow, 2 stack overflow
-  uint32_t n_items[2];                  // work items (phase A, phase B)
-  uint32_t item_ctr[2];                 // persistent-kernel work counters
   uint32_t r_mark, s_mark;              // end of the last enqueued ranges
   uint32_t q_total[2];                  // entries in this pass's queues
   uint32_t vlog_count;                  // visit-log entries of this pass
+  uint32_t shade_count;                 // accepted hits of this pass (shade records)
   unsigned long long useful, radiance_visits, shadow_visits, shades, final_hits, queue_entries,
       pruned, deferred, passes, node_visits, tri_tests, log_count;
 };
@@ -78,16 +70,22 @@ struct TraceParams {
   const int* tile_rec;       // [tile_y, tile_h, ray_depth(window offset)]
   // queues
   uint32_t* cnt;      // [2][nd] entries per domain (radiance, shadow)
-  uint32_t* off;      // [2][nd + 1] exclusive offsets
+  uint32_t* off;      // [2][nd + 1] queue offsets in schedule order
+  uint32_t* dbase;    // [2][nd] queue offset of each domain (by id)
   uint32_t* cursor;   // [2][nd]
+  uint32_t* wctr;     // [nd + 2] work counters: per domain (plain), phase A / B (batched)
   unsigned long long* score;  // [nd]
   int* order;         // [nd] schedule
   QEntry* q[2];       // queues (radiance, shadow)
   uint32_t q_cap[2];
-  WorkItem* items;    // [2 phases][max items]
-  uint32_t items_cap;
-  int chunk;
+  int refill;         // a wave refetches work once this many lanes are idle
   uint32_t* ovf;      // traversal stack overflow areas
+  // shade records of the pass (accepted hits), SoA
+  uint32_t* sh_ray;
+  uint32_t* sh_dom;
+  float* sh_tuv;      // t, u, v
+  int* sh_prim;
+  uint32_t sh_cap;
   // learned form
   uint8_t* phase_b;   // per radiance queue entry: 1 = deferred
   int learned;
@@ -99,6 +97,11 @@ struct TraceParams {
   unsigned long long log_cap;  // samples
 };
 
+struct Ctx;
+// mlp.hip: assigns every radiance queue entry of the pass to phase A or the
+// deferred phase B (learned form)
+void predictorPhase(Ctx* c, const TraceParams& P);
+
 // ---- history buffer (VBuf, ooc/ooc_vbuf.cc:41-112) -------------------------
 // Each of the 4 depth slots of a sample is one 64-bit word: the accepted t in
 // the high half and the t of the parent slot in the low half.  A slot whose
@@ -109,20 +112,30 @@ __device__ __forceinline__ uint32_t wP(unsigned long long w) { return (uint32_t)
 __device__ __forceinline__ unsigned long long mkW(float t, uint32_t parent) {
   return ((unsigned long long)__float_as_uint(t) << 32) | parent;
 }
+// Relaxed device-scope loads read the L2 copy (no L1 invalidation).  Within
+// one launch only stale-chain candidates can race with a prefix change, and
+// their writes are invalid by construction, so no acquire ordering is needed;
+// launches are ordered by the stream.
 __device__ __forceinline__ unsigned long long ldW(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // VBuf::correct: history[0..n) equals the buffer's chain
 __device__ __forceinline__ bool vbufCorrect(const unsigned long long* vb, int samid,
                                             const float h[4], int n) {
   const unsigned long long* w = vb + (size_t)samid * kHistory;
-  for (int i = 0; i < n; ++i) {
-    unsigned long long x = ldW(&w[i]);
-    if (wT(x) != h[i]) return false;
-    if (i > 0 && wP(x) != __float_as_uint(h[i - 1])) return false;
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < kHistory; ++i) {  // constant indices keep h[] in registers
+    if (ok && i < n) {
+      unsigned long long x = ldW(&w[i]);
+      ok = wT(x) == h[i] && (i == 0 || wP(x) == __float_as_uint(h[i > 0 ? i - 1 : 0]));
+    }
   }
-  return true;
+  return ok;
+}
+__device__ __forceinline__ float pick4(const float h[4], int i) {
+  return i == 0 ? h[0] : (i == 1 ? h[1] : (i == 2 ? h[2] : h[3]));
 }
 
 // VBuf::update for a ray at `depth` with hit distance t: true when the
@@ -130,13 +143,15 @@ __device__ __forceinline__ bool vbufCorrect(const unsigned long long* vb, int sa
 __device__ __forceinline__ bool vbufUpdate(unsigned long long* vb, int samid, const float h[4],
                                            int depth, float t) {
   unsigned long long* w = vb + (size_t)samid * kHistory + depth;
-  uint32_t parent = depth > 0 ? __float_as_uint(h[depth - 1]) : 0u;
+  uint32_t parent = depth > 0 ? __float_as_uint(pick4(h, depth - 1)) : 0u;
   unsigned long long want = mkW(t, parent);
+  unsigned long long old = ldW(w);
   while (true) {
-    unsigned long long old = ldW(w);
     if (!vbufCorrect(vb, samid, h, depth)) return false;
     if (wP(old) == parent && !(t < wT(old))) return false;
-    if (__hip_atomic_compare_exchange_strong(w, &old, want, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+    // on failure `old` receives the current value (lock-free: another
+    // candidate succeeded)
+    if (__hip_atomic_compare_exchange_strong(w, &old, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT))
       return true;
   }

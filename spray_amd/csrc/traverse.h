@@ -229,3 +229,127 @@ __device__ __forceinline__ bool anyHit(const BNode4* __restrict__ nodes,
 }
 
 }  // namespace spray
+
+namespace spray {
+
+// Resumable traversal of one ray (closest or any hit): one call of
+// travStep() processes one node, so a wave can refill lanes whose ray has
+// finished while the others keep traversing (Aila & Laine 2009,
+// "persistent while-while" with per-lane replacement).
+struct TravState {
+  const BNode4* nodes;
+  const LTri* tris;
+  float org[3], inv[3];
+  RayPre rp;
+  float tnear, tfar, tlim;  // tlim: current best t (closest) or tfar (any)
+  float u, v;
+  int prim;
+  uint32_t cur, guard, max_nodes;
+  bool any;
+};
+
+__device__ __forceinline__ void travInit(TravState& s, const BNode4* nodes, const LTri* tris,
+                                         uint32_t max_nodes, const float org[3],
+                                         const float dir[3], float tnear, float tfar, bool any,
+                                         LaneStack& st) {
+  s.nodes = nodes;
+  s.tris = tris;
+  s.org[0] = org[0];
+  s.org[1] = org[1];
+  s.org[2] = org[2];
+  s.inv[0] = 1.0f / dir[0];
+  s.inv[1] = 1.0f / dir[1];
+  s.inv[2] = 1.0f / dir[2];
+  s.rp = rayPre(dir);
+  s.tnear = tnear;
+  s.tfar = tfar;
+  s.tlim = tfar;
+  s.u = s.v = 0.f;
+  s.prim = -1;
+  s.cur = 0;
+  s.guard = 0;
+  s.max_nodes = max_nodes;
+  s.any = any;
+  st.reset();
+}
+
+__device__ __forceinline__ void cswap(float& ka, int& ia, float& kb, int& ib) {
+  bool sw = kb < ka;
+  float tk = sw ? kb : ka;
+  int ti = sw ? ib : ia;
+  kb = sw ? ka : kb;
+  ib = sw ? ia : ib;
+  ka = tk;
+  ia = ti;
+}
+
+// Processes one node; returns true when the ray is finished (stack empty,
+// any-hit found, or the visit bound exceeded -> st.overflowed).
+__device__ __forceinline__ bool travStep(TravState& s, LaneStack& st, TravCount& cnt) {
+  if (++s.guard > s.max_nodes) {
+    st.overflowed = true;
+    return true;
+  }
+  float tmin[4];
+  int child[4];
+  cnt.nodes++;
+  int mask = nodeBoxes(&s.nodes[s.cur], s.org, s.inv, s.tnear, s.tlim, tmin, child);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (!((mask >> i) & 1) || child[i] >= 0) continue;
+    int first = child[i] & 0x00ffffff;
+    int n = (child[i] >> 24) & 0x7f;
+    for (int k = 0; k < n; ++k) {
+      const float4* tq = (const float4*)&s.tris[first + k];
+      float4 a = tq[0], b = tq[1], c = tq[2];
+      float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
+      float t, u, v;
+      cnt.tris++;
+      if (triHit(s.rp, s.org, v0, v1, v2, s.tnear, s.any ? s.tlim : s.tfar, &t, &u, &v)) {
+        if (s.any) {
+          s.prim = 1;
+          return true;
+        }
+        int p = __float_as_int(a.w);
+        if (s.prim < 0 || t < s.tlim || (t == s.tlim && p < s.prim)) {
+          s.tlim = t;
+          s.u = u;
+          s.v = v;
+          s.prim = p;
+        }
+      }
+    }
+  }
+  // inner children, nearest first (sorting network on 4 keys; no local arrays)
+  const float lim = s.tlim * kSlabInflate;
+  float k0 = 3.4e38f, k1 = 3.4e38f, k2 = 3.4e38f, k3 = 3.4e38f;
+  float ks[4];
+  int m = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    bool ok = ((mask >> i) & 1) && child[i] >= 0 && tmin[i] <= lim;
+    ks[i] = ok ? fminf(tmin[i], 3.0e38f) : kInfF;
+    m += ok ? 1 : 0;
+  }
+  k0 = ks[0];
+  k1 = ks[1];
+  k2 = ks[2];
+  k3 = ks[3];
+  int i0 = child[0], i1 = child[1], i2 = child[2], i3 = child[3];
+  cswap(k0, i0, k1, i1);
+  cswap(k2, i2, k3, i3);
+  cswap(k0, i0, k2, i2);
+  cswap(k1, i1, k3, i3);
+  cswap(k1, i1, k2, i2);
+  if (m == 0) {
+    s.cur = st.pop();
+    return s.cur == kStackEmpty;
+  }
+  if (m > 3) st.push((uint32_t)i3);
+  if (m > 2) st.push((uint32_t)i2);
+  if (m > 1) st.push((uint32_t)i1);
+  s.cur = (uint32_t)i0;
+  return false;
+}
+
+}  // namespace spray
