@@ -104,7 +104,7 @@ def main():
             if name == "learned":
                 # train the predictor on hit logs of plain-schedule frames
                 ctx.set_schedule(0)
-                ctx.predictor_log(True, 50_000_000)
+                ctx.predictor_log(True, 16_000_000)
                 ctx.trace_frame(want_image=False)
                 ctx.predictor_log(False)
                 ctx.predictor_train(2, 4096, 2e-3)

@@ -172,6 +172,18 @@ int spray_predictor_eval(spray_ctx* ctx, float* loss_out, float* accuracy_out);
 int spray_predictor_save(spray_ctx* ctx, const char* path);
 int spray_predictor_load(spray_ctx* ctx, const char* path);
 
+/* Predictor test hooks (new): one forward/backward of the MLP kernels on
+ * host samples (8 floats each: origin, direction, domain, label) with host
+ * parameters (flat fp32: W1[128][22], b1[128], W2[128][128], b2[128],
+ * w3[128], b3, emb[ndomains][16]); returns the summed BCE loss and the
+ * gradient of the mean BCE (n a multiple of 32).  spray_mlp_logits runs the
+ * forward only (params may be NULL: current weights). */
+int spray_mlp_num_params(spray_ctx* ctx);
+int spray_mlp_grad(spray_ctx* ctx, const float* samples, int n, const float* params,
+                   float* loss_out, float* grad_out);
+int spray_mlp_logits(spray_ctx* ctx, const float* samples, int n, const float* params,
+                     float* logits_out);
+
 /* Synthetic scene writer: domains as PLY files plus a .spray file
  * (dir/name.spray); extra_lines (e.g. light lines) go to the header. */
 int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const char* name,

@@ -109,6 +109,9 @@ PROTOTYPES = {
     "spray_predictor_eval": (C.c_int, [_P, _FP, _FP]),
     "spray_predictor_save": (C.c_int, [_P, C.c_char_p]),
     "spray_predictor_load": (C.c_int, [_P, C.c_char_p]),
+    "spray_mlp_num_params": (C.c_int, [_P]),
+    "spray_mlp_grad": (C.c_int, [_P, _FP, C.c_int, _FP, _FP, _FP]),
+    "spray_mlp_logits": (C.c_int, [_P, _FP, C.c_int, _FP, _FP]),
     "spray_scenegen_write": (C.c_int, [C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_longlong)]),
 }
 

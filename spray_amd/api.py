@@ -186,6 +186,25 @@ class Context:
         self._check(self.lib.spray_predictor_eval(self.h, C.byref(loss), C.byref(acc)))
         return loss.value, acc.value
 
+    def mlp_num_params(self):
+        return self.lib.spray_mlp_num_params(self.h)
+
+    def mlp_grad(self, samples, params):
+        samples = np.ascontiguousarray(samples, np.float32)
+        params = np.ascontiguousarray(params, np.float32)
+        g = np.zeros(self.mlp_num_params(), np.float32)
+        loss = C.c_float()
+        self._check(self.lib.spray_mlp_grad(self.h, _fp(samples), len(samples), _fp(params),
+                                            C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def mlp_logits(self, samples, params=None):
+        samples = np.ascontiguousarray(samples, np.float32)
+        z = np.zeros(len(samples), np.float32)
+        pp = _fp(np.ascontiguousarray(params, np.float32)) if params is not None else None
+        self._check(self.lib.spray_mlp_logits(self.h, _fp(samples), len(samples), pp, _fp(z)))
+        return z
+
     def predictor_save(self, path):
         self._check(self.lib.spray_predictor_save(self.h, str(path).encode()))
 

@@ -117,6 +117,9 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st);
 void tracerRelease(Ctx* c);
 void tracerAutotune(Ctx* c, int* chunk, int* tile);
 void tracerKernelTime(Ctx* c, long long* launches, double* ms, int reset);
+void tracerSetLogging(Ctx* c, bool on, long long max_samples);
+long long tracerLogCount(const Ctx* c);
+const float* tracerLogBuffer(Ctx* c);
 // mlp.hip
 void predictorRelease(Ctx* c);
 

@@ -895,7 +895,7 @@ __global__ void kLogFinalize(TraceParams P) {
     float4 hh = P.R.h[r];
     float best = depth == 0 ? hh.x : depth == 1 ? hh.y : depth == 2 ? hh.z : hh.w;
     float label = (t != kInfF && t == best) ? 1.f : 0.f;
-    unsigned long long slot = atomicAdd(&c->log_count, 1ull);
+    unsigned long long slot = atomicAdd(P.log_count, 1ull);
     if (slot >= P.log_cap) continue;
     float4 op = P.R.op[r], ds = P.R.ds[r];
     float* o = P.log_buf + slot * 8;
@@ -920,6 +920,7 @@ struct TracerState {
       wctr, score, order, q[2], ovf, phase_b, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim;
   unsigned long long log_cap = 0;
   bool logging = false;
+  DevBuf log_ctr;
   int grid_persistent = 0;
   hipGraphExec_t graph = nullptr;
   int graph_key[6] = {0, 0, 0, 0, 0, 0};
@@ -968,7 +969,7 @@ void tracerRelease(Ctx* c) {
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->cursor, &T->wctr, &T->score, &T->order, &T->q[0],
                     &T->q[1], &T->ovf, &T->phase_b, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
-                    &T->sh_tuv, &T->sh_prim})
+                    &T->sh_tuv, &T->sh_prim, &T->log_ctr})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1148,6 +1149,7 @@ static TraceParams makeParams(Ctx* c) {
   P.vlog_cap = T->logging ? (uint32_t)std::min<size_t>(T->vlog.bytes / 12, 0xffffffffu) : 0u;
   P.log_buf = T->log.as<float>();
   P.log_cap = T->logging ? T->log_cap : 0ull;
+  P.log_count = T->log_ctr.as<unsigned long long>();
   (void)nd;
   return P;
 }
@@ -1269,6 +1271,11 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     bool batched = cfg.schedule != SPRAY_SCHEDULE_PLAIN;
     int block = 64 * std::max(1, c->batch_tile);
     ensureOverflow(T, std::max<size_t>((size_t)T->grid_persistent * block, 4096));
+    if (cfg.schedule == SPRAY_SCHEDULE_LEARNED) {
+      if (!predictorReady(c))
+        throw StatusError(SPRAY_ERR_STATE, "the learned schedule needs a trained or loaded predictor");
+      predictorPrepare(c, T->r_cap);
+    }
     TraceParams P = makeParams(c);
     P.ovf = T->ovf.as<uint32_t>();
     SPRAY_HIP(hipMemsetAsync(P.image, 0, (size_t)cfg.image_w * cfg.image_h * 16, s));
@@ -1334,6 +1341,36 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     return;
   }
   throw StatusError(SPRAY_ERR_CAPACITY, "ray pools exceeded after growing 8x");
+}
+
+void tracerSetLogging(Ctx* c, bool on, long long max_samples) {
+  ensureState(c, 1.0);
+  TracerState* T = c->tracer;
+  c->param_version++;
+  if (!on) {
+    T->logging = false;
+    return;
+  }
+  if (max_samples <= 0) max_samples = 50000000;
+  need(T, T->log, (size_t)max_samples * 32);
+  need(T, T->vlog, (size_t)T->q_cap[0] * 12);
+  need(T, T->log_ctr, 64);
+  SPRAY_HIP(hipMemset(T->log_ctr.p, 0, 8));
+  T->log_cap = (unsigned long long)max_samples;
+  T->logging = true;
+}
+
+long long tracerLogCount(const Ctx* c) {
+  const TracerState* T = c->tracer;
+  if (!T || !T->log_ctr.p) return 0;
+  unsigned long long n = 0;
+  if (hipMemcpy(&n, T->log_ctr.p, 8, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+  return (long long)std::min<unsigned long long>(n, T->log_cap);
+}
+
+const float* tracerLogBuffer(Ctx* c) {
+  if (!c->tracer || !c->tracer->log.p) throw StatusError(SPRAY_ERR_STATE, "no logged samples");
+  return c->tracer->log.as<float>();
 }
 
 void tracerKernelTime(Ctx* c, long long* launches, double* ms, int reset) {

@@ -95,12 +95,15 @@ struct TraceParams {
   uint32_t vlog_cap;           // entries of 3 words
   float* log_buf;              // 8 floats per sample
   unsigned long long log_cap;  // samples
+  unsigned long long* log_count;  // samples logged so far (kept across frames)
 };
 
 struct Ctx;
 // mlp.hip: assigns every radiance queue entry of the pass to phase A or the
 // deferred phase B (learned form)
 void predictorPhase(Ctx* c, const TraceParams& P);
+bool predictorReady(Ctx* c);
+void predictorPrepare(Ctx* c, size_t r_cap);
 
 // ---- history buffer (VBuf, ooc/ooc_vbuf.cc:41-112) -------------------------
 // Each of the 4 depth slots of a sample is one 64-bit word: the accepted t in
