@@ -228,6 +228,8 @@ __global__ void kResetKeys(TraceParams P, unsigned long long* rkey) {
     rkey[i] = 0ull;
 }
 
+// Splits the radiance queue into the phase-A list and the deferred list
+// (both compacted, queue order kept within a wave).
 __global__ void kAssignPhase(TraceParams P, const unsigned long long* rkey) {
   DevCounters* c = P.ctr;
   if (!c->active) return;
@@ -235,17 +237,19 @@ __global__ void kAssignPhase(TraceParams P, const unsigned long long* rkey) {
   unsigned long long deferred = 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t q = base + threadIdx.x;
-    bool d = false;
-    if (q < n) {
+    bool valid = q < n, d = false;
+    if (valid) {
       QEntry e = P.q[0][q];
       uint32_t enc = ~(uint32_t)rkey[e.ray];  // ordered bits of the chosen tdom
       d = orderedF(e.tdom) > enc;
-      P.phase_b[q] = d ? 1 : 0;
     }
-    unsigned long long v = d ? 1ull : 0ull;
-    v = waveSumULL(v);
-    deferred += (__lane_id() == 0) ? v : 0ull;
+    uint32_t ia = waveAppend(valid && !d, &c->n_phase[0], P.q_cap[0], &c->err);
+    uint32_t ib = waveAppend(valid && d, &c->n_phase[1], P.q_cap[0], &c->err);
+    if (ia != 0xffffffffu) P.qa[ia] = q;
+    if (ib != 0xffffffffu) P.qb[ib] = q;
+    deferred += d ? 1 : 0;
   }
+  deferred = waveSumULL(deferred);
   if (__lane_id() == 0 && deferred) atomicAdd(&c->deferred, deferred);
 }
 

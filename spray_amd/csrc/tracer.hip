@@ -137,6 +137,8 @@ __global__ void kPassBegin(TraceParams P) {
   if (c->active) c->passes++;
   c->vlog_count = 0;
   c->shade_count = 0;
+  c->n_phase[0] = 0;
+  c->n_phase[1] = 0;
 }
 
 __global__ void kClearQ(TraceParams P) {
@@ -150,7 +152,7 @@ __global__ void kClearQ(TraceParams P) {
     P.score[i] = 0;
     P.wctr[i] = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x < 2) P.wctr[nd + threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 3) P.wctr[nd + threadIdx.x] = 0;
 }
 
 // Ray of the current pass: index into the radiance range, then the shadow range.
@@ -200,6 +202,8 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     uint32_t r;
     passRay(P, i, &kind, &r);
     const RayPool& pool = kind ? P.S : P.R;
+    // shadow rays occluded in their spawning domain are done (procShads2)
+    if (kind == 1 && P.Sx.occluded[r]) continue;
     float4 op = pool.op[r], ds = pool.ds[r];
     float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
     int spawn = kind ? (__float_as_int(pool.wd[r].w) >> 8) : -1;
@@ -207,15 +211,13 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     float tt[kTopDepths];
     int ti[kTopDepths];
     int kept = 0;
-    bool spawn_listed = false;
     for (int dd = 0; dd < nd; ++dd) {
       float tmin;
       if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
+      // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
+      // spawning domain was tested right after shading
+      if (dd == spawn) continue;
       atomicAdd(&lc[kind * nd + dd], 1u);
-      if (dd == spawn) {
-        spawn_listed = true;
-        continue;  // the spawning domain's test is not a queued entry in the reference
-      }
       // insertion into the (t, id)-sorted top list (unrolled: stays in registers)
       if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
         float ct = tmin;
@@ -239,7 +241,6 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
         if (kept < kTopDepths) ++kept;
       }
     }
-    if (kind == 1 && spawn >= 0 && !spawn_listed) atomicAdd(&lc[nd + spawn], 1u);
     for (int k = 0; k < kTopDepths; ++k)
       if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
   }
@@ -351,18 +352,18 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
       d[0] = ds.x;
       d[1] = ds.y;
       d[2] = ds.z;
-      if (kind) spawn = __float_as_int(pool.wd[r].w) >> 8;
+      if (kind) {
+        spawn = __float_as_int(pool.wd[r].w) >> 8;
+        if (P.Sx.occluded[r]) valid = false;  // occluded in its spawning domain
+      }
     }
     // pass 1: local counts
-    bool spawn_listed = false;
     if (valid) {
       for (int dd = 0; dd < nd; ++dd) {
         float tmin;
-        if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
+        if (dd == spawn || !slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
         atomicAdd(&lc[kind * nd + dd], 1u);
-        spawn_listed |= (dd == spawn);
       }
-      if (kind == 1 && spawn >= 0 && !spawn_listed) atomicAdd(&lc[nd + spawn], 1u);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < 2 * nd; j += blockDim.x) {
@@ -375,16 +376,10 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
     if (valid) {
       for (int dd = 0; dd < nd; ++dd) {
         float tmin;
-        if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
+        if (dd == spawn || !slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
         uint32_t slot = lb[kind * nd + dd] + atomicAdd(&lc[kind * nd + dd], 1u);
         uint32_t dst = P.dbase[kind * nd + dd] + slot;
         if (dst < P.q_cap[kind]) P.q[kind][dst] = QEntry{r, tmin};
-      }
-      if (kind == 1 && spawn >= 0 && !spawn_listed) {
-        uint32_t slot = lb[nd + spawn] + atomicAdd(&lc[nd + spawn], 1u);
-        uint32_t dst = P.dbase[nd + spawn] + slot;
-        // the spawn domain always tests its own shadow rays (procShads2)
-        if (dst < P.q_cap[1]) P.q[1][dst] = QEntry{r, -kInfF};
       }
     }
     __syncthreads();
@@ -656,24 +651,42 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   DevCounters* c = P.ctr;
   if (!c->active) return;
   const int nd = P.ndomains;
-  uint32_t nr, ns, rbase, sbase;
+  // entry space of this launch: radiance entries [0, nr) then shadow [nr, nr + ns)
+  //   plain_dom >= 0: one domain's queues (plain schedule)
+  //   plain_dom == -1: every queue of the pass (batched); with the learned
+  //     form, phase 0 takes the phase-A radiance list, phase 1 the deferred list
+  //   plain_dom == -2: shadow rays spawned in this pass, in their spawning
+  //     domain (procShads2, ooc_tcontext.inl:51-70)
+  uint32_t nr = 0, ns = 0, rbase = 0, sbase = 0, local_base = 0;
+  const uint32_t* rlist = nullptr;
   uint32_t* wc;
+  const bool local = plain_dom == -2;
   if (plain_dom >= 0) {
     rbase = P.dbase[plain_dom];
     sbase = P.dbase[nd + plain_dom];
     nr = P.cnt[plain_dom];
     ns = P.cnt[nd + plain_dom];
     wc = &P.wctr[plain_dom];
-  } else {
-    rbase = 0;
-    sbase = 0;
+  } else if (local) {
+    local_base = c->s_mark;
+    uint32_t end = min(c->s_count, P.s_cap);
+    ns = end > local_base ? end - local_base : 0u;
+    wc = &P.wctr[nd + 2];
+  } else if (!P.learned) {
     nr = c->q_total[0];
+    ns = c->q_total[1];
+    wc = &P.wctr[nd];
+  } else {
+    nr = phase == 0 ? c->n_phase[0] : c->n_phase[1];
     ns = phase == 0 ? c->q_total[1] : 0u;
+    rlist = phase == 0 ? P.qa : P.qb;
     wc = &P.wctr[nd + phase];
   }
   // bounded by the queue capacities even after an overflow
-  nr = rbase < P.q_cap[0] ? min(nr, P.q_cap[0] - rbase) : 0u;
-  ns = sbase < P.q_cap[1] ? min(ns, P.q_cap[1] - sbase) : 0u;
+  if (!local) {
+    nr = rbase < P.q_cap[0] ? min(nr, P.q_cap[0] - rbase) : 0u;
+    ns = sbase < P.q_cap[1] ? min(ns, P.q_cap[1] - sbase) : 0u;
+  }
   const uint32_t total = nr + ns;
   LaneStack st;
   long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -699,12 +712,18 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
         uint32_t e = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         if (e < total) {
           kind = e < nr ? 0 : 1;
-          uint32_t q = kind == 0 ? rbase + e : sbase + (e - nr);
-          QEntry qe = P.q[kind][q];
-          dom = plain_dom >= 0 ? plain_dom : (int)findDomain(P, kind, q);
+          QEntry qe;
+          if (local) {
+            qe = QEntry{local_base + e, 0.f};
+          } else {
+            uint32_t q = kind == 0 ? (rlist ? rlist[e] : rbase + e) : sbase + (e - nr);
+            qe = P.q[kind][q];
+            dom = plain_dom >= 0 ? plain_dom : (int)findDomain(P, kind, q);
+          }
           ray = qe.ray;
           const RayPool& pool = kind ? P.S : P.R;
           float4 op = pool.op[ray], ds = pool.ds[ray], wd = pool.wd[ray], hh = pool.h[ray];
+          if (local) dom = __float_as_int(wd.w) >> 8;
           hist[0] = hh.x;
           hist[1] = hh.y;
           hist[2] = hh.z;
@@ -714,9 +733,10 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
           bool go;
           float tfar = kInfF;
           if (kind == 0) {
-            bool deferred = P.learned && P.phase_b[q];
-            go = (phase == 0 ? !deferred : deferred) && qe.tdom <= pick4(hist, depth) &&
-                 vbufCorrect(P.vbuf, samid, hist, depth);
+            go = qe.tdom <= pick4(hist, depth) && vbufCorrect(P.vbuf, samid, hist, depth);
+          } else if (local) {
+            go = true;  // spawned from a final-chain hit in this pass
+            tfar = P.Sx.tmax[ray];
           } else {
             go = !__hip_atomic_load(&P.Sx.occluded[ray], __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT) &&
@@ -917,7 +937,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, cursor,
-      wctr, score, order, q[2], ovf, phase_b, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim;
+      wctr, score, order, q[2], ovf, phase_b, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa, qb;
   unsigned long long log_cap = 0;
   bool logging = false;
   DevBuf log_ctr;
@@ -969,7 +989,7 @@ void tracerRelease(Ctx* c) {
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->cursor, &T->wctr, &T->score, &T->order, &T->q[0],
                     &T->q[1], &T->ovf, &T->phase_b, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
-                    &T->sh_tuv, &T->sh_prim, &T->log_ctr})
+                    &T->sh_tuv, &T->sh_prim, &T->log_ctr, &T->qa, &T->qb})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1041,7 +1061,9 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->off, (size_t)2 * (nd + 1) * 4 + 64);
   need(T, T->dbase, (size_t)2 * nd * 4 + 64);
   need(T, T->cursor, (size_t)2 * nd * 4 + 64);
-  need(T, T->wctr, (size_t)(nd + 2) * 4 + 64);
+  need(T, T->wctr, (size_t)(nd + 3) * 4 + 64);
+  need(T, T->qa, (size_t)T->q_cap[0] * 4);
+  need(T, T->qb, (size_t)T->q_cap[0] * 4);
   need(T, T->sh_ray, (size_t)T->q_cap[0] * 4);
   need(T, T->sh_dom, (size_t)T->q_cap[0] * 4);
   need(T, T->sh_tuv, (size_t)T->q_cap[0] * 12);
@@ -1143,6 +1165,8 @@ static TraceParams makeParams(Ctx* c) {
   P.sh_prim = T->sh_prim.as<int>();
   P.sh_cap = T->q_cap[0];
   P.phase_b = T->phase_b.as<uint8_t>();
+  P.qa = T->qa.as<uint32_t>();
+  P.qb = T->qb.as<uint32_t>();
   P.learned = (cfg.schedule == SPRAY_SCHEDULE_LEARNED) ? 1 : 0;
   P.log_on = T->logging ? 1 : 0;
   P.vlog = T->vlog.as<uint32_t>();
@@ -1172,8 +1196,14 @@ static void passQueues(Ctx* c, const TraceParams& P) {
 }
 
 static void passFinish(Ctx* c, const TraceParams& P) {
+  TracerState* T = c->tracer;
   hipStream_t s = c->stream;
   kShade<<<1024, kBlock, 0, s>>>(P);
+  // new shadow rays: any hit in the spawning domain first (procShads2)
+  TraceParams Q = P;
+  Q.ovf = T->ovf.as<uint32_t>();
+  int block = 64 * std::max(1, c->batch_tile);
+  kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -2, 0);
   kRetire<<<1024, 256, 0, s>>>(P);
   if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
 }

@@ -38,6 +38,7 @@ struct DevCounters {
   uint32_t q_total[2];                  // entries in this pass's queues
   uint32_t vlog_count;                  // visit-log entries of this pass
   uint32_t shade_count;                 // accepted hits of this pass (shade records)
+  uint32_t n_phase[2];                  // learned form: radiance entries in phase A / B
   unsigned long long useful, radiance_visits, shadow_visits, shades, final_hits, queue_entries,
       pruned, deferred, passes, node_visits, tri_tests, log_count;
 };
@@ -73,7 +74,7 @@ struct TraceParams {
   uint32_t* off;      // [2][nd + 1] queue offsets in schedule order
   uint32_t* dbase;    // [2][nd] queue offset of each domain (by id)
   uint32_t* cursor;   // [2][nd]
-  uint32_t* wctr;     // [nd + 2] work counters: per domain (plain), phase A / B (batched)
+  uint32_t* wctr;     // [nd + 3] work counters: per domain (plain), phase A / B, local shadows
   unsigned long long* score;  // [nd]
   int* order;         // [nd] schedule
   QEntry* q[2];       // queues (radiance, shadow)
@@ -88,6 +89,8 @@ struct TraceParams {
   uint32_t sh_cap;
   // learned form
   uint8_t* phase_b;   // per radiance queue entry: 1 = deferred
+  uint32_t* qa;       // learned form: radiance queue indices of phase A
+  uint32_t* qb;       //   and of the deferred phase B
   int learned;
   // predictor logging: visit log of the pass (ray, domain, t) and samples
   int log_on;
