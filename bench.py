@@ -162,9 +162,12 @@ def main():
         avg_ms = r["trace_kernel_ms"] / max(1, r["trace_kernel_launches"])
         bytes_per_launch = r["trace_kernel_bytes"] / max(1, r["trace_kernel_launches"])
         achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        traffic = measured_traffic(best)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "kTracePersistent" if best != "plain" else "kTraceDomain",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic["bytes_per_dispatch"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
+                "kernel": "spray::kTraverse",
                 "avg_launch_ms": round(avg_ms, 4), "launches_per_frame": r["trace_kernel_launches"],
                 "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
@@ -190,6 +193,26 @@ def main():
     }
     print(json.dumps(line))
     ctx.close()
+
+
+def measured_traffic(schedule):
+    """HBM bytes per trace-kernel dispatch measured by two rocprofv3 --pmc
+    runs (FETCH_SIZE, WRITE_SIZE) of this benchmark, scaled by the same
+    counters on a copy of known size; recorded as `key: value` lines."""
+    path = os.path.join(ROOT, "profiles", f"kTraverse_c2_{schedule}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+    try:
+        return {"bytes_per_dispatch": int(float(kv["traffic_bytes_per_dispatch"])),
+                "source": os.path.relpath(path, ROOT)}
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(scene_dir, kw, ntiles):

@@ -1203,7 +1203,9 @@ static void passFinish(Ctx* c, const TraceParams& P) {
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
   int block = 64 * std::max(1, c->batch_tile);
+  int ei = c->profile ? T->evBegin(s) : -1;
   kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -2, 0);
+  if (ei >= 0) T->evEnd(ei, s);
   kRetire<<<1024, 256, 0, s>>>(P);
   if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
 }
