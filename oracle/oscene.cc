@@ -219,7 +219,7 @@ void mergeBounds(Scene* s) {
   }
   for (auto& d : s->domains) {
     // scene.inl:314-317: bound, vertex and face counts are required
-    bool valid = d.lo[0] <= d.hi[0] && d.lo[1] <= d.hi[1] && d.lo[2] <= d.hi[2];
+    bool valid = d.has_bound && d.lo[0] <= d.hi[0] && d.lo[1] <= d.hi[1] && d.lo[2] <= d.hi[2];
     if (!valid) fail("domain " + std::to_string(d.id) + ": invalid bound");
     if (d.num_vertices == 0 || d.num_faces == 0)
       fail("domain " + std::to_string(d.id) + ": missing vertex/face count");

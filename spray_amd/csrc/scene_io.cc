@@ -170,6 +170,7 @@ int parseSceneFile(const std::string& spray_file, const std::string& ply_path, H
         d.obj_lo[k] = num(t[1 + k]);
         d.obj_hi[k] = num(t[4 + k]);
       }
+      d.has_bound = true;
     } else if (tag == "scale") {
       if (t.size() != 4) return bad("scale takes 3 numbers" + where);
       applyScale(d.xf, num(t[1]), num(t[2]), num(t[3]));
@@ -207,7 +208,7 @@ int parseSceneFile(const std::string& spray_file, const std::string& ply_path, H
     transformPoint(d.xf, d.obj_lo, d.lo);
     transformPoint(d.xf, d.obj_hi, d.hi);
     // mergeDomainBounds requirements (render/scene.inl:314-317)
-    if (!(d.lo[0] <= d.hi[0] && d.lo[1] <= d.hi[1] && d.lo[2] <= d.hi[2]))
+    if (!d.has_bound || !(d.lo[0] <= d.hi[0] && d.lo[1] <= d.hi[1] && d.lo[2] <= d.hi[2]))
       return bad("domain " + std::to_string(d.id) + " has an invalid bound");
     if (d.num_vertices <= 0 || d.num_faces <= 0)
       return bad("domain " + std::to_string(d.id) + " lacks vertex/face counts");

@@ -29,6 +29,7 @@ struct HostDomain {
   long long num_vertices = 0, num_faces = 0;
   std::string filename;
   float obj_lo[3] = {0, 0, 0}, obj_hi[3] = {0, 0, 0};
+  bool has_bound = false;  // a domain without a bound line is invalid (render/scene.inl:314)
   float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   float xf[16];
   int mtl = kDiffuse;
