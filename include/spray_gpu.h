@@ -189,6 +189,11 @@ int spray_mlp_logits(spray_ctx* ctx, const float* samples, int n, const float* p
 int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const char* name,
                          const char* extra_lines, long long* triangles_out);
 
+/* HdrImage::writePpm (display/image.h:183-204): writes a W*H*4 rgba float
+ * image (as returned by spray_trace_frame) as an ASCII P3 file with maximum
+ * value 1023, top row first.  Host only. */
+int spray_write_ppm(const float* rgba, int w, int h, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,6 +112,7 @@ PROTOTYPES = {
     "spray_mlp_num_params": (C.c_int, [_P]),
     "spray_mlp_grad": (C.c_int, [_P, _FP, C.c_int, _FP, _FP, _FP]),
     "spray_mlp_logits": (C.c_int, [_P, _FP, C.c_int, _FP, _FP]),
+    "spray_write_ppm": (C.c_int, [_FP, C.c_int, C.c_int, C.c_char_p]),
     "spray_scenegen_write": (C.c_int, [C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_longlong)]),
 }
 

@@ -54,6 +54,17 @@ def scenegen_write(directory, name, nx, ny, nz, res, a=6.0, b=17.0, phase=0.0, r
     return tris.value
 
 
+def write_ppm(path, rgba):
+    """Writes an (h, w, 4) float image as the reference's P3 / 1023 PPM."""
+    import numpy as np
+    lib = L.load()
+    a = np.ascontiguousarray(rgba, np.float32)
+    h, w = a.shape[:2]
+    rc = lib.spray_write_ppm(_fp(a), w, h, str(path).encode())
+    if rc != L.SPRAY_OK:
+        raise SprayError(rc, lib.spray_last_error(None).decode())
+
+
 class Context:
     """One rank's tracer (spray_ctx)."""
 

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -415,6 +416,40 @@ int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const 
     return rc;
   }
   if (triangles_out) *triangles_out = tris;
+  return SPRAY_OK;
+}
+
+// HdrImage::writePpm (display/image.h:183-204): ASCII P3, maximum value
+// 1023, rows written from the top (y = h-1) down, each channel
+// clamp(v * 1023, 0, 1023) truncated to an integer.
+int spray_write_ppm(const float* rgba, int w, int h, const char* path) {
+  g_create_err.clear();
+  if (!rgba || !path || w <= 0 || h <= 0) {
+    g_create_err = "invalid image or path";
+    return SPRAY_ERR_INVALID;
+  }
+  FILE* f = std::fopen(path, "w");
+  if (!f) {
+    g_create_err = std::string("unable to open ") + path;
+    return SPRAY_ERR_IO;
+  }
+  std::fprintf(f, "P3\n%d %d\n1023\n", w, h);
+  for (int y = h - 1; y >= 0; --y) {
+    for (int x = 0; x < w; ++x) {
+      const float* px = rgba + 4 * ((size_t)y * w + x);
+      unsigned c[3];
+      for (int k = 0; k < 3; ++k) {
+        float v = px[k] * 1023.f;
+        v = std::fmin(std::fmax(v, 0.0f), 1023.0f);  // NaN becomes 0
+        c[k] = (unsigned)v;
+      }
+      std::fprintf(f, "%u %u %u\n", c[0], c[1], c[2]);
+    }
+  }
+  if (std::fclose(f) != 0) {
+    g_create_err = std::string("write failed: ") + path;
+    return SPRAY_ERR_IO;
+  }
   return SPRAY_OK;
 }
 

@@ -166,3 +166,25 @@ def test_scenegen_bad_arguments():
         sa.scenegen_write("/nonexistent_dir_for_test", "x", 1, 1, 1, 8)
     with pytest.raises(sa.SprayError):
         sa.scenegen_write("/tmp", "x", 0, 1, 1, 8)
+
+
+def test_write_ppm_matches_reference_format(tmp_path):
+    """display/image.h:183-204: P3, max 1023, top row first, values
+    clamp(v * 1023, 0, 1023) truncated."""
+    import spray_amd as sa
+    img = np.zeros((2, 3, 4), np.float32)
+    img[0, 0, :3] = [0.0, 0.5, 1.0]        # bottom-left
+    img[0, 2, :3] = [2.0, -1.0, 0.9999]     # bottom-right: clamped
+    img[1, 1, :3] = [0.25, 1e-4, 0.001]     # top-middle
+    img[1, 2, 0] = np.nan
+    p = tmp_path / "o.ppm"
+    sa.write_ppm(p, img)
+    lines = p.read_text().split("\n")
+    assert lines[:3] == ["P3", "3 2", "1023"]
+    px = [tuple(int(x) for x in l.split()) for l in lines[3:] if l]
+    assert len(px) == 6
+    # top row (y = 1) first
+    assert px[0] == (0, 0, 0) and px[1] == (255, 0, 1) and px[2] == (0, 0, 0)
+    assert px[3] == (0, 511, 1023) and px[4] == (0, 0, 0) and px[5] == (1023, 0, 1022)
+    with pytest.raises(sa.SprayError):
+        sa.write_ppm("/nonexistent_dir_for_test/x.ppm", img)
