@@ -612,19 +612,31 @@ __global__ void __launch_bounds__(kQBlock) kTraceRays(
   LaneStack st;
   long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   st.init(ring, ovf + gtid * kOverflow);
-  for (long long i = gtid; i < n; i += (long long)gridDim.x * blockDim.x) {
-    float o[3] = {org[i * 3], org[i * 3 + 1], org[i * 3 + 2]};
-    float d[3] = {dir[i * 3], dir[i * 3 + 1], dir[i * 3 + 2]};
-    if (mode == 0) {
-      float t, u, v;
-      int p;
-      closestHit<false>(nodes, tris, o, d, tnear[i], tfar[i], max_nodes, st, &t, &u, &v, &p, nullptr);
-      t_o[i] = t;
-      u_o[i] = u;
-      v_o[i] = v;
-      prim_o[i] = p;
-    } else {
-      prim_o[i] = anyHit<false>(nodes, tris, o, d, tnear[i], tfar[i], max_nodes, st, nullptr) ? 1 : 0;
+  TravCount tc;
+  TravState s;
+  // the loop bound is uniform per block, so every wave runs the phases whole
+  for (long long base = blockIdx.x * (long long)blockDim.x; base < n;
+       base += (long long)gridDim.x * blockDim.x) {
+    const long long i = base + threadIdx.x;
+    const bool active = i < n;
+    if (active) {
+      float o[3] = {org[i * 3], org[i * 3 + 1], org[i * 3 + 2]};
+      float d[3] = {dir[i * 3], dir[i * 3 + 1], dir[i * 3 + 2]};
+      travInit(s, nodes, tris, max_nodes, o, d, tnear[i], tfar[i], mode != 0, st);
+    }
+    while (__ballot(active && !travDone(s)) != 0ull) {
+      travNodePhase(s, st, tc, active);
+      travLeafPhase(s, st, tc, active);
+    }
+    if (active) {
+      if (mode == 0) {
+        t_o[i] = s.prim >= 0 ? s.tlim : s.tfar;
+        u_o[i] = s.u;
+        v_o[i] = s.v;
+        prim_o[i] = s.prim;
+      } else {
+        prim_o[i] = s.prim > 0 ? 1 : 0;
+      }
     }
   }
   if (st.overflowed) atomicExch(err, 1);

@@ -763,7 +763,9 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
       if (exhausted) break;
       continue;
     }
-    if (active && travStep(s, st, tc)) {
+    travNodePhase(s, st, tc, active);
+    travLeafPhase(s, st, tc, active);
+    if (active && travDone(s)) {
       active = false;
       if (kind == 0) {
         bool upd = s.prim >= 0 && vbufUpdate(P.vbuf, samid, hist, depth, s.tlim);

@@ -95,147 +95,18 @@ __device__ __forceinline__ int nodeBoxes(const BNode4* __restrict__ nd, const fl
   return mask;
 }
 
-// Closest hit in (tnear, tfar): on return prim >= 0 on a hit with t/u/v set;
-// equal t resolves to the lower primitive id.
-template <bool kCount>
-__device__ __forceinline__ void closestHit(const BNode4* __restrict__ nodes,
-                                           const LTri* __restrict__ tris, const float org[3],
-                                           const float dir[3], float tnear, float tfar,
-                                           uint32_t max_nodes, LaneStack& st, float* t_o,
-                                           float* u_o, float* v_o, int* prim_o, TravCount* cnt) {
-  RayPre rp = rayPre(dir);
-  float inv[3] = {1.0f / dir[0], 1.0f / dir[1], 1.0f / dir[2]};
-  float tbest = tfar, ub = 0.f, vb = 0.f;
-  int pbest = -1;
-  st.reset();
-  uint32_t cur = 0;
-  uint32_t guard = 0;
-  while (cur != kStackEmpty) {
-    // a ray visits each node of a tree at most once; more visits mean a corrupt tree
-    if (++guard > max_nodes) {
-      st.overflowed = true;
-      break;
-    }
-    float tmin[4];
-    int child[4];
-    if (kCount) cnt->nodes++;
-    int mask = nodeBoxes(&nodes[cur], org, inv, tnear, tbest, tmin, child);
-    // leaves in slot order
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (!((mask >> i) & 1) || child[i] >= 0) continue;
-      int first = child[i] & 0x00ffffff;
-      int n = (child[i] >> 24) & 0x7f;
-      for (int k = 0; k < n; ++k) {
-        const float4* tq = (const float4*)&tris[first + k];
-        float4 a = tq[0], b = tq[1], c = tq[2];
-        float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
-        int p = __float_as_int(a.w);
-        float t, u, v;
-        if (kCount) cnt->tris++;
-        if (triHit(rp, org, v0, v1, v2, tnear, tfar, &t, &u, &v)) {
-          if (pbest < 0 || t < tbest || (t == tbest && p < pbest)) {
-            tbest = t;
-            ub = u;
-            vb = v;
-            pbest = p;
-          }
-        }
-      }
-    }
-    // inner children, nearest first
-    int ids[4];
-    float ts[4];
-    int m = 0;
-    float lim = tbest * kSlabInflate;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (((mask >> i) & 1) && child[i] >= 0 && tmin[i] <= lim) {
-        ids[m] = child[i];
-        ts[m] = tmin[i];
-        ++m;
-      }
-    }
-    // insertion sort by entry distance (m <= 4)
-    for (int i = 1; i < m; ++i) {
-      int id = ids[i];
-      float t = ts[i];
-      int j = i - 1;
-      while (j >= 0 && ts[j] > t) {
-        ids[j + 1] = ids[j];
-        ts[j + 1] = ts[j];
-        --j;
-      }
-      ids[j + 1] = id;
-      ts[j + 1] = t;
-    }
-    if (m == 0) {
-      cur = st.pop();
-    } else {
-      for (int i = m - 1; i >= 1; --i) st.push((uint32_t)ids[i]);
-      cur = (uint32_t)ids[0];
-    }
-  }
-  *t_o = pbest >= 0 ? tbest : tfar;
-  *u_o = ub;
-  *v_o = vb;
-  *prim_o = pbest;
-}
-
-template <bool kCount>
-__device__ __forceinline__ bool anyHit(const BNode4* __restrict__ nodes,
-                                       const LTri* __restrict__ tris, const float org[3],
-                                       const float dir[3], float tnear, float tfar,
-                                       uint32_t max_nodes, LaneStack& st, TravCount* cnt) {
-  RayPre rp = rayPre(dir);
-  float inv[3] = {1.0f / dir[0], 1.0f / dir[1], 1.0f / dir[2]};
-  st.reset();
-  uint32_t cur = 0;
-  uint32_t guard = 0;
-  while (cur != kStackEmpty) {
-    if (++guard > max_nodes) {
-      st.overflowed = true;
-      break;
-    }
-    float tmin[4];
-    int child[4];
-    if (kCount) cnt->nodes++;
-    int mask = nodeBoxes(&nodes[cur], org, inv, tnear, tfar, tmin, child);
-    for (int i = 0; i < 4; ++i) {
-      if (!((mask >> i) & 1) || child[i] >= 0) continue;
-      int first = child[i] & 0x00ffffff;
-      int n = (child[i] >> 24) & 0x7f;
-      for (int k = 0; k < n; ++k) {
-        const float4* tq = (const float4*)&tris[first + k];
-        float4 a = tq[0], b = tq[1], c = tq[2];
-        float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
-        float t, u, v;
-        if (kCount) cnt->tris++;
-        if (triHit(rp, org, v0, v1, v2, tnear, tfar, &t, &u, &v)) return true;
-      }
-    }
-    uint32_t next = kStackEmpty;
-    for (int i = 0; i < 4; ++i) {
-      if (((mask >> i) & 1) && child[i] >= 0) {
-        if (next == kStackEmpty)
-          next = (uint32_t)child[i];
-        else
-          st.push((uint32_t)child[i]);
-      }
-    }
-    cur = (next != kStackEmpty) ? next : st.pop();
-  }
-  return false;
-}
-
 }  // namespace spray
 
 namespace spray {
 
-// Resumable traversal of one ray (closest or any hit): one call of
-// travStep() processes one node, so a wave can refill lanes whose ray has
-// finished while the others keep traversing (Aila & Laine 2009,
-// "persistent while-while" with per-lane replacement).
+// Resumable traversal of one ray (closest or any hit) in two wave-wide
+// phases (Aila & Laine 2009, "while-while" with speculative traversal):
+// in the node phase every lane descends inner nodes; a lane that reaches a
+// leaf postpones it (one slot, `pend`) and keeps descending while other
+// lanes of its wave have not found a leaf yet.  In the leaf phase every lane
+// tests its postponed leaf and then the leaves at the top of its stack.
+// Node refs and leaf refs share the stack: a leaf ref is the child word of
+// its parent (bit 31, count << 24, first triangle).
 struct TravState {
   const BNode4* nodes;
   const LTri* tris;
@@ -244,9 +115,12 @@ struct TravState {
   float tnear, tfar, tlim;  // tlim: current best t (closest) or tfar (any)
   float u, v;
   int prim;
-  uint32_t cur, guard, max_nodes;
+  uint32_t cur, pend, guard, max_nodes;
   bool any;
 };
+
+__device__ __forceinline__ bool isInnerRef(uint32_t r) { return (int32_t)r >= 0; }
+__device__ __forceinline__ bool isLeafRef(uint32_t r) { return (int32_t)r < 0 && r != kStackEmpty; }
 
 __device__ __forceinline__ void travInit(TravState& s, const BNode4* nodes, const LTri* tris,
                                          uint32_t max_nodes, const float org[3],
@@ -267,10 +141,15 @@ __device__ __forceinline__ void travInit(TravState& s, const BNode4* nodes, cons
   s.u = s.v = 0.f;
   s.prim = -1;
   s.cur = 0;
+  s.pend = kStackEmpty;
   s.guard = 0;
   s.max_nodes = max_nodes;
   s.any = any;
   st.reset();
+}
+
+__device__ __forceinline__ bool travDone(const TravState& s) {
+  return s.cur == kStackEmpty && s.pend == kStackEmpty;
 }
 
 __device__ __forceinline__ void cswap(float& ka, int& ia, float& kb, int& ib) {
@@ -283,58 +162,33 @@ __device__ __forceinline__ void cswap(float& ka, int& ia, float& kb, int& ib) {
   ia = ti;
 }
 
-// Processes one node; returns true when the ray is finished (stack empty,
-// any-hit found, or the visit bound exceeded -> st.overflowed).
-__device__ __forceinline__ bool travStep(TravState& s, LaneStack& st, TravCount& cnt) {
-  if (++s.guard > s.max_nodes) {
+// One inner node: the hit children (inner nodes and leaves) in order of
+// entry distance; the nearest becomes `cur`, the others are pushed.
+__device__ __forceinline__ void travNode(TravState& s, LaneStack& st, TravCount& cnt) {
+  if (++s.guard > s.max_nodes) {  // a ray visits a node at most once: corrupt tree
     st.overflowed = true;
-    return true;
+    s.cur = s.pend = kStackEmpty;
+    return;
   }
   float tmin[4];
   int child[4];
   cnt.nodes++;
   int mask = nodeBoxes(&s.nodes[s.cur], s.org, s.inv, s.tnear, s.tlim, tmin, child);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    if (!((mask >> i) & 1) || child[i] >= 0) continue;
-    int first = child[i] & 0x00ffffff;
-    int n = (child[i] >> 24) & 0x7f;
-    for (int k = 0; k < n; ++k) {
-      const float4* tq = (const float4*)&s.tris[first + k];
-      float4 a = tq[0], b = tq[1], c = tq[2];
-      float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
-      float t, u, v;
-      cnt.tris++;
-      if (triHit(s.rp, s.org, v0, v1, v2, s.tnear, s.any ? s.tlim : s.tfar, &t, &u, &v)) {
-        if (s.any) {
-          s.prim = 1;
-          return true;
-        }
-        int p = __float_as_int(a.w);
-        if (s.prim < 0 || t < s.tlim || (t == s.tlim && p < s.prim)) {
-          s.tlim = t;
-          s.u = u;
-          s.v = v;
-          s.prim = p;
-        }
-      }
-    }
-  }
-  // inner children, nearest first (sorting network on 4 keys; no local arrays)
-  const float lim = s.tlim * kSlabInflate;
-  float k0 = 3.4e38f, k1 = 3.4e38f, k2 = 3.4e38f, k3 = 3.4e38f;
-  float ks[4];
+  float k0, k1, k2, k3;
   int m = 0;
+  {
+    float ks[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    bool ok = ((mask >> i) & 1) && child[i] >= 0 && tmin[i] <= lim;
-    ks[i] = ok ? fminf(tmin[i], 3.0e38f) : kInfF;
-    m += ok ? 1 : 0;
+    for (int i = 0; i < 4; ++i) {
+      bool ok = (mask >> i) & 1;
+      ks[i] = ok ? fminf(tmin[i], 3.0e38f) : kInfF;
+      m += ok ? 1 : 0;
+    }
+    k0 = ks[0];
+    k1 = ks[1];
+    k2 = ks[2];
+    k3 = ks[3];
   }
-  k0 = ks[0];
-  k1 = ks[1];
-  k2 = ks[2];
-  k3 = ks[3];
   int i0 = child[0], i1 = child[1], i2 = child[2], i3 = child[3];
   cswap(k0, i0, k1, i1);
   cswap(k2, i2, k3, i3);
@@ -343,13 +197,79 @@ __device__ __forceinline__ bool travStep(TravState& s, LaneStack& st, TravCount&
   cswap(k1, i1, k2, i2);
   if (m == 0) {
     s.cur = st.pop();
-    return s.cur == kStackEmpty;
+    return;
   }
   if (m > 3) st.push((uint32_t)i3);
   if (m > 2) st.push((uint32_t)i2);
   if (m > 1) st.push((uint32_t)i1);
   s.cur = (uint32_t)i0;
+}
+
+// Tests the triangles of one leaf; returns true when an any-hit query is answered.
+__device__ __forceinline__ bool travLeaf(TravState& s, uint32_t leaf, TravCount& cnt) {
+  const int first = (int)(leaf & 0x00ffffffu);
+  const int n = (int)((leaf >> 24) & 0x7fu);
+  for (int k = 0; k < n; ++k) {
+    const float4* tq = (const float4*)&s.tris[first + k];
+    float4 a = tq[0], b = tq[1], c = tq[2];
+    float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
+    float t, u, v;
+    cnt.tris++;
+    if (triHit(s.rp, s.org, v0, v1, v2, s.tnear, s.any ? s.tlim : s.tfar, &t, &u, &v)) {
+      if (s.any) {
+        s.prim = 1;
+        return true;
+      }
+      // closest hit; an exact tie in t goes to the lower primitive id
+      int p = __float_as_int(a.w);
+      if (s.prim < 0 || t < s.tlim || (t == s.tlim && p < s.prim)) {
+        s.tlim = t;
+        s.u = u;
+        s.v = v;
+        s.prim = p;
+      }
+    }
+  }
   return false;
+}
+
+// Node phase over the wave: lanes with an inner `cur` descend until every
+// busy lane holds a postponed leaf (or none can descend further).
+__device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravCount& cnt,
+                                              bool active) {
+  while (true) {
+    if (active && isInnerRef(s.cur)) {
+      travNode(s, st, cnt);
+      if (isLeafRef(s.cur) && s.pend == kStackEmpty) {
+        s.pend = s.cur;
+        s.cur = st.pop();
+      }
+    }
+    bool more = active && isInnerRef(s.cur);
+    bool hungry = active && s.pend == kStackEmpty && s.cur != kStackEmpty;
+    if (__ballot(more) == 0ull || __ballot(hungry) == 0ull) break;
+  }
+}
+
+// Leaf phase over the wave: postponed leaf first, then consecutive leaves
+// from the stack.
+__device__ __forceinline__ void travLeafPhase(TravState& s, LaneStack& st, TravCount& cnt,
+                                              bool active) {
+  while (true) {
+    bool leaf = active && (s.pend != kStackEmpty || isLeafRef(s.cur));
+    if (__ballot(leaf) == 0ull) break;
+    if (leaf) {
+      uint32_t L;
+      if (s.pend != kStackEmpty) {
+        L = s.pend;
+        s.pend = kStackEmpty;
+      } else {
+        L = s.cur;
+        s.cur = st.pop();
+      }
+      if (travLeaf(s, L, cnt)) s.cur = s.pend = kStackEmpty;
+    }
+  }
 }
 
 }  // namespace spray
