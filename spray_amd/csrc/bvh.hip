@@ -610,8 +610,7 @@ __global__ void __launch_bounds__(kQBlock) kTraceRays(
     uint32_t* ovf, int* err) {
   __shared__ uint32_t ring[kRing * kQBlock];
   LaneStack st;
-  long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  st.init(ring, ovf + gtid * kOverflow);
+  st.init(ring, ovf);
   TravCount tc;
   TravState s;
   // the loop bound is uniform per block, so every wave runs the phases whole
@@ -622,15 +621,15 @@ __global__ void __launch_bounds__(kQBlock) kTraceRays(
     if (active) {
       float o[3] = {org[i * 3], org[i * 3 + 1], org[i * 3 + 2]};
       float d[3] = {dir[i * 3], dir[i * 3 + 1], dir[i * 3 + 2]};
-      travInit(s, nodes, tris, max_nodes, o, d, tnear[i], tfar[i], mode != 0, st);
+      travInit(s, 0u, 0u, o, d, tnear[i], tfar[i], mode != 0, st);
     }
     while (__ballot(active && !travDone(s)) != 0ull) {
-      travNodePhase(s, st, tc, active);
-      travLeafPhase(s, st, tc, active);
+      travNodePhase(s, st, tc, active, nodes);
+      travLeafPhase(s, st, tc, active, tris);
     }
     if (active) {
       if (mode == 0) {
-        t_o[i] = s.prim >= 0 ? s.tlim : s.tfar;
+        t_o[i] = s.tlim;
         u_o[i] = s.u;
         v_o[i] = s.v;
         prim_o[i] = s.prim;
@@ -639,7 +638,7 @@ __global__ void __launch_bounds__(kQBlock) kTraceRays(
       }
     }
   }
-  if (st.overflowed) atomicExch(err, 1);
+  if (st.overflowed()) atomicExch(err, 1);
 }
 
 void traceRaysDevice(Ctx* c, const float* d_org, const float* d_dir, const float* d_tnear,

@@ -379,7 +379,10 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
         if (dd == spawn || !slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
         uint32_t slot = lb[kind * nd + dd] + atomicAdd(&lc[kind * nd + dd], 1u);
         uint32_t dst = P.dbase[kind * nd + dd] + slot;
-        if (dst < P.q_cap[kind]) P.q[kind][dst] = QEntry{r, tmin};
+        if (dst < P.q_cap[kind]) {
+          P.q[kind][dst] = QEntry{r, tmin};
+          P.qdom[kind][dst] = (uint16_t)dd;
+        }
       }
     }
     __syncthreads();
@@ -626,24 +629,11 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
 //     not yet occluded and still on the chain; any hit sets the flag.
 // A wave refetches entries for its idle lanes with one atomic, so long
 // traversals do not hold up the rest of the wave.
-struct LaneCounts {
-  unsigned long long rvis = 0, svis = 0, shades = 0, pruned = 0, nodes = 0, tris = 0;
-};
-
 constexpr int kTravBlock = 256;
 
-__device__ __forceinline__ uint32_t findDomain(const TraceParams& P, int kind, uint32_t q) {
-  // largest k with off[k] <= q, over the schedule-ordered offsets
-  const uint32_t* off = P.off + kind * (P.ndomains + 1);
-  int lo = 0, hi = P.ndomains - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= q)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return (uint32_t)P.order[lo];
+// Adds a wave-uniform count once per wave.
+__device__ __forceinline__ void flushWave(unsigned long long* dst, unsigned long long v) {
+  if (__lane_id() == 0 && v) atomicAdd(dst, v);
 }
 
 __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain_dom, int phase) {
@@ -689,16 +679,14 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   }
   const uint32_t total = nr + ns;
   LaneStack st;
-  long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  st.init(ring, P.ovf + gtid * kOverflow);
-  LaneCounts lc;
+  st.init(ring, P.ovf);
   TravCount tc;
   TravState s;
+  unsigned long long rvis = 0, svis = 0, pruned = 0;  // wave-uniform
   const int lane = __lane_id();
   bool active = false, exhausted = false;
-  int kind = 0, dom = 0, depth = 0, samid = 0;
+  int kind = 0, dom = 0;
   uint32_t ray = 0;
-  float hist[4] = {kInfF, kInfF, kInfF, kInfF};
   while (true) {
     unsigned long long idle = __ballot(!active);
     int nidle = __popcll(idle);
@@ -708,9 +696,11 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
       if (lane == leader) base = atomicAdd(wc, (uint32_t)nidle);
       base = __shfl(base, leader, 64);
       if (base + (uint32_t)nidle >= total) exhausted = true;
+      bool go = false, fetched = false;
       if (!active) {
         uint32_t e = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         if (e < total) {
+          fetched = true;
           kind = e < nr ? 0 : 1;
           QEntry qe;
           if (local) {
@@ -718,19 +708,15 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
           } else {
             uint32_t q = kind == 0 ? (rlist ? rlist[e] : rbase + e) : sbase + (e - nr);
             qe = P.q[kind][q];
-            dom = plain_dom >= 0 ? plain_dom : (int)findDomain(P, kind, q);
+            dom = plain_dom >= 0 ? plain_dom : (int)P.qdom[kind][q];
           }
           ray = qe.ray;
           const RayPool& pool = kind ? P.S : P.R;
           float4 op = pool.op[ray], ds = pool.ds[ray], wd = pool.wd[ray], hh = pool.h[ray];
           if (local) dom = __float_as_int(wd.w) >> 8;
-          hist[0] = hh.x;
-          hist[1] = hh.y;
-          hist[2] = hh.z;
-          hist[3] = hh.w;
-          depth = __float_as_int(wd.w) & 0xff;
-          samid = __float_as_int(ds.w);
-          bool go;
+          float hist[4] = {hh.x, hh.y, hh.z, hh.w};
+          const int depth = __float_as_int(wd.w) & 0xff;
+          const int samid = __float_as_int(ds.w);
           float tfar = kInfF;
           if (kind == 0) {
             go = qe.tdom <= pick4(hist, depth) && vbufCorrect(P.vbuf, samid, hist, depth);
@@ -746,28 +732,30 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
           if (go) {
             const DomainDesc& D = P.desc[dom];
             float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
-            travInit(s, P.nodes + D.node_off, P.tris + D.tri_off, (uint32_t)D.num_nodes, o, d,
-                     kRayEps, tfar, kind == 1, st);
+            travInit(s, (uint32_t)D.node_off, (uint32_t)D.tri_off, o, d, kRayEps, tfar, kind == 1,
+                     st);
             active = true;
-            if (kind == 0)
-              lc.rvis++;
-            else
-              lc.svis++;
-          } else {
-            lc.pruned++;
           }
         }
       }
+      rvis += (unsigned long long)__popcll(__ballot(go && kind == 0));
+      svis += (unsigned long long)__popcll(__ballot(go && kind == 1));
+      pruned += (unsigned long long)__popcll(__ballot(fetched && !go));
     }
     if (__ballot(active) == 0ull) {
       if (exhausted) break;
       continue;
     }
-    travNodePhase(s, st, tc, active);
-    travLeafPhase(s, st, tc, active);
+    travNodePhase(s, st, tc, active, P.nodes);
+    travLeafPhase(s, st, tc, active, P.tris);
     if (active && travDone(s)) {
       active = false;
       if (kind == 0) {
+        // the history prefix the buffer update checks is fixed for this pass
+        const float4 wd = P.R.wd[ray], hh = P.R.h[ray];
+        const int depth = __float_as_int(wd.w) & 0xff;
+        const int samid = __float_as_int(P.R.ds[ray].w);
+        float hist[4] = {hh.x, hh.y, hh.z, hh.w};
         bool upd = s.prim >= 0 && vbufUpdate(P.vbuf, samid, hist, depth, s.tlim);
         if (P.log_on) {  // visit log for the predictor: (ray, domain, t)
           uint32_t i = atomicAdd(&c->vlog_count, 1u);
@@ -792,12 +780,12 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
       }
     }
   }
-  flushCount(&c->radiance_visits, lc.rvis);
-  flushCount(&c->shadow_visits, lc.svis);
-  flushCount(&c->pruned, lc.pruned);
-  flushCount(&c->node_visits, tc.nodes);
+  flushWave(&c->radiance_visits, rvis);
+  flushWave(&c->shadow_visits, svis);
+  flushWave(&c->pruned, pruned);
+  flushWave(&c->node_visits, tc.nodes);
   flushCount(&c->tri_tests, tc.tris);
-  if (st.overflowed) atomicOr(&c->err, 4u);
+  if (st.overflowed()) atomicOr(&c->err, 4u);
 }
 
 // Shading after the trace step: every hit whose history entry is still the
@@ -939,7 +927,8 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, cursor,
-      wctr, score, order, q[2], ovf, phase_b, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa, qb;
+      wctr, score, order, q[2], qdom[2], ovf, phase_b, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa,
+      qb;
   unsigned long long log_cap = 0;
   bool logging = false;
   DevBuf log_ctr;
@@ -990,7 +979,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->cursor, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->ovf, &T->phase_b, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
+                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->ovf, &T->phase_b, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
                     &T->sh_tuv, &T->sh_prim, &T->log_ctr, &T->qa, &T->qb})
     b->release();
   delete T;
@@ -1074,6 +1063,8 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->order, (size_t)nd * 4 + 64);
   need(T, T->q[0], (size_t)T->q_cap[0] * sizeof(QEntry));
   need(T, T->q[1], (size_t)T->q_cap[1] * sizeof(QEntry));
+  need(T, T->qdom[0], (size_t)T->q_cap[0] * 2 + 64);
+  need(T, T->qdom[1], (size_t)T->q_cap[1] * 2 + 64);
   need(T, T->phase_b, (size_t)T->q_cap[0] + 64);
   T->h_cnt.resize(2 * nd);
   T->h_order.resize(nd);
@@ -1158,6 +1149,8 @@ static TraceParams makeParams(Ctx* c) {
   P.order = T->order.as<int>();
   P.q[0] = T->q[0].as<QEntry>();
   P.q[1] = T->q[1].as<QEntry>();
+  P.qdom[0] = T->qdom[0].as<uint16_t>();
+  P.qdom[1] = T->qdom[1].as<uint16_t>();
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
   P.refill = std::min(64, std::max(1, c->batch_chunk));

@@ -78,6 +78,7 @@ struct TraceParams {
   unsigned long long* score;  // [nd]
   int* order;         // [nd] schedule
   QEntry* q[2];       // queues (radiance, shadow)
+  uint16_t* qdom[2];  // domain of each queue entry
   uint32_t q_cap[2];
   int refill;         // a wave refetches work once this many lanes are idle
   uint32_t* ovf;      // traversal stack overflow areas

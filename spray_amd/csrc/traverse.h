@@ -1,7 +1,9 @@
 // Closest-hit and any-hit traversal of one domain's 4-wide BVH.
 //
-// Per-lane short stack: a 16-entry ring in LDS (slot s of lane t at
-// lds[s * blockDim.x + t], so a wave's pushes hit 64 consecutive banks);
+// Per-lane short stack: a 16-entry ring in LDS.  Each wave owns kRing x 64
+// words (slot s of lane l of wave w at lds[(w * kRing + s) * 64 + l], so a
+// wave's pushes hit 64 consecutive banks and the address needs no block
+// size);
 // when the ring is full its bottom entry spills to a per-lane overflow
 // stack in HBM, which is popped only once the ring is empty.  The overflow
 // depth bounds the BVH depth; the build rejects deeper trees.
@@ -17,51 +19,65 @@ constexpr int kOverflow = 112;
 constexpr uint32_t kStackEmpty = 0xffffffffu;
 // robust slab test (Ize, "Robust BVH Ray Traversal", JCGT 2013): 1 + 2*gamma(3)
 constexpr float kSlabInflate = 1.0000003576278687f;
+// a ray visits each node of a tree at most once; the bound only stops a
+// corrupt (cyclic) tree from spinning forever
+constexpr uint32_t kMaxNodeVisits = 1u << 22;
 
+// The stack state is one packed word per lane: ring head (bits 0-3), ring
+// fill (4-8), overflow fill (9-15), overflow error (16).
 struct LaneStack {
-  uint32_t* lds;  // this block's ring base
-  uint32_t* ovf;  // this lane's overflow area (kOverflow entries)
-  int head, count, spilled;
-  bool overflowed;
-  __device__ void init(uint32_t* lds_base, uint32_t* ovf_lane) {
+  uint32_t* lds;       // this block's ring base
+  uint32_t* ovf;       // overflow areas of the grid (kOverflow words per thread)
+  uint32_t w;
+  __device__ void init(uint32_t* lds_base, uint32_t* ovf_base) {
     lds = lds_base;
-    ovf = ovf_lane;
-    head = 0;
-    count = 0;
-    spilled = 0;
-    overflowed = false;
+    ovf = ovf_base;
+    w = 0;
   }
-  __device__ void reset() {
-    head = 0;
-    count = 0;
-    spilled = 0;
+  __device__ void reset() { w &= 1u << 16; }
+  __device__ bool overflowed() const { return (w >> 16) & 1u; }
+  __device__ uint32_t* lane_ovf() const {  // only on the rare spill path
+    return ovf + (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * kOverflow;
+  }
+  __device__ uint32_t& slot(uint32_t s) const {
+    return lds[(((threadIdx.x >> 6) * kRing + s) << 6) + (threadIdx.x & 63)];
   }
   __device__ void push(uint32_t x) {
+    uint32_t head = w & 15u, count = (w >> 4) & 31u, spilled = (w >> 9) & 127u, err = w >> 16;
     if (count == kRing) {
-      uint32_t bottom = lds[head * blockDim.x + threadIdx.x];
-      if (spilled < kOverflow) {
-        ovf[spilled++] = bottom;
-      } else {
-        overflowed = true;  // reported to the host; never silently dropped
-      }
+      uint32_t bottom = slot(head);
+      if (spilled < kOverflow)
+        lane_ovf()[spilled++] = bottom;
+      else
+        err = 1;  // reported to the host; never silently dropped
       head = (head + 1) & (kRing - 1);
       --count;
     }
-    lds[((head + count) & (kRing - 1)) * blockDim.x + threadIdx.x] = x;
+    slot((head + count) & (kRing - 1)) = x;
     ++count;
+    w = head | (count << 4) | (spilled << 9) | (err << 16);
   }
   __device__ uint32_t pop() {
+    uint32_t head = w & 15u, count = (w >> 4) & 31u, spilled = (w >> 9) & 127u;
     if (count > 0) {
       --count;
-      return lds[((head + count) & (kRing - 1)) * blockDim.x + threadIdx.x];
+      w = (w & ~(31u << 4)) | (count << 4);
+      return slot((head + count) & (kRing - 1));
     }
-    if (spilled > 0) return ovf[--spilled];
+    if (spilled > 0) {
+      --spilled;
+      w = (w & ~(127u << 9)) | (spilled << 9);
+      return lane_ovf()[spilled];
+    }
     return kStackEmpty;
   }
 };
 
+// Work counters: node steps per wave (a sum of ballots, kept in scalar
+// registers) and triangle tests per lane.
 struct TravCount {
-  uint32_t nodes = 0, tris = 0;
+  unsigned long long nodes = 0;
+  uint32_t tris = 0;
 };
 
 // Tests the 4 child boxes of a node against [tnear, tfar]; returns a 4-bit
@@ -95,10 +111,6 @@ __device__ __forceinline__ int nodeBoxes(const BNode4* __restrict__ nd, const fl
   return mask;
 }
 
-}  // namespace spray
-
-namespace spray {
-
 // Resumable traversal of one ray (closest or any hit) in two wave-wide
 // phases (Aila & Laine 2009, "while-while" with speculative traversal):
 // in the node phase every lane descends inner nodes; a lane that reaches a
@@ -107,44 +119,46 @@ namespace spray {
 // tests its postponed leaf and then the leaves at the top of its stack.
 // Node refs and leaf refs share the stack: a leaf ref is the child word of
 // its parent (bit 31, count << 24, first triangle).
+//
+// Closest hit: `tlim` is the best t so far (initially the ray's tfar, which
+// is therefore also the reported t of a miss); an exact tie in t goes to the
+// lower primitive id.  Any hit: `tlim` stays tfar and prim becomes 1.
 struct TravState {
-  const BNode4* nodes;
-  const LTri* tris;
   float org[3], inv[3];
-  RayPre rp;
-  float tnear, tfar, tlim;  // tlim: current best t (closest) or tfar (any)
-  float u, v;
+  float Sx, Sy, Sz;
+  uint32_t kk;          // kx | ky << 2 | kz << 4 | any << 8
+  float tnear, tlim, u, v;
   int prim;
-  uint32_t cur, pend, guard, max_nodes;
-  bool any;
+  uint32_t cur, pend, guard;
+  uint32_t node_base, tri_base;  // the domain's first node / leaf-ordered triangle
 };
 
 __device__ __forceinline__ bool isInnerRef(uint32_t r) { return (int32_t)r >= 0; }
 __device__ __forceinline__ bool isLeafRef(uint32_t r) { return (int32_t)r < 0 && r != kStackEmpty; }
 
-__device__ __forceinline__ void travInit(TravState& s, const BNode4* nodes, const LTri* tris,
-                                         uint32_t max_nodes, const float org[3],
-                                         const float dir[3], float tnear, float tfar, bool any,
-                                         LaneStack& st) {
-  s.nodes = nodes;
-  s.tris = tris;
+__device__ __forceinline__ void travInit(TravState& s, uint32_t node_base, uint32_t tri_base,
+                                         const float org[3], const float dir[3], float tnear,
+                                         float tfar, bool any, LaneStack& st) {
   s.org[0] = org[0];
   s.org[1] = org[1];
   s.org[2] = org[2];
   s.inv[0] = 1.0f / dir[0];
   s.inv[1] = 1.0f / dir[1];
   s.inv[2] = 1.0f / dir[2];
-  s.rp = rayPre(dir);
+  RayPre rp = rayPre(dir);
+  s.Sx = rp.Sx;
+  s.Sy = rp.Sy;
+  s.Sz = rp.Sz;
+  s.kk = (uint32_t)rp.kx | ((uint32_t)rp.ky << 2) | ((uint32_t)rp.kz << 4) | (any ? 256u : 0u);
   s.tnear = tnear;
-  s.tfar = tfar;
   s.tlim = tfar;
   s.u = s.v = 0.f;
   s.prim = -1;
   s.cur = 0;
   s.pend = kStackEmpty;
   s.guard = 0;
-  s.max_nodes = max_nodes;
-  s.any = any;
+  s.node_base = node_base;
+  s.tri_base = tri_base;
   st.reset();
 }
 
@@ -164,31 +178,25 @@ __device__ __forceinline__ void cswap(float& ka, int& ia, float& kb, int& ib) {
 
 // One inner node: the hit children (inner nodes and leaves) in order of
 // entry distance; the nearest becomes `cur`, the others are pushed.
-__device__ __forceinline__ void travNode(TravState& s, LaneStack& st, TravCount& cnt) {
-  if (++s.guard > s.max_nodes) {  // a ray visits a node at most once: corrupt tree
-    st.overflowed = true;
+__device__ __forceinline__ void travNode(TravState& s, LaneStack& st,
+                                         const BNode4* __restrict__ nodes) {
+  if (++s.guard > kMaxNodeVisits) {
+    st.w |= 1u << 16;
     s.cur = s.pend = kStackEmpty;
     return;
   }
   float tmin[4];
   int child[4];
-  cnt.nodes++;
-  int mask = nodeBoxes(&s.nodes[s.cur], s.org, s.inv, s.tnear, s.tlim, tmin, child);
-  float k0, k1, k2, k3;
+  int mask = nodeBoxes(&nodes[s.node_base + s.cur], s.org, s.inv, s.tnear, s.tlim, tmin, child);
+  float ks[4];
   int m = 0;
-  {
-    float ks[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      bool ok = (mask >> i) & 1;
-      ks[i] = ok ? fminf(tmin[i], 3.0e38f) : kInfF;
-      m += ok ? 1 : 0;
-    }
-    k0 = ks[0];
-    k1 = ks[1];
-    k2 = ks[2];
-    k3 = ks[3];
+  for (int i = 0; i < 4; ++i) {
+    bool ok = (mask >> i) & 1;
+    ks[i] = ok ? fminf(tmin[i], 3.0e38f) : kInfF;
+    m += ok ? 1 : 0;
   }
+  float k0 = ks[0], k1 = ks[1], k2 = ks[2], k3 = ks[3];
   int i0 = child[0], i1 = child[1], i2 = child[2], i3 = child[3];
   cswap(k0, i0, k1, i1);
   cswap(k2, i2, k3, i3);
@@ -205,24 +213,31 @@ __device__ __forceinline__ void travNode(TravState& s, LaneStack& st, TravCount&
   s.cur = (uint32_t)i0;
 }
 
-// Tests the triangles of one leaf; returns true when an any-hit query is answered.
-__device__ __forceinline__ bool travLeaf(TravState& s, uint32_t leaf, TravCount& cnt) {
-  const int first = (int)(leaf & 0x00ffffffu);
+// Tests the triangles of one leaf; returns true when an any-hit query is
+// answered.  Returns the number of triangles tested in *ntested.
+__device__ __forceinline__ bool travLeaf(TravState& s, uint32_t leaf, const LTri* __restrict__ tris,
+                                         int* ntested) {
+  const uint32_t first = s.tri_base + (leaf & 0x00ffffffu);
   const int n = (int)((leaf >> 24) & 0x7fu);
+  *ntested = n;
+  const RayPre rp{(int)(s.kk & 3u), (int)((s.kk >> 2) & 3u), (int)((s.kk >> 4) & 3u), s.Sx, s.Sy,
+                  s.Sz};
+  const bool any = (s.kk >> 8) & 1u;
   for (int k = 0; k < n; ++k) {
-    const float4* tq = (const float4*)&s.tris[first + k];
+    const float4* tq = (const float4*)&tris[first + k];
     float4 a = tq[0], b = tq[1], c = tq[2];
     float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
+    // closest: accept t <= tlim once a hit exists (ties go to the lower id)
+    float ub = (any || s.prim < 0) ? s.tlim : __uint_as_float(__float_as_uint(s.tlim) + 1u);
     float t, u, v;
-    cnt.tris++;
-    if (triHit(s.rp, s.org, v0, v1, v2, s.tnear, s.any ? s.tlim : s.tfar, &t, &u, &v)) {
-      if (s.any) {
+    if (triHit(rp, s.org, v0, v1, v2, s.tnear, ub, &t, &u, &v)) {
+      if (any) {
         s.prim = 1;
+        *ntested = k + 1;
         return true;
       }
-      // closest hit; an exact tie in t goes to the lower primitive id
       int p = __float_as_int(a.w);
-      if (s.prim < 0 || t < s.tlim || (t == s.tlim && p < s.prim)) {
+      if (s.prim < 0 || t < s.tlim || p < s.prim) {
         s.tlim = t;
         s.u = u;
         s.v = v;
@@ -236,10 +251,12 @@ __device__ __forceinline__ bool travLeaf(TravState& s, uint32_t leaf, TravCount&
 // Node phase over the wave: lanes with an inner `cur` descend until every
 // busy lane holds a postponed leaf (or none can descend further).
 __device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravCount& cnt,
-                                              bool active) {
+                                              bool active, const BNode4* __restrict__ nodes) {
   while (true) {
-    if (active && isInnerRef(s.cur)) {
-      travNode(s, st, cnt);
+    const bool step = active && isInnerRef(s.cur);
+    cnt.nodes += (unsigned long long)__popcll(__ballot(step));
+    if (step) {
+      travNode(s, st, nodes);
       if (isLeafRef(s.cur) && s.pend == kStackEmpty) {
         s.pend = s.cur;
         s.cur = st.pop();
@@ -254,10 +271,11 @@ __device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravC
 // Leaf phase over the wave: postponed leaf first, then consecutive leaves
 // from the stack.
 __device__ __forceinline__ void travLeafPhase(TravState& s, LaneStack& st, TravCount& cnt,
-                                              bool active) {
+                                              bool active, const LTri* __restrict__ tris) {
   while (true) {
     bool leaf = active && (s.pend != kStackEmpty || isLeafRef(s.cur));
     if (__ballot(leaf) == 0ull) break;
+    int nt = 0;
     if (leaf) {
       uint32_t L;
       if (s.pend != kStackEmpty) {
@@ -267,8 +285,9 @@ __device__ __forceinline__ void travLeafPhase(TravState& s, LaneStack& st, TravC
         L = s.cur;
         s.cur = st.pop();
       }
-      if (travLeaf(s, L, cnt)) s.cur = s.pend = kStackEmpty;
+      if (travLeaf(s, L, tris, &nt)) s.cur = s.pend = kStackEmpty;
     }
+    cnt.tris += (uint32_t)nt;
   }
 }
 
