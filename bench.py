@@ -1,12 +1,17 @@
 """Benchmark: SpRay per-rank tracing on one MI355X.
 
-Workload (the second configuration of the benchmark set): 64 domains in a
-4x4x4 grid of unit cubes, each a marching-cubes isosurface of
-f = sin(6x)sin(6y)sin(6z) + 0.3 sin(17(x+y+z)) on a 64^3 grid (~91k
-triangles per domain, ~5.8M total), path tracing with albedo 0.7 (vertex
-colour 179), one rectangular area light, 1024x1024, 8 spp, depth 5, Russian
-roulette from bounce 3, fp32 rays.  A step is one frame (8 blocking tiles of
-1,048,576 samples).  Synthetic geometry generated in-process.
+Default workload (the second configuration of the benchmark set, "c2"): 64
+domains in a 4x4x4 grid of unit cubes, each a marching-cubes isosurface of
+f = sin(6x)sin(6y)sin(6z) + 0.3 sin(17(x+y+z)) on a 64^3 grid (~91k triangles
+per domain, ~5.8M total), path tracing with albedo 0.7 (vertex colour 179),
+one rectangular area light, 1024x1024, 8 spp, depth 5, Russian roulette from
+bounce 3, fp32 rays.  A step is one frame (8 blocking tiles of 1,048,576
+samples).  Synthetic geometry generated in-process.
+
+Other workloads (--workload): "c1" AO on the same geometry (512x512, 1 spp,
+8 AO rays, max distance 0.5) and "c3" path tracing on 256 domains in an
+8x8x4 grid with log-uniform per-domain triangle budgets (10k..2M, rescaled to
+~120M in total), 1920x1080, 16 spp, depth 6.
 
 Prints one JSON line.  `value` is Mrays/s of useful ray queries (eye +
 non-stale bounce + non-stale shadow rays, identical for every schedule) over
@@ -28,17 +33,43 @@ C2_CAM = dict(pos=[2.0, 2.0, 2.0 + 2.0 * math.sqrt(3.0)], lookat=[2.0, 2.0, 2.0]
 RECT_LIGHT = "light rect 0.5 4.5 0.5 3 0 0 0 0 3 4 4 4"
 HBM_PEAK_GBS = 8000.0
 
+# name -> scene generator arguments, render settings, camera (None: the
+# reference's default framing of the scene bound), CPU tiles timed by default
+WORKLOADS = {
+    "c1": dict(grid=(4, 4, 4), gen=dict(res=64, rgb=0xFFFFFF, extra_lines="light point 2 6 2 1 1 1"),
+               render=dict(image_w=512, image_h=512, pixel_samples=1, ao_mode=1, ao_samples=8, bounces=1,
+                           ao_max_dist=0.5),
+               cam=C2_CAM, cpu_tiles=1,
+               title="C1 ambient occlusion: 64 domains (4x4x4 MC isosurfaces, {tris} tris), 512x512, 1 spp, "
+                     "8 AO rays, max dist 0.5"),
+    "c2": dict(grid=(4, 4, 4), gen=dict(res=64, rgb=0xB3B3B3, extra_lines=RECT_LIGHT),
+               render=dict(image_w=1024, image_h=1024, pixel_samples=8, ao_mode=0, ao_samples=1, bounces=5,
+                           rr_start=3),
+               cam=C2_CAM, cpu_tiles=8,
+               title="C2 path tracing: 64 domains (4x4x4 MC isosurfaces, {tris} tris), {w}x{h}, {spp} spp, "
+                     "depth 5, RR>=3, rect light"),
+    "c3": dict(grid=(8, 8, 4), gen=dict(res=0, per_domain_random=True, seed=0xC3, tri_total=120e6, rgb=0xB3B3B3,
+                                        extra_lines="light rect 1 8.5 0.5 6 0 0 0 0 3 4 4 4"),
+               render=dict(image_w=1920, image_h=1080, pixel_samples=16, ao_mode=0, ao_samples=1, bounces=6,
+                           rr_start=3),
+               cam=None, cpu_tiles=1,
+               title="C3 path tracing: 256 domains (8x8x4 MC isosurfaces, log-uniform 10k-2M tris each, "
+                     "{tris} tris), {w}x{h}, {spp} spp, depth 6, RR>=3, rect light"),
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--spp", type=int, default=8)
+    ap.add_argument("--size", type=int, default=0, help="square image size (0: the workload's)")
+    ap.add_argument("--spp", type=int, default=0, help="samples per pixel (0: the workload's)")
     ap.add_argument("--schedules", default="plain,batched,learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
-    ap.add_argument("--cpu-tiles", type=int, default=8, help="blocking tiles timed on the CPU")
+    ap.add_argument("--cpu-tiles", type=int, default=0,
+                    help="blocking tiles timed on the CPU (0: the workload's default)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default batch parameters")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every pass directly (kernel-level profilers)")
@@ -79,19 +110,32 @@ def main():
     import numpy as np
     import spray_amd as sa
 
-    tmp = tempfile.mkdtemp(prefix="spray_c2_")
+    wl = WORKLOADS[args.workload]
+    tmp = tempfile.mkdtemp(prefix=f"spray_{args.workload}_")
     t0 = time.time()
-    tris = sa.scenegen_write(tmp, "c2", 4, 4, 4, 64, rgb=0xB3B3B3, extra_lines=RECT_LIGHT)
+    nx, ny, nz = wl["grid"]
+    g = dict(wl["gen"])
+    tris = sa.scenegen_write(tmp, args.workload, nx, ny, nz, g.pop("res"), **g)
     gen_s = time.time() - t0
-    W = H = args.size
-    kw = dict(image_w=W, image_h=H, pixel_samples=args.spp, ao_mode=0, ao_samples=1, bounces=5,
-              rr_start=3)
-    cfg = sa.default_config(has_camera=1, camera_pos=C2_CAM["pos"], camera_lookat=C2_CAM["lookat"],
-                            camera_up=C2_CAM["up"], nthreads=min(16, os.cpu_count() or 1), **kw)
+    kw = dict(wl["render"])
+    if args.size:
+        kw["image_w"] = kw["image_h"] = args.size
+    if args.spp:
+        kw["pixel_samples"] = args.spp
+    W, H = kw["image_w"], kw["image_h"]
+    cam = wl["cam"]
+    camkw = dict(has_camera=1, camera_pos=cam["pos"], camera_lookat=cam["lookat"],
+                 camera_up=cam["up"]) if cam else dict(has_camera=0)
+    cfg = sa.default_config(nthreads=min(16, os.cpu_count() or 1), **camkw, **kw)
     ctx = sa.Context(cfg)
     t0 = time.time()
-    ctx.load_scene(os.path.join(tmp, "c2.spray"), tmp)
+    ctx.load_scene(os.path.join(tmp, args.workload + ".spray"), tmp)
     load_s = time.time() - t0
+    print(f"# {args.workload}: {tris} triangles, generated in {gen_s:.1f} s, loaded in {load_s:.1f} s",
+          file=sys.stderr, flush=True)
+    if cam is None:
+        cam = default_framing(ctx)
+        ctx.set_camera(cam["pos"], cam["lookat"], cam["up"], 90.0)
 
     schedules = {"plain": 0, "batched": 1, "learned": 2}
     wanted = [s for s in args.schedules.split(",") if s]
@@ -134,6 +178,8 @@ def main():
         _, st_prof = ctx.trace_frame(want_image=False)
         launches, kern_ms = ctx.kernel_time(reset=True)
         ctx.profile_kernels(args.no_graph)
+        print(f"# {name}: {st['useful_rays'] * args.steps / total_s / 1e6:.1f} Mrays/s, "
+              f"{1000.0 * total_s / args.steps:.1f} ms/frame", file=sys.stderr, flush=True)
         results[name] = dict(
             mrays_s=st["useful_rays"] * args.steps / total_s / 1e6,
             ms_per_frame=1000.0 * total_s / args.steps,
@@ -162,7 +208,7 @@ def main():
         avg_ms = r["trace_kernel_ms"] / max(1, r["trace_kernel_launches"])
         bytes_per_launch = r["trace_kernel_bytes"] / max(1, r["trace_kernel_launches"])
         achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
-        traffic = measured_traffic(best)
+        traffic = measured_traffic(args.workload, best)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": traffic["bytes_per_dispatch"] if traffic else None,
@@ -173,7 +219,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(tmp, kw, args.cpu_tiles)
+        cpu = cpu_baseline(tmp, args.workload, kw, cam, args.cpu_tiles or wl["cpu_tiles"])
 
     line = {
         "metric": "Mrays/s (AO + path) on 1 MI355X; learned vs plain schedule at equal image",
@@ -183,8 +229,7 @@ def main():
         "ms_per_step": round(ok[best]["ms_per_frame"], 3) if best else None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"C2 path tracing: 64 domains (4x4x4 MC isosurfaces, {tris} tris), "
-                               f"{W}x{H}, {args.spp} spp, depth 5, RR>=3, rect light",
+        "config": {"workload": wl["title"].format(tris=tris, w=W, h=H, spp=kw["pixel_samples"]),
                    "schedule": best, "triangles": tris, "scene_gen_s": round(gen_s, 2),
                    "load_build_s": round(load_s, 2)},
         "schedules": results,
@@ -195,11 +240,11 @@ def main():
     ctx.close()
 
 
-def measured_traffic(schedule):
+def measured_traffic(workload, schedule):
     """HBM bytes per trace-kernel dispatch measured by two rocprofv3 --pmc
     runs (FETCH_SIZE, WRITE_SIZE) of this benchmark, scaled by the same
     counters on a copy of known size; recorded as `key: value` lines."""
-    path = os.path.join(ROOT, "profiles", f"kTraverse_c2_{schedule}_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"kTraverse_{workload}_{schedule}_traffic.txt")
     if not os.path.exists(path):
         return None
     kv = {}
@@ -215,15 +260,34 @@ def measured_traffic(schedule):
         return None
 
 
-def cpu_baseline(scene_dir, kw, ntiles):
+def default_framing(ctx):
+    """The reference's camera for a scene without one (spray_renderer.inl:
+    95-104): look at the bound's centre from +z at half the diagonal, in the
+    library's float32 order of operations."""
+    import numpy as np
+    f = np.float32
+    lo = np.full(3, np.inf, np.float32)
+    hi = np.full(3, -np.inf, np.float32)
+    for i in range(ctx.num_domains):
+        _, _, a, b = ctx.domain_info(i)
+        lo, hi = np.minimum(lo, a), np.maximum(hi, b)
+    ext = (hi - lo).astype(np.float32)
+    look = (0.5 * (lo + hi).astype(np.float64)).astype(np.float32)
+    ln = f(np.sqrt(f(f(ext[0] * ext[0]) + f(ext[1] * ext[1])) + f(ext[2] * ext[2])))
+    z = f(f(ln * f(0.5)) * f(1.0))
+    pos = [float(look[0]), float(look[1]), float(f(look[2] + z))]
+    return dict(pos=pos, lookat=[float(x) for x in look], up=[0.0, 1.0, 0.0])
+
+
+def cpu_baseline(scene_dir, name, kw, cam, ntiles):
     """The oracle (the CPU restatement of the reference's OpenMP tracer) on
     the first `ntiles` blocking tiles of the same frame, on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
     threads = min(16, os.cpu_count() or 1)
-    ref = orc.Scene(os.path.join(scene_dir, "c2.spray"), scene_dir, nthreads=threads)
-    _, st = ref.render(nthreads=threads, cam_pos=C2_CAM["pos"], cam_lookat=C2_CAM["lookat"],
-                       cam_up=C2_CAM["up"], tile_begin=0, tile_end=ntiles, **kw)
+    ref = orc.Scene(os.path.join(scene_dir, name + ".spray"), scene_dir, nthreads=threads)
+    _, st = ref.render(nthreads=threads, cam_pos=cam["pos"], cam_lookat=cam["lookat"],
+                       cam_up=cam["up"], tile_begin=0, tile_end=ntiles, **kw)
     return {"value": round(st["useful_rays"] / st["seconds"] / 1e6, 4), "unit": "Mrays/s",
             "cores": threads, "kind": "port",
             "sample": f"first {ntiles} of {orc.num_tiles(**kw)} blocking tiles of the same frame "

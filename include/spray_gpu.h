@@ -149,10 +149,18 @@ int spray_domain_list(spray_ctx* ctx, const float* org, const float* dir, size_t
 /* Selects the speculative schedule (new). */
 int spray_set_schedule(spray_ctx* ctx, int schedule);
 
-/* Batched form: autotunes queue-chunk size and waves per work item on the
- * loaded scene and current camera; the choice is kept in the context. */
+/* Batched form: autotunes the queue chunk (how many lanes of a wave must be
+ * idle before the wave fetches that many queue entries with one atomic,
+ * 1..64) and the wavefront tile (waves per workgroup of the persistent trace
+ * kernel, 1..4) on the loaded scene and current camera; the choice is kept
+ * in the context. */
 int spray_autotune(spray_ctx* ctx, int* chunk_out, int* tile_out);
 int spray_set_batch_params(spray_ctx* ctx, int chunk, int tile);
+
+/* Batched forms (new): how many consecutive blocking tiles (tile.cc:317-391)
+ * share every pass; each keeps its own sample numbering, so the image does
+ * not depend on it.  0 (default) groups tiles up to about 8M samples. */
+int spray_set_tiles_in_flight(spray_ctx* ctx, int tiles);
 
 /* Kernel timing (new): when enabled, every trace-kernel launch (the
  * per-domain kernel of the plain form, the persistent kernel of the batched

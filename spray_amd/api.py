@@ -132,6 +132,9 @@ class Context:
     def set_batch_params(self, chunk, tile):
         self._check(self.lib.spray_set_batch_params(self.h, chunk, tile))
 
+    def set_tiles_in_flight(self, tiles):
+        self._check(self.lib.spray_set_tiles_in_flight(self.h, int(tiles)))
+
     def autotune(self):
         ch, tl = C.c_int(), C.c_int()
         self._check(self.lib.spray_autotune(self.h, C.byref(ch), C.byref(tl)))

@@ -385,11 +385,20 @@ int spray_autotune(spray_ctx* ctx, int* chunk_out, int* tile_out) {
 int spray_set_batch_params(spray_ctx* ctx, int chunk, int tile) {
   if (!ctx) return SPRAY_ERR_INVALID;
   return guarded(ctx, [&] {
-    if (chunk < 64 || chunk > 65536 || (chunk & 63)) throw StatusError(SPRAY_ERR_INVALID, "chunk must be a multiple of 64 in [64, 65536]");
-    if (tile < 1 || tile > 16) throw StatusError(SPRAY_ERR_INVALID, "tile must be in [1, 16]");
-    if (tile > 4) throw StatusError(SPRAY_ERR_INVALID, "tile must be in [1, 4]");
+    // chunk: idle lanes a wave gathers before it fetches queue entries for
+    // them at once (1..64); tile: waves per workgroup of the trace kernel
+    if (chunk < 1 || chunk > 64) throw StatusError(SPRAY_ERR_INVALID, "chunk must be in [1, 64]");
+    if (tile < 1 || tile > 4) throw StatusError(SPRAY_ERR_INVALID, "tile must be in [1, 4]");
     ctx->c.batch_chunk = chunk;
     ctx->c.batch_tile = tile;
+  });
+}
+
+int spray_set_tiles_in_flight(spray_ctx* ctx, int tiles) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (tiles < 0 || tiles > 4096) throw StatusError(SPRAY_ERR_INVALID, "tiles must be in [0, 4096]");
+    ctx->c.tiles_in_flight = tiles;
   });
 }
 

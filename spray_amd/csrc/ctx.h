@@ -85,6 +85,9 @@ struct DomainPool {
 struct TracerState;  // tracer.hip
 struct Predictor;    // mlp.hip
 
+// Samples in flight the batched forms aim for when they group blocking tiles.
+constexpr long long kBatchSamples = 8ll << 20;
+
 struct Ctx {
   spray_config cfg;
   int device = 0;
@@ -96,7 +99,8 @@ struct Ctx {
   bool camera_set = false;
   TracerState* tracer = nullptr;
   Predictor* pred = nullptr;
-  int batch_chunk = 1024;
+  int batch_chunk = 16;  // refill threshold of the trace kernel (idle lanes per wave)
+  int tiles_in_flight = 0;  // blocking tiles traced together (batched forms); 0 = automatic
   int batch_tile = 4;      // waves per block of the persistent kernel (block = 64 * tile)
   int param_version = 0;   // bumped when camera / config change (graph re-capture)
   bool profile = false;    // time every trace-kernel launch with hipEvents

@@ -159,61 +159,159 @@ __device__ __forceinline__ uint32_t orderedF(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__global__ void __launch_bounds__(kMlpBlock) kPredictKeys(TraceParams P, MlpDev M,
-                                                          unsigned long long* rkey) {
-  __shared__ __attribute__((aligned(16))) __bf16 lds[4 * kLdsPerWave];
+// Weights live in LDS for the whole (persistent) launch: W1 [128][32] and
+// W2 [128][128] (out, in) with padded row strides so that the 16-byte
+// B-fragment reads of the 16 lanes sharing a k-slice hit distinct banks.
+// Each wave runs 32 queue entries (two 16-row blocks) per iteration, so
+// every B fragment read from LDS feeds two MFMAs.
+constexpr int kPredRows = 32;
+constexpr int kLdX = kInPad + 8;   // bf16 row stride of x and W1 in LDS
+constexpr int kLdH = kHid + 8;     // bf16 row stride of h1 and W2 in LDS
+constexpr int kPredWaves = 8;
+constexpr size_t kPredLdsWeights = (size_t)kHid * kLdX + (size_t)kHid * kLdH;  // bf16 elements
+constexpr size_t kPredLdsPerWave = (size_t)kPredRows * kLdX + (size_t)kPredRows * kLdH;
+constexpr size_t kPredLdsBytes =
+    (kPredLdsWeights + kPredWaves * kPredLdsPerWave) * 2 + 3 * kHid * 4 + 16;
+
+__global__ void __launch_bounds__(64 * kPredWaves) kPredictKeys(TraceParams P, MlpDev M,
+                                                               unsigned long long* rkey) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const DevCounters* c = P.ctr;
   if (!c->active) return;
+  __bf16* w1s = (__bf16*)smem;                 // [128][kLdX]
+  __bf16* w2s = w1s + (size_t)kHid * kLdX;     // [128][kLdH]
+  __bf16* wave_base = w2s + (size_t)kHid * kLdH;
+  float* b1s = (float*)(wave_base + kPredWaves * kPredLdsPerWave);
+  float* b2s = b1s + kHid;
+  float* w3s = b2s + kHid;
+  for (int i = threadIdx.x; i < kHid * kInPad / 8; i += blockDim.x) {
+    int r = i / (kInPad / 8), k8 = i % (kInPad / 8);
+    *(bf16x8*)(w1s + r * kLdX + 8 * k8) = *(const bf16x8*)(M.w1 + r * kInPad + 8 * k8);
+  }
+  for (int i = threadIdx.x; i < kHid * kHid / 8; i += blockDim.x) {
+    int r = i / (kHid / 8), k8 = i % (kHid / 8);
+    *(bf16x8*)(w2s + r * kLdH + 8 * k8) = *(const bf16x8*)(M.w2 + r * kHid + 8 * k8);
+  }
+  for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+    b1s[i] = M.p[oB1 + i];
+    b2s[i] = M.p[oB2 + i];
+    w3s[i] = M.p[oW3 + i];
+  }
+  const float b3 = M.p[oB3];
+  __syncthreads();
   const uint32_t n = c->q_total[0];
   const int w = threadIdx.x >> 6, l = __lane_id();
-  __bf16* xs = lds + w * kLdsPerWave;
-  __bf16* h1s = xs + 16 * kInPad;
-  for (uint32_t base = (blockIdx.x * 4 + w) * 16u; base < n; base += gridDim.x * 64u) {
-    // lanes 0..15 build the 16 feature rows
-    uint32_t q = base + (l & 15);
-    bool valid = q < n;
-    // domain of each entry: binary search over the schedule-ordered offsets
-    int dom = 0;
-    if (valid) {
-      const uint32_t* off = P.off;
-      int lo = 0, hi = P.ndomains - 1;
-      while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= q)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      dom = P.order[lo];
+  __bf16* xs = wave_base + w * kPredLdsPerWave;  // [32][kLdX]
+  __bf16* hs = xs + kPredRows * kLdX;            // [32][kLdH]
+  for (uint32_t base = (blockIdx.x * kPredWaves + w) * (uint32_t)kPredRows; base < n;
+       base += gridDim.x * kPredWaves * (uint32_t)kPredRows) {
+    // lanes 0..31 build the feature rows; rays with a single candidate
+    // domain this pass need no ordering and are skipped
+    const uint32_t q = base + (uint32_t)(l & 31);
+    bool use = false;
+    QEntry e = QEntry{0, 0.f};
+    if (l < kPredRows && q < n) {
+      e = P.q[0][q];
+      use = P.rcand[e.ray] > 1;
     }
-    if (l < 16) {
+    if (l < kPredRows) {
       float x[kInPad];
-      QEntry e = valid ? P.q[0][q] : QEntry{0, 0.f};
-      float4 op = valid ? P.R.op[e.ray] : make_float4(0, 0, 0, 0);
-      float4 ds = valid ? P.R.ds[e.ray] : make_float4(0, 0, 0, 0);
-      float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
-      features(M, o, d, dom, x);
+      if (use) {
+        float4 op = P.R.op[e.ray], ds = P.R.ds[e.ray];
+        float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
+        features(M, o, d, (int)P.qdom[0][q], x);
+      } else {
 #pragma unroll
-      for (int k = 0; k < kInPad; ++k) xs[l * kInPad + k] = tobf(x[k]);
+        for (int k = 0; k < kInPad; ++k) x[k] = 0.f;
+      }
+#pragma unroll
+      for (int k8 = 0; k8 < kInPad / 8; ++k8) {
+        bf16x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = tobf(x[8 * k8 + j]);
+        *(bf16x8*)(xs + l * kLdX + 8 * k8) = v;
+      }
     }
+    if (__ballot(use) == 0ull) continue;
     waveSync();
-    f32x4 h1acc[8], h2[8];
-    float z[4];
-    forward16(M, xs, h1s, h2, h1acc, z);
-    // the 16 logits: row r lives in lanes with (l >> 4) == r / 4, slot r % 4
-    float zr = 0.f;
-    {
-      int r = l & 15;
-      int src = 16 * (r >> 2);
-      float zz[4];
+    // layer 1 (K = 32): h1 = relu(x W1^T + b1) -> bf16 rows
+    f32x4 acc[2][8];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) zz[i] = __shfl(z[i], src, 64);
-      zr = (r & 3) == 0 ? zz[0] : (r & 3) == 1 ? zz[1] : (r & 3) == 2 ? zz[2] : zz[3];
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) acc[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    {
+      bf16x8 a0 = ldsA(xs, kLdX, 0), a1 = ldsA(xs + 16 * kLdX, kLdX, 0);
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) {
+        bf16x8 bw = ldB(w1s, kLdX, nb, 0);
+        acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw, acc[0][nb], 0, 0, 0);
+        acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw, acc[1][nb], 0, 0, 0);
+      }
     }
-    if (l < 16 && valid) {
-      QEntry e = P.q[0][q];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) {
+        int col = 16 * nb + (l & 15);
+        float bb = b1s[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          hs[(16 * rb + 4 * (l >> 4) + i) * kLdH + col] = tobf(fmaxf(acc[rb][nb][i] + bb, 0.f));
+      }
+    waveSync();
+    // layer 2 (K = 128)
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) acc[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < kHid / 32; ++kb) {
+      bf16x8 a0 = ldsA(hs, kLdH, kb), a1 = ldsA(hs + 16 * kLdH, kLdH, kb);
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) {
+        bf16x8 bw = ldB(w2s, kLdH, nb, kb);
+        acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw, acc[0][nb], 0, 0, 0);
+        acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw, acc[1][nb], 0, 0, 0);
+      }
+    }
+    // logits: relu(h2 + b2) . w3 + b3, reduced over the 16 lanes of a row group
+    float z[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) {
+        int col = 16 * nb + (l & 15);
+        float bb = b2s[col], ww = w3s[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum[i] += fmaxf(acc[rb][nb][i] + bb, 0.f) * ww;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = sum[i];
+        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+        z[rb][i] = v + b3;
+      }
+    }
+    // entry r (lane r < 32): row block r / 16, row r % 16 held by lanes
+    // 16 * ((r % 16) / 4) .. at slot r % 4
+    const int r = l & 31, rb = r >> 4, rl = r & 15, src = 16 * (rl >> 2);
+    float zz[2][4];
+#pragma unroll
+    for (int b2 = 0; b2 < 2; ++b2)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) zz[b2][i] = __shfl(z[b2][i], src, 64);
+    float zr = 0.f;
+#pragma unroll
+    for (int b2 = 0; b2 < 2; ++b2)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (b2 == rb && i == (rl & 3)) zr = zz[b2][i];
+    if (use) {
       // argmax over a ray's entries of (logit, -tdom): smaller tdom wins ties
-      unsigned long long key = ((unsigned long long)orderedF(zr) << 32) | (uint32_t)(~orderedF(e.tdom));
+      unsigned long long key =
+          ((unsigned long long)orderedF(zr) << 32) | (uint32_t)(~orderedF(e.tdom));
       atomicMax(&rkey[e.ray], key);
     }
     waveSync();
@@ -228,29 +326,69 @@ __global__ void kResetKeys(TraceParams P, unsigned long long* rkey) {
     rkey[i] = 0ull;
 }
 
-// Splits the radiance queue into the phase-A list and the deferred list
-// (both compacted, queue order kept within a wave).
-__global__ void kAssignPhase(TraceParams P, const unsigned long long* rkey) {
+// Splits the radiance queue into the phase-A list and the deferred list.
+// Entries of rays without a key (one candidate domain) stay in phase A.
+// Each block compacts 8 x 256 entries per step with one atomic per list.
+constexpr int kAssignItems = 8;
+__global__ void __launch_bounds__(256) kAssignPhase(TraceParams P, const unsigned long long* rkey) {
   DevCounters* c = P.ctr;
   if (!c->active) return;
+  __shared__ uint32_t wcnt[2][kAssignItems][4];
+  __shared__ uint32_t bases[2];
   const uint32_t n = c->q_total[0];
+  const int w = threadIdx.x >> 6, l = __lane_id();
+  const unsigned long long below = (1ull << l) - 1ull;
   unsigned long long deferred = 0;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    uint32_t q = base + threadIdx.x;
-    bool valid = q < n, d = false;
-    if (valid) {
-      QEntry e = P.q[0][q];
-      uint32_t enc = ~(uint32_t)rkey[e.ray];  // ordered bits of the chosen tdom
-      d = orderedF(e.tdom) > enc;
+  const uint32_t step = gridDim.x * 256u * kAssignItems;
+  for (uint32_t base = blockIdx.x * 256u * kAssignItems; base < n; base += step) {
+    bool dflag[kAssignItems], vflag[kAssignItems];
+    uint32_t rank[kAssignItems];
+#pragma unroll
+    for (int k = 0; k < kAssignItems; ++k) {
+      uint32_t q = base + k * 256u + threadIdx.x;
+      bool valid = q < n, d = false;
+      if (valid) {
+        QEntry e = P.q[0][q];
+        unsigned long long key = rkey[e.ray];
+        d = key != 0ull && orderedF(e.tdom) > ~(uint32_t)key;  // beyond the chosen domain
+      }
+      vflag[k] = valid;
+      dflag[k] = d;
+      unsigned long long ba = __ballot(valid && !d), bb = __ballot(valid && d);
+      rank[k] = (uint32_t)__popcll((d ? bb : ba) & below);
+      if (l == 0) {
+        wcnt[0][k][w] = (uint32_t)__popcll(ba);
+        wcnt[1][k][w] = (uint32_t)__popcll(bb);
+      }
+      deferred += (valid && d) ? 1 : 0;
     }
-    uint32_t ia = waveAppend(valid && !d, &c->n_phase[0], P.q_cap[0], &c->err);
-    uint32_t ib = waveAppend(valid && d, &c->n_phase[1], P.q_cap[0], &c->err);
-    if (ia != 0xffffffffu) P.qa[ia] = q;
-    if (ib != 0xffffffffu) P.qb[ib] = q;
-    deferred += d ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x < 2) {  // exclusive prefix over (item, wave) in place, one atomic per list
+      uint32_t acc = 0;
+      for (int k = 0; k < kAssignItems; ++k)
+        for (int ww = 0; ww < 4; ++ww) {
+          uint32_t t = wcnt[threadIdx.x][k][ww];
+          wcnt[threadIdx.x][k][ww] = acc;
+          acc += t;
+        }
+      bases[threadIdx.x] = acc ? atomicAdd(&c->n_phase[threadIdx.x], acc) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kAssignItems; ++k) {
+      if (!vflag[k]) continue;
+      const int list = dflag[k] ? 1 : 0;
+      uint32_t idx = bases[list] + wcnt[list][k][w] + rank[k];
+      if (idx < P.q_cap[0]) {
+        (list ? P.qb : P.qa)[idx] = base + k * 256u + threadIdx.x;
+      } else {
+        atomicOr(&c->err, 1u);
+      }
+    }
+    __syncthreads();
   }
   deferred = waveSumULL(deferred);
-  if (__lane_id() == 0 && deferred) atomicAdd(&c->deferred, deferred);
+  if (l == 0 && deferred) atomicAdd(&c->deferred, deferred);
 }
 
 // ---------------------------------------------------------------------------
@@ -491,6 +629,7 @@ struct Predictor {
   bool trained = false;
   float center[3], inv_half[3];
   int tbatch = 0;
+  int pred_grid = 0;  // persistent inference grid
   MlpDev dev() const {
     MlpDev M;
     M.w1 = w1.as<__bf16>();
@@ -613,7 +752,8 @@ void predictorPhase(Ctx* c, const TraceParams& P) {
   Predictor* R = c->pred;  // rkey was sized by predictorPrepare (no allocation here: graph capture)
   hipStream_t s = c->stream;
   kResetKeys<<<1024, 256, 0, s>>>(P, R->rkey.as<unsigned long long>());
-  kPredictKeys<<<2048, kMlpBlock, 0, s>>>(P, R->dev(), R->rkey.as<unsigned long long>());
+  kPredictKeys<<<R->pred_grid, 64 * kPredWaves, kPredLdsBytes, s>>>(
+      P, R->dev(), R->rkey.as<unsigned long long>());
   kAssignPhase<<<1024, 256, 0, s>>>(P, R->rkey.as<unsigned long long>());
 }
 
@@ -622,6 +762,13 @@ bool predictorReady(Ctx* c) { return c->pred && c->pred->trained; }
 void predictorPrepare(Ctx* c, size_t r_cap) {
   if (!c->pred) throw StatusError(SPRAY_ERR_STATE, "learned schedule needs a trained predictor");
   c->pred->rkey.reserve(r_cap * 8);
+  if (!c->pred->pred_grid) {
+    hipDeviceProp_t prop;
+    SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
+    SPRAY_HIP(hipFuncSetAttribute((const void*)kPredictKeys,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPredLdsBytes));
+    c->pred->pred_grid = prop.multiProcessorCount;  // one 8-wave block per CU
+  }
 }
 
 }  // namespace spray

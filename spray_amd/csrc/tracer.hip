@@ -57,8 +57,10 @@ __global__ void kGenEyes(TraceParams P) {
   const long long S = (long long)tile_h * P.image_w * P.spp;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < S;
        i += (long long)gridDim.x * blockDim.x) {
-    int bufid = (int)i;
-    int pix_local = bufid / P.spp;
+    // samid = index in the batch of tiles (history-buffer slot); the
+    // jitter seed is the tile-local sample index, as with one tile at a time
+    int bufid = (int)(i % P.tile_samples);
+    int pix_local = (int)(i / P.spp);
     int x = pix_local % P.image_w;
     int y = tile_y + pix_local / P.image_w;
     f3 d;
@@ -71,7 +73,7 @@ __global__ void kGenEyes(TraceParams P) {
       d = cameraDir(P.cam, (float)x, (float)y);
     }
     P.R.op[i] = make_float4(P.cam.pos.x, P.cam.pos.y, P.cam.pos.z, __int_as_float(y * P.image_w + x));
-    P.R.ds[i] = make_float4(d.x, d.y, d.z, __int_as_float(bufid));
+    P.R.ds[i] = make_float4(d.x, d.y, d.z, __int_as_float((int)i));
     P.R.wd[i] = make_float4(1.f, 1.f, 1.f, __int_as_float(0));
     P.R.h[i] = make_float4(kInfF, kInfF, kInfF, kInfF);
   }
@@ -210,13 +212,14 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     // the 16 nearest (t, id) entries carry score weight
     float tt[kTopDepths];
     int ti[kTopDepths];
-    int kept = 0;
+    int kept = 0, hits = 0;
     for (int dd = 0; dd < nd; ++dd) {
       float tmin;
       if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
       // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
       // spawning domain was tested right after shading
       if (dd == spawn) continue;
+      ++hits;
       atomicAdd(&lc[kind * nd + dd], 1u);
       // insertion into the (t, id)-sorted top list (unrolled: stays in registers)
       if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
@@ -243,6 +246,7 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     }
     for (int k = 0; k < kTopDepths; ++k)
       if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
+    if (kind == 0) P.rcand[r] = (uint8_t)min(hits, 255);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
@@ -525,7 +529,7 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
     }
   } else {
     bool lit = act && !delta;
-    uint32_t ls = rngInit(samid * next_actual);
+    uint32_t ls = rngInit((samid % P.tile_samples) * next_actual);
     f3 ks = mk(P.ks[0], P.ks[1], P.ks[2]);
     for (int l = 0; l < P.nlights; ++l) {
       const DevLight& L = P.lights[l];
@@ -599,7 +603,7 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
     f3 Lr = mk(0, 0, 0);
     bool go = false;
     if (bounce) {
-      uint32_t rng = rngInit(samid * next_actual);
+      uint32_t rng = rngInit((samid % P.tile_samples) * next_actual);
       float u1 = rng1D(rng), u2 = rng1D(rng);
       cosineSample(u1, u2, nff, &wi, &pdf);
       costheta = clamp01(dot(nff, wi));
@@ -667,7 +671,10 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     ns = c->q_total[1];
     wc = &P.wctr[nd];
   } else {
-    nr = phase == 0 ? c->n_phase[0] : c->n_phase[1];
+    // learned: phase 0 takes the phase-A radiance list and the shadow
+    // queues; phase 1 (launched after it) the deferred list, filtered
+    // against the histories phase A updated
+    nr = min(phase == 0 ? c->n_phase[0] : c->n_phase[1], P.q_cap[0]);
     ns = phase == 0 ? c->q_total[1] : 0u;
     rlist = phase == 0 ? P.qa : P.qb;
     wc = &P.wctr[nd + phase];
@@ -706,7 +713,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
           if (local) {
             qe = QEntry{local_base + e, 0.f};
           } else {
-            uint32_t q = kind == 0 ? (rlist ? rlist[e] : rbase + e) : sbase + (e - nr);
+            uint32_t q = kind == 1 ? sbase + (e - nr) : (rlist ? rlist[e] : rbase + e);
             qe = P.q[kind][q];
             dom = plain_dom >= 0 ? plain_dom : (int)P.qdom[kind][q];
           }
@@ -927,7 +934,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, cursor,
-      wctr, score, order, q[2], qdom[2], ovf, phase_b, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa,
+      wctr, score, order, q[2], qdom[2], rcand, ovf, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa,
       qb;
   unsigned long long log_cap = 0;
   bool logging = false;
@@ -979,7 +986,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->cursor, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->ovf, &T->phase_b, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
+                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->ovf, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
                     &T->sh_tuv, &T->sh_prim, &T->log_ctr, &T->qa, &T->qb})
     b->release();
   delete T;
@@ -1012,11 +1019,30 @@ static void need(TracerState* T, DevBuf& b, size_t bytes) {
   b.reserve(bytes);
 }
 
+// Blocking tiles traced together (batched forms): consecutive tiles share
+// every pass, so each launch holds enough queue entries to keep the GPU busy
+// while its longest traversals finish.  Each tile keeps its own sample
+// numbering (RNG seeds), so the image does not depend on the batch size.
+static int tilesPerBatch(const Ctx* c, const std::vector<int>& hs) {
+  const long long per_tile = (long long)c->cfg.image_w * hs[0] * c->cfg.pixel_samples;
+  long long k = c->tiles_in_flight > 0 ? c->tiles_in_flight
+                                       : std::max<long long>(1, kBatchSamples / per_tile);
+  return (int)std::max<long long>(1, std::min<long long>(k, (long long)hs.size()));
+}
+
 static void ensureState(Ctx* c, double grow) {
   const spray_config& cfg = c->cfg;
   std::vector<int> ys, hs;
   tileList(cfg, &ys, &hs);
-  long long S = (long long)cfg.image_w * (*std::max_element(hs.begin(), hs.end())) * cfg.pixel_samples;
+  const int per_batch = cfg.schedule != SPRAY_SCHEDULE_PLAIN ? tilesPerBatch(c, hs) : 1;
+  long long rows = 0;
+  for (size_t b0 = 0; b0 < hs.size(); b0 += per_batch) {
+    long long r = 0;
+    for (size_t ti = b0; ti < std::min(hs.size(), b0 + (size_t)per_batch); ++ti) r += hs[ti];
+    rows = std::max(rows, r);
+  }
+  long long S = (long long)cfg.image_w * rows * cfg.pixel_samples;
+  if (S > 0x7fffffffLL) throw StatusError(SPRAY_ERR_CAPACITY, "too many samples in flight");
   if (!c->tracer) c->tracer = new TracerState();
   TracerState* T = c->tracer;
   int nd = c->pool.ndomains;
@@ -1046,6 +1072,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->s_tmax, (size_t)T->s_cap * 4);
   need(T, T->s_occ, (size_t)T->s_cap * 4);
   need(T, T->vbuf, (size_t)S * kHistory * 8);
+  need(T, T->rcand, (size_t)T->r_cap + 64);
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
   need(T, T->cnt, (size_t)2 * nd * 4 + 64);
@@ -1065,7 +1092,6 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->q[1], (size_t)T->q_cap[1] * sizeof(QEntry));
   need(T, T->qdom[0], (size_t)T->q_cap[0] * 2 + 64);
   need(T, T->qdom[1], (size_t)T->q_cap[1] * 2 + 64);
-  need(T, T->phase_b, (size_t)T->q_cap[0] + 64);
   T->h_cnt.resize(2 * nd);
   T->h_order.resize(nd);
   need(T, T->image, (size_t)cfg.image_w * cfg.image_h * 16);
@@ -1085,6 +1111,11 @@ static TraceParams makeParams(Ctx* c) {
   P.cam = c->cam;
   P.image_w = cfg.image_w;
   P.spp = cfg.pixel_samples;
+  {
+    std::vector<int> ys, hs;
+    tileList(cfg, &ys, &hs);
+    P.tile_samples = (long long)cfg.image_w * hs[0] * cfg.pixel_samples;
+  }
   P.ao_mode = cfg.ao_mode;
   P.ao_samples = cfg.ao_samples;
   P.bounces = cfg.bounces;
@@ -1151,6 +1182,7 @@ static TraceParams makeParams(Ctx* c) {
   P.q[1] = T->q[1].as<QEntry>();
   P.qdom[0] = T->qdom[0].as<uint16_t>();
   P.qdom[1] = T->qdom[1].as<uint16_t>();
+  P.rcand = T->rcand.as<uint8_t>();
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
   P.refill = std::min(64, std::max(1, c->batch_chunk));
@@ -1159,7 +1191,6 @@ static TraceParams makeParams(Ctx* c) {
   P.sh_tuv = T->sh_tuv.as<float>();
   P.sh_prim = T->sh_prim.as<int>();
   P.sh_cap = T->q_cap[0];
-  P.phase_b = T->phase_b.as<uint8_t>();
   P.qa = T->qa.as<uint32_t>();
   P.qb = T->qb.as<uint32_t>();
   P.learned = (cfg.schedule == SPRAY_SCHEDULE_LEARNED) ? 1 : 0;
@@ -1325,8 +1356,11 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
       SPRAY_HIP(hipGraphDestroy(g));
       std::memcpy(T->graph_key, key, sizeof(key));
     }
-    for (size_t ti = 0; ti < ys.size(); ++ti) {
-      int rec[4] = {ys[ti], hs[ti], 0, 0};
+    const int per_batch = batched ? tilesPerBatch(c, hs) : 1;
+    for (size_t b0 = 0; b0 < ys.size(); b0 += per_batch) {
+      int rows = 0;
+      for (size_t ti = b0; ti < std::min(ys.size(), b0 + (size_t)per_batch); ++ti) rows += hs[ti];
+      int rec[4] = {ys[b0], rows, 0, 0};
       SPRAY_HIP(hipMemcpyAsync(T->tile_rec.p, rec, sizeof(rec), hipMemcpyHostToDevice, s));
       if (use_graph)
         SPRAY_HIP(hipGraphLaunch(T->graph, s));

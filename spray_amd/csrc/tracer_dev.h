@@ -46,6 +46,7 @@ struct DevCounters {
 struct TraceParams {
   CameraDev cam;
   int image_w, spp, tile_y, tile_w, tile_h;  // tile_* come from the device tile record
+  int tile_samples;  // samples of a full blocking tile (RNG seeds are tile-local)
   int ao_mode, ao_samples, bounces, rr_start, nlights;
   float ao_max_dist, shininess, ks[3];
   float inv_spp_f;
@@ -79,6 +80,7 @@ struct TraceParams {
   int* order;         // [nd] schedule
   QEntry* q[2];       // queues (radiance, shadow)
   uint16_t* qdom[2];  // domain of each queue entry
+  uint8_t* rcand;     // radiance rays: queue entries of this pass (saturating)
   uint32_t q_cap[2];
   int refill;         // a wave refetches work once this many lanes are idle
   uint32_t* ovf;      // traversal stack overflow areas
@@ -89,7 +91,6 @@ struct TraceParams {
   int* sh_prim;
   uint32_t sh_cap;
   // learned form
-  uint8_t* phase_b;   // per radiance queue entry: 1 = deferred
   uint32_t* qa;       // learned form: radiance queue indices of phase A
   uint32_t* qb;       //   and of the deferred phase B
   int learned;

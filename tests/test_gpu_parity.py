@@ -140,23 +140,35 @@ def test_ao_frame_c1(ctx_c1, ref_c1, schedule):
     assert rmse <= 1e-4 and mx <= 1e-3
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
-def test_pt_frame_small(c1, schedule):
-    """Path tracing with the rect light, depth 5, RR from bounce 3, several
-    blocking tiles and two speculation windows, vs the oracle."""
+PT_SMALL = dict(image_w=128, image_h=128, pixel_samples=4, ao_mode=0, ao_samples=1, bounces=5,
+                rr_start=3, max_samples_per_rank=16384)  # 4 blocking tiles of 32 rows
+
+
+@pytest.fixture(scope="module")
+def ref_pt_small(c1):
+    ref = orc.Scene(os.path.join(c1, "c2.spray"), c1)
+    return ref.render(nthreads=8, cam_pos=C1_CAM["pos"], cam_lookat=C1_CAM["lookat"],
+                      cam_up=C1_CAM["up"], **PT_SMALL)
+
+
+@pytest.mark.parametrize("schedule,tiles", [(0, 0), (1, 0), (1, 1), (1, 3)])
+def test_pt_frame_small(c1, ref_pt_small, schedule, tiles):
+    """Path tracing with the rect light, depth 5, RR from bounce 3, four
+    blocking tiles and two speculation windows, vs the oracle; the batched
+    form traces the tiles all together (0 = automatic), one by one, or in
+    groups of 3 + 1, with the same result."""
     import spray_amd as sa
-    kw = dict(image_w=128, image_h=128, pixel_samples=4, ao_mode=0, ao_samples=1, bounces=5,
-              rr_start=3, max_samples_per_rank=16384)
     cfg = sa.default_config(has_camera=1, camera_pos=C1_CAM["pos"], camera_lookat=C1_CAM["lookat"],
-                            camera_up=C1_CAM["up"], nthreads=8, schedule=schedule, **kw)
+                            camera_up=C1_CAM["up"], nthreads=8, schedule=schedule, **PT_SMALL)
     with sa.Context(cfg) as ctx:
         ctx.load_scene(os.path.join(c1, "c2.spray"), c1)
+        ctx.set_tiles_in_flight(tiles)
         img, st = ctx.trace_frame()
-    ref = orc.Scene(os.path.join(c1, "c2.spray"), c1)
-    rimg, rst = ref.render(nthreads=8, cam_pos=C1_CAM["pos"], cam_lookat=C1_CAM["lookat"],
-                           cam_up=C1_CAM["up"], **kw)
+    rimg, rst = ref_pt_small
     rmse, mx = _img_close(img, rimg)
-    print(f"PT schedule {schedule}: rmse {rmse:.3g} max {mx:.3g} useful {st['useful_rays']} "
-          f"oracle {rst['useful_rays']}")
+    print(f"PT schedule {schedule} tiles {tiles}: rmse {rmse:.3g} max {mx:.3g} "
+          f"useful {st['useful_rays']} oracle {rst['useful_rays']}")
+    assert st["tiles"] == 4
     assert st["useful_rays"] == rst["useful_rays"]
+    assert st["final_hits"] == rst["final_hits"]
     assert rmse <= 1e-4 and mx <= 1e-3
