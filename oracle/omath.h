@@ -241,28 +241,31 @@ struct Camera {
 };
 
 // ---------------------------------------------------------------------------
-// Ray/box slab test: render/aabb.h:139-169 (intersectAabb with float[3])
+// Ray/box slab test: render/aabb.h:139-169 (intersectAabb with float[3]).
+// The reference's slab distances are (plane - org) / dir; a ray parallel to
+// a slab (dir 0, 1/dir infinite) whose origin lies on one of its planes gives
+// 0 * inf = NaN there, which the reference's comparisons then mishandle
+// (deviation Q13: it drops the domain).  Here such a ray is inside the slab
+// when its origin is on the lower plane and outside when on the upper one
+// (unless the slab is flat), so a ray running along a face shared by two
+// domains belongs to exactly one of them.  Without NaNs the result equals
+// the reference's (its comparison chain computes the same max of entries
+// and min of exits).
 inline bool intersectAabb(const float lo[3], const float hi[3], const float org[3],
                           const float dir[3], float t0, float t1, float* tmin_o) {
-  float inv[3] = {1.0f / dir[0], 1.0f / dir[1], 1.0f / dir[2]};
-  const float* b0x = inv[0] < 0.0f ? hi : lo;
-  const float* b1x = inv[0] < 0.0f ? lo : hi;
-  const float* b0y = inv[1] < 0.0f ? hi : lo;
-  const float* b1y = inv[1] < 0.0f ? lo : hi;
-  const float* b0z = inv[2] < 0.0f ? hi : lo;
-  const float* b1z = inv[2] < 0.0f ? lo : hi;
-  float tmin = (b0x[0] - org[0]) * inv[0];
-  float tmax = (b1x[0] - org[0]) * inv[0];
-  float tymin = (b0y[1] - org[1]) * inv[1];
-  float tymax = (b1y[1] - org[1]) * inv[1];
-  if ((tmin > tymax) || (tymin > tmax)) return false;
-  if (tymin > tmin) tmin = tymin;
-  if (tymax < tmax) tmax = tymax;
-  float tzmin = (b0z[2] - org[2]) * inv[2];
-  float tzmax = (b1z[2] - org[2]) * inv[2];
-  if ((tmin > tzmax) || (tzmin > tmax)) return false;
-  if (tzmin > tmin) tmin = tzmin;
-  if (tzmax < tmax) tmax = tzmax;
+  float tmin = -kInf, tmax = kInf;
+  for (int k = 0; k < 3; ++k) {
+    float inv = 1.0f / dir[k];
+    float a = ((inv < 0.0f ? hi[k] : lo[k]) - org[k]) * inv;
+    float b = ((inv < 0.0f ? lo[k] : hi[k]) - org[k]) * inv;
+    if (a != a || b != b) {  // parallel ray on a slab plane
+      if (org[k] == hi[k] && lo[k] < hi[k]) return false;
+      continue;
+    }
+    if (a > tmin) tmin = a;
+    if (b < tmax) tmax = b;
+  }
+  if (tmin > tmax) return false;
   *tmin_o = tmin;
   return (tmin < t1) && (tmax > t0);
 }

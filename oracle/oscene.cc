@@ -640,9 +640,11 @@ inline bool boxHit(const BvhNode& n, const float org[3], const float inv[3], flo
   for (int k = 0; k < 3; ++k) {
     float a = (n.lo[k] - org[k]) * inv[k];
     float b = (n.hi[k] - org[k]) * inv[k];
-    float lo = std::fmin(a, b), hi = std::fmax(a, b);
-    t0 = std::fmax(t0, lo);  // fmax ignores a NaN operand: conservative
-    t1 = std::fmin(t1, hi);
+    // a ray parallel to the slab with its origin on a slab plane gives
+    // 0 * inf = NaN: inside the slab, no bound on this axis
+    if (a != a || b != b) continue;
+    t0 = std::fmax(t0, std::fmin(a, b));
+    t1 = std::fmin(t1, std::fmax(a, b));
   }
   // robust slab test (Ize 2013): widen the far bound by 1 + 2*gamma(3)
   t1 = t1 * 1.0000003576278687f;

@@ -171,23 +171,69 @@ SPRAY_HD f3 cameraDir(const CameraDev& c, float x, float y) {
 }
 
 // render/aabb.h:139-169, returns tmin through *tmin_o
+// Domain AABB slab test (render/aabb.h:139-169), entry distance in *tmin_o.
+// A ray parallel to a slab whose origin lies on one of its planes gives
+// 0 * inf = NaN there: the ray is inside the slab when its origin is on the
+// lower plane and outside when on the upper one (unless the slab is flat),
+// so a ray along a face shared by two domains belongs to exactly one of
+// them.  Only rays with a zero (or denormal) direction component can
+// produce NaN: safe = false (uniform over a wave) skips the check.
+SPRAY_HD bool slabDomainS(const float lo[3], const float hi[3], const float org[3],
+                          const float inv[3], bool safe, float* tmin_o) {
+  float tmin = -kInfF, tmax = kInfF;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float a = ((inv[k] < 0.0f ? hi[k] : lo[k]) - org[k]) * inv[k];
+    float b = ((inv[k] < 0.0f ? lo[k] : hi[k]) - org[k]) * inv[k];
+    if (safe && (a != a || b != b)) {
+      if (org[k] == hi[k] && lo[k] < hi[k]) return false;
+      continue;
+    }
+    if (a > tmin) tmin = a;
+    if (b < tmax) tmax = b;
+  }
+  *tmin_o = tmin;
+  return tmin <= tmax && (tmin < kInfF) && (tmax > kRayEps);
+}
+
+// Branch-free form of the same test against a box held as two float4 (lo,
+// hi).  For finite 1/dir, min/max of the two plane distances equal the
+// reference's sign-selected ones bit for bit (they are the same products).
+// kSafe (uniform over a wave) adds the NaN rule of slabDomainS.
+SPRAY_HD bool slabBox4(const float4& lo, const float4& hi, const float org[3], const float inv[3],
+                       bool kSafe, float* tmin_o) {
+  const float ax = (lo.x - org[0]) * inv[0], bx = (hi.x - org[0]) * inv[0];
+  const float ay = (lo.y - org[1]) * inv[1], by = (hi.y - org[1]) * inv[1];
+  const float az = (lo.z - org[2]) * inv[2], bz = (hi.z - org[2]) * inv[2];
+  float nx = fminf(ax, bx), fx = fmaxf(ax, bx);
+  float ny = fminf(ay, by), fy = fmaxf(ay, by);
+  float nz = fminf(az, bz), fz = fmaxf(az, bz);
+  bool out = false;
+  if (kSafe) {
+    const bool qx = ax != ax || bx != bx, qy = ay != ay || by != by, qz = az != az || bz != bz;
+    out = (qx && org[0] == hi.x && lo.x < hi.x) || (qy && org[1] == hi.y && lo.y < hi.y) ||
+          (qz && org[2] == hi.z && lo.z < hi.z);
+    nx = qx ? -kInfF : nx;
+    fx = qx ? kInfF : fx;
+    ny = qy ? -kInfF : ny;
+    fy = qy ? kInfF : fy;
+    nz = qz ? -kInfF : nz;
+    fz = qz ? kInfF : fz;
+  }
+  const float tmin = fmaxf(fmaxf(nx, ny), nz), tmax = fminf(fminf(fx, fy), fz);
+  *tmin_o = tmin;
+  return !out && tmin <= tmax && tmin < kInfF && tmax > kRayEps;
+}
+
+// True when 1/dir has an infinite component (dir component 0 or denormal).
+SPRAY_HD bool hasInfInv(const float inv[3]) {
+  return fabsf(inv[0]) == kInfF || fabsf(inv[1]) == kInfF || fabsf(inv[2]) == kInfF;
+}
+
 SPRAY_HD bool slabDomain(const float lo[3], const float hi[3], const float org[3],
                          const float dir[3], float* tmin_o) {
-  float inv0 = 1.0f / dir[0], inv1 = 1.0f / dir[1], inv2 = 1.0f / dir[2];
-  float tmin = ((inv0 < 0.0f ? hi[0] : lo[0]) - org[0]) * inv0;
-  float tmax = ((inv0 < 0.0f ? lo[0] : hi[0]) - org[0]) * inv0;
-  float tymin = ((inv1 < 0.0f ? hi[1] : lo[1]) - org[1]) * inv1;
-  float tymax = ((inv1 < 0.0f ? lo[1] : hi[1]) - org[1]) * inv1;
-  if ((tmin > tymax) || (tymin > tmax)) return false;
-  if (tymin > tmin) tmin = tymin;
-  if (tymax < tmax) tmax = tymax;
-  float tzmin = ((inv2 < 0.0f ? hi[2] : lo[2]) - org[2]) * inv2;
-  float tzmax = ((inv2 < 0.0f ? lo[2] : hi[2]) - org[2]) * inv2;
-  if ((tmin > tzmax) || (tzmin > tmax)) return false;
-  if (tzmin > tmin) tmin = tzmin;
-  if (tzmax < tmax) tmax = tzmax;
-  *tmin_o = tmin;
-  return (tmin < kInfF) && (tmax > kRayEps);
+  float inv[3] = {1.0f / dir[0], 1.0f / dir[1], 1.0f / dir[2]};
+  return slabDomainS(lo, hi, org, inv, true, tmin_o);
 }
 
 // Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1), 2013).

@@ -176,6 +176,46 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
   return false;
 }
 
+// One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
+// entries sorted (registers) and the number of domains hit.
+__device__ __forceinline__ int rayDomains(const float4* box, int nd, const float o[3],
+                                          const float inv[3], bool safe, int spawn, uint32_t* lc_kind,
+                                          float tt[kTopDepths], int ti[kTopDepths], int* kept_o) {
+  int kept = 0, hits = 0;
+  for (int dd = 0; dd < nd; ++dd) {
+    float tmin;
+    // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
+    // spawning domain was tested right after shading
+    if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
+    ++hits;
+    atomicAdd(&lc_kind[dd], 1u);
+    // insertion into the (t, id)-sorted top list (unrolled: stays in registers)
+    if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
+      float ct = tmin;
+      int ci = dd;
+#pragma unroll
+      for (int k = 0; k < kTopDepths; ++k) {
+        if (k < kept) {
+          if (ct < tt[k] || (ct == tt[k] && ci < ti[k])) {
+            float xt = tt[k];
+            int xi = ti[k];
+            tt[k] = ct;
+            ti[k] = ci;
+            ct = xt;
+            ci = xi;
+          }
+        } else if (k == kept) {
+          tt[k] = ct;
+          ti[k] = ci;
+        }
+      }
+      if (kept < kTopDepths) ++kept;
+    }
+  }
+  *kept_o = kept;
+  return hits;
+}
+
 // Isector::isectAll / isectWithoutCurrentDomain (ooc_isector.h:126-174) as
 // counts: entries per domain and the DomainStats score
 // (ooc_domain_stats.cc:60-88) weight max(0, 16 - position in the ray's list).
@@ -183,14 +223,11 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
   const int nd = P.ndomains;
-  float* boxes = (float*)sm;                    // 6 * nd
-  uint32_t* lc = (uint32_t*)(boxes + 6 * nd);   // 2 * nd counts
-  uint32_t* ls = lc + 2 * nd;                   // nd score (32-bit per block)
+  float4* box = (float4*)sm;               // 2 * nd: (lo, 0), (hi, 0)
+  uint32_t* lc = (uint32_t*)(box + 2 * nd);  // 2 * nd counts
+  uint32_t* ls = lc + 2 * nd;                // nd score (32-bit per block)
+  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
-    for (int k = 0; k < 3; ++k) {
-      boxes[i * 6 + k] = P.desc[i].lo[k];
-      boxes[i * 6 + 3 + k] = P.desc[i].hi[k];
-    }
     lc[i] = 0;
     lc[nd + i] = 0;
     ls[i] = 0;
@@ -198,55 +235,41 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   __syncthreads();
   const DevCounters* c = P.ctr;
   const long long total = (long long)(c->r_end - c->r_beg) + (c->s_end - c->s_beg);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    int kind;
-    uint32_t r;
-    passRay(P, i, &kind, &r);
-    const RayPool& pool = kind ? P.S : P.R;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  // wave-uniform loop (the slab-test variant is chosen per wave)
+  for (long long base = blockIdx.x * (long long)blockDim.x; base < total; base += stride) {
+    const long long i = base + threadIdx.x;
+    int kind = 0;
+    uint32_t r = 0;
+    bool valid = passRay(P, i, &kind, &r);
     // shadow rays occluded in their spawning domain are done (procShads2)
-    if (kind == 1 && P.Sx.occluded[r]) continue;
-    float4 op = pool.op[r], ds = pool.ds[r];
-    float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
-    int spawn = kind ? (__float_as_int(pool.wd[r].w) >> 8) : -1;
+    if (valid && kind == 1 && P.Sx.occluded[r]) valid = false;
     // the 16 nearest (t, id) entries carry score weight
     float tt[kTopDepths];
     int ti[kTopDepths];
     int kept = 0, hits = 0;
-    for (int dd = 0; dd < nd; ++dd) {
-      float tmin;
-      if (!slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
-      // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
-      // spawning domain was tested right after shading
-      if (dd == spawn) continue;
-      ++hits;
-      atomicAdd(&lc[kind * nd + dd], 1u);
-      // insertion into the (t, id)-sorted top list (unrolled: stays in registers)
-      if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
-        float ct = tmin;
-        int ci = dd;
-#pragma unroll
-        for (int k = 0; k < kTopDepths; ++k) {
-          if (k < kept) {
-            if (ct < tt[k] || (ct == tt[k] && ci < ti[k])) {
-              float xt = tt[k];
-              int xi = ti[k];
-              tt[k] = ct;
-              ti[k] = ci;
-              ct = xt;
-              ci = xi;
-            }
-          } else if (k == kept) {
-            tt[k] = ct;
-            ti[k] = ci;
-          }
-        }
-        if (kept < kTopDepths) ++kept;
-      }
+    float o[3] = {0.f, 0.f, 0.f}, inv[3] = {1.f, 1.f, 1.f};
+    int spawn = -1;
+    if (valid) {
+      const RayPool& pool = kind ? P.S : P.R;
+      float4 op = pool.op[r], ds = pool.ds[r];
+      o[0] = op.x;
+      o[1] = op.y;
+      o[2] = op.z;
+      inv[0] = 1.0f / ds.x;
+      inv[1] = 1.0f / ds.y;
+      inv[2] = 1.0f / ds.z;
+      spawn = kind ? (__float_as_int(pool.wd[r].w) >> 8) : -1;
     }
-    for (int k = 0; k < kTopDepths; ++k)
-      if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
-    if (kind == 0) P.rcand[r] = (uint8_t)min(hits, 255);
+    // NaN-checked slab tests only in waves with an axis-parallel ray
+    const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
+    if (valid) {
+      hits = rayDomains(box, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti, &kept);
+#pragma unroll
+      for (int k = 0; k < kTopDepths; ++k)
+        if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
+      if (kind == 0) P.rcand[r] = (uint8_t)min(hits, 255);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
@@ -322,19 +345,36 @@ __global__ void __launch_bounds__(1024) kSchedule(TraceParams P) {
   }
 }
 
+// The domains of one ray: kWrite = false counts them per domain (lc),
+// kWrite = true writes the queue entries at the block's reserved slots.
+template <bool kSafe, bool kWrite>
+__device__ __forceinline__ void scatterDomains(const float4* box, int nd, const float o[3],
+                                               const float inv[3], int spawn, uint32_t* lc_kind,
+                                               const uint32_t* lb_kind, const TraceParams& P,
+                                               int kind, uint32_t r) {
+  for (int dd = 0; dd < nd; ++dd) {
+    float tmin;
+    if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, kSafe, &tmin) || dd == spawn) continue;
+    const uint32_t slot = atomicAdd(&lc_kind[dd], 1u);
+    if (kWrite) {
+      uint32_t dst = P.dbase[kind * nd + dd] + lb_kind[dd] + slot;
+      if (dst < P.q_cap[kind]) {
+        P.q[kind][dst] = QEntry{r, tmin};
+        P.qdom[kind][dst] = (uint16_t)dd;
+      }
+    }
+  }
+}
+
 // Scatter the queue entries (block-aggregated slot reservation).
 __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
   const int nd = P.ndomains;
-  float* boxes = (float*)sm;
-  uint32_t* lc = (uint32_t*)(boxes + 6 * nd);  // 2 * nd local counts
-  uint32_t* lb = lc + 2 * nd;                  // 2 * nd block bases
-  for (int i = threadIdx.x; i < nd; i += blockDim.x)
-    for (int k = 0; k < 3; ++k) {
-      boxes[i * 6 + k] = P.desc[i].lo[k];
-      boxes[i * 6 + 3 + k] = P.desc[i].hi[k];
-    }
+  float4* box = (float4*)sm;                 // 2 * nd: (lo, 0), (hi, 0)
+  uint32_t* lc = (uint32_t*)(box + 2 * nd);  // 2 * nd local counts
+  uint32_t* lb = lc + 2 * nd;                // 2 * nd block bases
+  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
   const DevCounters* c = P.ctr;
   const long long total = (long long)(c->r_end - c->r_beg) + (c->s_end - c->s_beg);
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -345,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
     int kind = 0;
     uint32_t r = 0;
     bool valid = passRay(P, i, &kind, &r);
-    float o[3] = {0, 0, 0}, d[3] = {1, 0, 0};
+    float o[3] = {0, 0, 0}, inv[3] = {1, 1, 1};
     int spawn = -1;
     if (valid) {
       const RayPool& pool = kind ? P.S : P.R;
@@ -353,21 +393,21 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
       o[0] = op.x;
       o[1] = op.y;
       o[2] = op.z;
-      d[0] = ds.x;
-      d[1] = ds.y;
-      d[2] = ds.z;
+      inv[0] = 1.0f / ds.x;
+      inv[1] = 1.0f / ds.y;
+      inv[2] = 1.0f / ds.z;
       if (kind) {
         spawn = __float_as_int(pool.wd[r].w) >> 8;
         if (P.Sx.occluded[r]) valid = false;  // occluded in its spawning domain
       }
     }
+    const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
     // pass 1: local counts
     if (valid) {
-      for (int dd = 0; dd < nd; ++dd) {
-        float tmin;
-        if (dd == spawn || !slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
-        atomicAdd(&lc[kind * nd + dd], 1u);
-      }
+      if (safe)
+        scatterDomains<true, false>(box, nd, o, inv, spawn, lc + kind * nd, nullptr, P, kind, r);
+      else
+        scatterDomains<false, false>(box, nd, o, inv, spawn, lc + kind * nd, nullptr, P, kind, r);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < 2 * nd; j += blockDim.x) {
@@ -378,16 +418,10 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
     __syncthreads();
     // pass 2: write
     if (valid) {
-      for (int dd = 0; dd < nd; ++dd) {
-        float tmin;
-        if (dd == spawn || !slabDomain(&boxes[dd * 6], &boxes[dd * 6 + 3], o, d, &tmin)) continue;
-        uint32_t slot = lb[kind * nd + dd] + atomicAdd(&lc[kind * nd + dd], 1u);
-        uint32_t dst = P.dbase[kind * nd + dd] + slot;
-        if (dst < P.q_cap[kind]) {
-          P.q[kind][dst] = QEntry{r, tmin};
-          P.qdom[kind][dst] = (uint16_t)dd;
-        }
-      }
+      if (safe)
+        scatterDomains<true, true>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
+      else
+        scatterDomains<false, true>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
     }
     __syncthreads();
   }
@@ -934,7 +968,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, cursor,
-      wctr, score, order, q[2], qdom[2], rcand, ovf, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa,
+      wctr, score, order, q[2], qdom[2], rcand, dbox, ovf, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa,
       qb;
   unsigned long long log_cap = 0;
   bool logging = false;
@@ -943,6 +977,7 @@ struct TracerState {
   hipGraphExec_t graph = nullptr;
   int graph_key[6] = {0, 0, 0, 0, 0, 0};
   std::vector<uint32_t> h_cnt;
+  std::vector<float> h_box;
   std::vector<int> h_order;
   // kernel timing of the trace launches (hipEvents on the tracer's stream)
   std::vector<hipEvent_t> ev;
@@ -986,7 +1021,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->cursor, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->ovf, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
+                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
                     &T->sh_tuv, &T->sh_prim, &T->log_ctr, &T->qa, &T->qb})
     b->release();
   delete T;
@@ -1073,6 +1108,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->s_occ, (size_t)T->s_cap * 4);
   need(T, T->vbuf, (size_t)S * kHistory * 8);
   need(T, T->rcand, (size_t)T->r_cap + 64);
+  need(T, T->dbox, (size_t)nd * 32 + 64);
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
   need(T, T->cnt, (size_t)2 * nd * 4 + 64);
@@ -1144,6 +1180,7 @@ static TraceParams makeParams(Ctx* c) {
   }
   const DomainPool& pool = c->pool;
   P.desc = pool.d_desc.as<DomainDesc>();
+  P.dbox = c->tracer->dbox.as<float4>();
   P.nodes = pool.nodes.as<BNode4>();
   P.tris = pool.tris.as<LTri>();
   P.vtx = pool.vtx.as<float>();
@@ -1204,8 +1241,8 @@ static TraceParams makeParams(Ctx* c) {
   return P;
 }
 
-static size_t countShmem(int nd) { return (size_t)nd * (6 + 3) * 4; }
-static size_t scatterShmem(int nd) { return (size_t)nd * (6 + 4) * 4; }
+static size_t countShmem(int nd) { return (size_t)nd * (8 + 3) * 4; }
+static size_t scatterShmem(int nd) { return (size_t)nd * (8 + 4) * 4; }
 
 static void ensureOverflow(TracerState* T, size_t lanes) { need(T, T->ovf, lanes * kOverflow * 4); }
 
@@ -1336,6 +1373,16 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     }
     TraceParams P = makeParams(c);
     P.ovf = T->ovf.as<uint32_t>();
+    {  // domain boxes for the ray/domain slab tests (scalar-loaded by the kernels)
+      const int nd = c->pool.ndomains;
+      T->h_box.assign((size_t)nd * 8, 0.f);
+      for (int d = 0; d < nd; ++d)
+        for (int k = 0; k < 3; ++k) {
+          T->h_box[(size_t)d * 8 + k] = c->pool.desc[d].lo[k];
+          T->h_box[(size_t)d * 8 + 4 + k] = c->pool.desc[d].hi[k];
+        }
+      SPRAY_HIP(hipMemcpyAsync(T->dbox.p, T->h_box.data(), (size_t)nd * 32, hipMemcpyHostToDevice, s));
+    }
     SPRAY_HIP(hipMemsetAsync(P.image, 0, (size_t)cfg.image_w * cfg.image_h * 16, s));
     SPRAY_HIP(hipMemsetAsync(P.ctr, 0, sizeof(DevCounters), s));
     int windows = (cfg.bounces + kHistory - 1) / kHistory;

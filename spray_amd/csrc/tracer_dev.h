@@ -55,6 +55,7 @@ struct TraceParams {
   DevLight lights[kMaxLights];
   // pool
   const DomainDesc* desc;
+  const float4* dbox;  // per domain: (lo, 0), (hi, 0)
   const BNode4* nodes;
   const LTri* tris;
   const float* vtx;

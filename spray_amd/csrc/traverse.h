@@ -81,10 +81,13 @@ struct TravCount {
 };
 
 // Tests the 4 child boxes of a node against [tnear, tfar]; returns a 4-bit
-// hit mask and the entry distances.
+// hit mask and the entry distances.  safe: a ray parallel to a slab with
+// its origin on one of the slab's planes gives 0 * inf = NaN there; that
+// axis then bounds nothing (only rays with an infinite 1/dir component
+// can produce it, so waves without such rays skip the check).
 __device__ __forceinline__ int nodeBoxes(const BNode4* __restrict__ nd, const float org[3],
                                          const float inv[3], float tnear, float tfar,
-                                         float tmin_o[4], int child_o[4]) {
+                                         float tmin_o[4], int child_o[4], bool safe) {
   const float4* q = (const float4*)nd;
   float4 lox = q[0], hix = q[1], loy = q[2], hiy = q[3], loz = q[4], hiz = q[5];
   int4 ch = ((const int4*)nd)[6];
@@ -101,9 +104,16 @@ __device__ __forceinline__ int nodeBoxes(const BNode4* __restrict__ nd, const fl
     float ax = (lx[i] - org[0]) * inv[0], bx = (hx[i] - org[0]) * inv[0];
     float ay = (ly[i] - org[1]) * inv[1], by = (hy[i] - org[1]) * inv[1];
     float az = (lz[i] - org[2]) * inv[2], bz = (hz[i] - org[2]) * inv[2];
-    // fminf/fmaxf drop a NaN operand (0 * inf on a slab plane): conservative
-    float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tnear));
-    float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tfar));
+    float lx0 = fminf(ax, bx), lx1 = fmaxf(ax, bx);
+    float ly0 = fminf(ay, by), ly1 = fmaxf(ay, by);
+    float lz0 = fminf(az, bz), lz1 = fmaxf(az, bz);
+    if (safe) {  // wave-uniform
+      if (ax != ax || bx != bx) lx0 = -kInfF, lx1 = kInfF;
+      if (ay != ay || by != by) ly0 = -kInfF, ly1 = kInfF;
+      if (az != az || bz != bz) lz0 = -kInfF, lz1 = kInfF;
+    }
+    float t0 = fmaxf(fmaxf(lx0, ly0), fmaxf(lz0, tnear));
+    float t1 = fminf(fminf(lx1, ly1), fminf(lz1, tfar));
     t1 = t1 * kSlabInflate;
     tmin_o[i] = t0;
     if (t0 <= t1 && lx[i] <= hx[i]) mask |= 1 << i;
@@ -179,7 +189,7 @@ __device__ __forceinline__ void cswap(float& ka, int& ia, float& kb, int& ib) {
 // One inner node: the hit children (inner nodes and leaves) in order of
 // entry distance; the nearest becomes `cur`, the others are pushed.
 __device__ __forceinline__ void travNode(TravState& s, LaneStack& st,
-                                         const BNode4* __restrict__ nodes) {
+                                         const BNode4* __restrict__ nodes, bool safe) {
   if (++s.guard > kMaxNodeVisits) {
     st.w |= 1u << 16;
     s.cur = s.pend = kStackEmpty;
@@ -187,7 +197,8 @@ __device__ __forceinline__ void travNode(TravState& s, LaneStack& st,
   }
   float tmin[4];
   int child[4];
-  int mask = nodeBoxes(&nodes[s.node_base + s.cur], s.org, s.inv, s.tnear, s.tlim, tmin, child);
+  const BNode4* nd = &nodes[s.node_base + s.cur];
+  int mask = nodeBoxes(nd, s.org, s.inv, s.tnear, s.tlim, tmin, child, safe);
   float ks[4];
   int m = 0;
 #pragma unroll
@@ -252,11 +263,13 @@ __device__ __forceinline__ bool travLeaf(TravState& s, uint32_t leaf, const LTri
 // busy lane holds a postponed leaf (or none can descend further).
 __device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravCount& cnt,
                                               bool active, const BNode4* __restrict__ nodes) {
+  // wave-uniform: NaN-checked box tests only when some ray is axis-parallel
+  const bool safe = __ballot(active && hasInfInv(s.inv)) != 0ull;
   while (true) {
     const bool step = active && isInnerRef(s.cur);
     cnt.nodes += (unsigned long long)__popcll(__ballot(step));
     if (step) {
-      travNode(s, st, nodes);
+      travNode(s, st, nodes, safe);
       if (isLeafRef(s.cur) && s.pend == kStackEmpty) {
         s.pend = s.cur;
         s.cur = st.pop();

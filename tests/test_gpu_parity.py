@@ -172,3 +172,57 @@ def test_pt_frame_small(c1, ref_pt_small, schedule, tiles):
     assert st["useful_rays"] == rst["useful_rays"]
     assert st["final_hits"] == rst["final_hits"]
     assert rmse <= 1e-4 and mx <= 1e-3
+
+
+def test_long_domain_lists(tmp_path):
+    """Rays along a row of 24 domains cross more than 16 of them (the
+    scheduler's score weighs the 16 nearest); the image centre's rays run
+    along faces shared by domains.  Frames still match the oracle."""
+    import spray_amd as sa
+    d = str(tmp_path)
+    sa.scenegen_write(d, "row", 24, 1, 1, 10, rgb=0xB3B3B3, extra_lines="light point 12 3 0.5 1 1 1")
+    cam = dict(pos=[-3.0, 0.5, 0.5], lookat=[24.0, 0.5, 0.5], up=[0.0, 1.0, 0.0])
+    kw = dict(image_w=64, image_h=64, pixel_samples=2, ao_mode=0, ao_samples=1, bounces=3, fov=20.0,
+              max_samples_per_rank=4096)
+    cfg = sa.default_config(has_camera=1, camera_pos=cam["pos"], camera_lookat=cam["lookat"],
+                            camera_up=cam["up"], nthreads=8, schedule=1, **kw)
+    ref = orc.Scene(os.path.join(d, "row.spray"), d)
+    rimg, rst = ref.render(nthreads=8, cam_pos=cam["pos"], cam_lookat=cam["lookat"], cam_up=cam["up"], **kw)
+    with sa.Context(cfg) as ctx:
+        ctx.load_scene(os.path.join(d, "row.spray"), d)
+        n, ids, ts = ctx.domain_list(np.tile(np.array(cam["pos"], np.float32), (1, 1)),
+                                     np.array([[1.0, 0.0, 0.0]], np.float32), 32)
+        assert n[0] >= 20
+        img, st = ctx.trace_frame()
+    rmse, mx = _img_close(img, rimg)
+    print(f"long lists: rmse {rmse:.3g} max {mx:.3g} useful {st['useful_rays']} oracle {rst['useful_rays']}")
+    assert st["useful_rays"] == rst["useful_rays"]
+    assert rmse <= 1e-4 and mx <= 1e-3
+
+
+def test_axis_parallel_rays(ctx_c1, ref_c1):
+    """Rays with zero direction components and origins on grid planes (0 * inf
+    in slab tests): closest hits, any hits and domain lists equal the oracle's."""
+    rng = np.random.default_rng(21)
+    n = 20000
+    org = rng.uniform(0.0, 4.0, (n, 3))
+    k = np.arange(n) % 3
+    org[np.arange(n), k] = np.round(org[np.arange(n), k] * 64) / 64  # on a cell plane
+    dirs = rng.normal(size=(n, 3))
+    dirs[np.arange(n), k] = 0.0
+    rows = np.arange(n // 4)
+    dirs[rows, (k[rows] + 1) % 3] = 0.0  # some rays along an axis
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    org, dirs = org.astype(np.float32), dirs.astype(np.float32)
+    cnt, ids, ts = ctx_c1.domain_list(org, dirs, 16)
+    rcnt, rids, rts = ref_c1.domain_list(org, dirs, 16)
+    assert np.array_equal(cnt, rcnt)
+    m = np.arange(16)[None, :] < np.minimum(cnt, 16)[:, None]
+    assert np.array_equal(ids[m], rids[m]) and np.array_equal(ts[m], rts[m])
+    for dom in (0, 21, 42):
+        t, u, v, p = ctx_c1.trace_rays(org, dirs, dom)
+        rt, ru, rv, rp = ref_c1.closest(dom, org, dirs)
+        assert (p >= 0).sum() > 100
+        assert np.array_equal(p, rp) and np.array_equal(t, rt), dom
+        occ = ctx_c1.trace_rays(org, dirs, dom, mode=1)
+        assert np.array_equal(occ.astype(bool), rp >= 0), dom

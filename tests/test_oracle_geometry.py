@@ -332,3 +332,49 @@ def test_tetrahedron_vertex_normals(tmp_path):
     # interpolation at a vertex returns that vertex's (unnormalised) normal
     _, ns = s.interpolate(0, np.array([0], np.int32), np.array([0.0], np.float32), np.array([0.0], np.float32))
     assert np.allclose(ns[0], n[f[0][0]])
+
+
+def test_axis_parallel_rays_on_box_planes(tmp_path):
+    """Rays with a zero direction component whose origin lies on a grid plane
+    of the mesh (0 * inf = NaN in a slab test): the BVH still finds exactly
+    the brute-force hits, and the domain list still holds every domain the
+    ray passes through (deviation Q13: NaN-safe slabs)."""
+    import spray_amd as sa
+    d = str(tmp_path)
+    sa.scenegen_write(d, "row", 6, 1, 1, 10, rgb=0xB3B3B3, extra_lines="light point 3 3 0.5 1 1 1")
+    s = orc.Scene(os.path.join(d, "row.spray"), d)
+    rng = np.random.default_rng(12)
+    n = 3000
+    org = np.zeros((n, 3), np.float32)
+    dirs = np.zeros((n, 3), np.float32)
+    for i in range(n):
+        k = i % 3                      # the axis the ray is parallel to
+        o = rng.uniform(0.0, 1.0, 3)
+        o[0] = rng.uniform(0.0, 6.0)
+        o[k] = np.round(o[k] * 10) / 10  # on a cell plane of the 10^3 grid
+        dv = rng.normal(size=3)
+        dv[k] = 0.0
+        org[i] = o
+        dirs[i] = dv / np.linalg.norm(dv)
+    assert (dirs == 0).sum() >= n
+    mism = 0
+    for dom in range(6):
+        t, _, _, p = s.closest(dom, org, dirs)
+        tb, _, _, pb = s.closest(dom, org, dirs, brute=True)
+        assert np.array_equal(p, pb) and np.array_equal(t, tb), dom
+        mism += int((p >= 0).sum())
+        occ = s.any_hit(dom, org, dirs, np.inf)
+        assert np.array_equal(occ, pb >= 0), dom
+    assert mism > 100
+    # a ray along the row inside the plane z = 0.5 crosses all six domains
+    cnt, ids, ts = s.domain_list(np.array([[-1.0, 0.5, 0.5]], np.float32),
+                                 np.array([[1.0, 0.0, 0.0]], np.float32))
+    assert cnt[0] == 6 and list(ids[0, :6]) == [0, 1, 2, 3, 4, 5]
+    # along the face y = 1 shared by two rows: only the upper row (half-open
+    # domain membership keeps exact ties between domains out)
+    sa.scenegen_write(d, "two", 3, 2, 1, 10, rgb=0xB3B3B3, extra_lines="light point 1 3 0.5 1 1 1")
+    s2 = orc.Scene(os.path.join(d, "two.spray"), d, parse_only=True)
+    assert s2.domain(3)[2][1] == 1.0 and s2.domain(0)[3][1] == 1.0  # the rows meet at y = 1
+    cnt, ids, ts = s2.domain_list(np.array([[-1.0, 1.0, 0.5]], np.float32),
+                                  np.array([[1.0, 0.0, 0.0]], np.float32))
+    assert cnt[0] == 3 and list(ids[0, :3]) == [3, 4, 5]
