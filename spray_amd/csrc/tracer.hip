@@ -578,7 +578,8 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
           // blinnPhong (render/reflection.h:202-214)
           f3 half = normalize(add(wi, wo));
           float ndh = clamp01(dot(nff, half));
-          float pw = (float)pow((double)ndh, (double)P.shininess);
+          // with ks = 0 the term ks * pw is exactly 0 (pw lies in [0, 1])
+          float pw = P.specular ? (float)pow((double)ndh, (double)P.shininess) : 0.0f;
           f3 bp = mul(Li, add(scl(surf, costheta), scl(ks, pw)));
           float sc = (L.type == kPointLight) ? (1.0f / pdf) : (1.0f / (pdf * (float)nsamp));
           Lr = scl(mul(Lin, bp), sc);
@@ -832,7 +833,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
 // Shading after the trace step: every hit whose history entry is still the
 // buffer's final entry for this pass is shaded once (stale speculative hits
 // would only spawn rays that can never retire, so they are skipped).
-__global__ void __launch_bounds__(kBlock) kShade(TraceParams P) {
+__global__ void __launch_bounds__(kBlock, 4) kShade(TraceParams P) {
   DevCounters* c = P.ctr;
   if (!c->active) return;
   const uint32_t n = min(c->shade_count, P.sh_cap);
@@ -1060,9 +1061,12 @@ static void need(TracerState* T, DevBuf& b, size_t bytes) {
 // numbering (RNG seeds), so the image does not depend on the batch size.
 static int tilesPerBatch(const Ctx* c, const std::vector<int>& hs) {
   const long long per_tile = (long long)c->cfg.image_w * hs[0] * c->cfg.pixel_samples;
-  long long k = c->tiles_in_flight > 0 ? c->tiles_in_flight
-                                       : std::max<long long>(1, kBatchSamples / per_tile);
-  return (int)std::max<long long>(1, std::min<long long>(k, (long long)hs.size()));
+  const long long n = (long long)hs.size();
+  if (c->tiles_in_flight > 0) return (int)std::min<long long>(c->tiles_in_flight, n);
+  // as many tiles as fit the sample budget, spread evenly over the batches
+  const long long kmax = std::max<long long>(1, std::min<long long>(n, kBatchSamples / per_tile));
+  const long long batches = (n + kmax - 1) / kmax;
+  return (int)((n + batches - 1) / batches);
 }
 
 static void ensureState(Ctx* c, double grow) {
@@ -1159,6 +1163,7 @@ static TraceParams makeParams(Ctx* c) {
   P.ao_max_dist = cfg.ao_max_dist;
   P.shininess = cfg.shininess;
   for (int k = 0; k < 3; ++k) P.ks[k] = cfg.ks[k];
+  P.specular = (cfg.ks[0] != 0.f || cfg.ks[1] != 0.f || cfg.ks[2] != 0.f) ? 1 : 0;
   P.inv_spp = 1.0 / (double)cfg.pixel_samples;
   P.inv_spp_f = (float)P.inv_spp;
   P.ndomains = c->pool.ndomains;

@@ -49,6 +49,7 @@ struct TraceParams {
   int tile_samples;  // samples of a full blocking tile (RNG seeds are tile-local)
   int ao_mode, ao_samples, bounces, rr_start, nlights;
   float ao_max_dist, shininess, ks[3];
+  int specular;  // any ks component non-zero (the specular term is ks * pow(.))
   float inv_spp_f;
   double inv_spp;
   int ndomains;
