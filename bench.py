@@ -70,6 +70,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-tiles", type=int, default=0,
                     help="blocking tiles timed on the CPU (0: the workload's default)")
+    ap.add_argument("--bvh", default="lbvh", choices=["ploc", "lbvh"], help="per-domain BVH builder")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default batch parameters")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every pass directly (kernel-level profilers)")
@@ -128,6 +129,7 @@ def main():
                  camera_up=cam["up"]) if cam else dict(has_camera=0)
     cfg = sa.default_config(nthreads=min(16, os.cpu_count() or 1), **camkw, **kw)
     ctx = sa.Context(cfg)
+    ctx.set_bvh_builder({"lbvh": 0, "ploc": 1}[args.bvh])
     t0 = time.time()
     ctx.load_scene(os.path.join(tmp, args.workload + ".spray"), tmp)
     load_s = time.time() - t0

@@ -162,6 +162,13 @@ int spray_set_batch_params(spray_ctx* ctx, int chunk, int tile);
  * not depend on it.  0 (default) groups tiles up to about 8M samples. */
 int spray_set_tiles_in_flight(spray_ctx* ctx, int tiles);
 
+/* BVH builder used by the next spray_load_scene: 0 = Karras LBVH (default),
+ * 1 = PLOC clustering over Morton-ordered triangles.  Stands in for the
+ * Embree scene build-quality flags of the per-domain scenes
+ * (render/trimesh_buffer.cc:112, RTC_SCENE_DYNAMIC); hits do not depend on
+ * the choice, only traversal cost does. */
+int spray_set_bvh_builder(spray_ctx* ctx, int builder);
+
 /* Kernel timing (new): when enabled, every trace-kernel launch (the
  * per-domain kernel of the plain form, the persistent kernel of the batched
  * forms) is bracketed by hipEvents on the context's stream (the batched

@@ -102,6 +102,7 @@ PROTOTYPES = {
     "spray_autotune": (C.c_int, [_P, _IP, _IP]),
     "spray_set_batch_params": (C.c_int, [_P, C.c_int, C.c_int]),
     "spray_set_tiles_in_flight": (C.c_int, [_P, C.c_int]),
+    "spray_set_bvh_builder": (C.c_int, [_P, C.c_int]),
     "spray_profile_kernels": (C.c_int, [_P, C.c_int]),
     "spray_kernel_time": (C.c_int, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.c_int]),
     "spray_predictor_log": (C.c_int, [_P, C.c_int, C.c_longlong]),

@@ -135,6 +135,10 @@ class Context:
     def set_tiles_in_flight(self, tiles):
         self._check(self.lib.spray_set_tiles_in_flight(self.h, int(tiles)))
 
+    def set_bvh_builder(self, builder):
+        """0 = Karras LBVH (default), 1 = PLOC; applies to the next load_scene."""
+        self._check(self.lib.spray_set_bvh_builder(self.h, int(builder)))
+
     def autotune(self):
         ch, tl = C.c_int(), C.c_int()
         self._check(self.lib.spray_autotune(self.h, C.byref(ch), C.byref(tl)))

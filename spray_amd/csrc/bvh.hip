@@ -1,5 +1,6 @@
-// HBM domain pool: vertex normals, LBVH build (Morton codes, LDS radix
-// sort, Karras hierarchy, atomic bottom-up refit, collapse to a 4-wide BVH)
+// HBM domain pool: vertex normals, BVH build (Morton codes, LDS radix sort,
+// PLOC clustering or a Karras hierarchy with atomic bottom-up refit,
+// collapse to a 4-wide BVH)
 // and the batch ray queries behind spray_trace_rays / spray_domain_list.
 //
 // Replaces the reference's per-domain Embree scene (rtcCommit in
@@ -351,6 +352,20 @@ __global__ void kCollapse(const FrontierItem* __restrict__ fin, int nin, Frontie
       ch[best] = left[c];
       ch[nc++] = right[c];
     }
+    // the inner children of a node take consecutive slots (siblings share
+    // cache lines when a ray enters several of them)
+    int ninner = 0;
+    for (int i = 0; i < nc; ++i) ninner += (ch[i] >= 0 && count(ch[i]) > kLeafMax) ? 1 : 0;
+    int slot = 0, q = 0;
+    if (ninner > 0) {
+      slot = atomicAdd(node_counter, ninner);
+      if (slot + ninner > node_cap) {
+        atomicExch(overflow, 1);
+        ninner = 0;
+      } else {
+        q = atomicAdd(nout, ninner);
+      }
+    }
     BNode4 nd;
     for (int i = 0; i < 4; ++i) {
       float b[6];
@@ -369,16 +384,11 @@ __global__ void kCollapse(const FrontierItem* __restrict__ fin, int nin, Frontie
           word = (int)(0x80000000u | (1u << 24) | (uint32_t)(~c));
         } else if (count(c) <= kLeafMax) {
           word = (int)(0x80000000u | ((uint32_t)count(c) << 24) | (uint32_t)first[c]);
+        } else if (ninner == 0) {  // capacity exceeded (reported through overflow)
+          word = (int)0x80000000u;
         } else {
-          int slot = atomicAdd(node_counter, 1);
-          if (slot >= node_cap) {
-            atomicExch(overflow, 1);
-            word = (int)0x80000000u;
-          } else {
-            int q = atomicAdd(nout, 1);
-            fout[q] = FrontierItem{c, slot};
-            word = slot;
-          }
+          fout[q++] = FrontierItem{c, slot};
+          word = slot++;
         }
       }
       nd.lox[i] = b[0];
@@ -415,6 +425,154 @@ __global__ void kTinyRoot(int nf, const float* __restrict__ leafbox, BNode4* nod
     nd.pad[i] = 0;
   }
   nodes[0] = nd;
+}
+
+// ---------------------------------------------------------------------------
+// PLOC hierarchy (Meister & Bittner, "Parallel Locally-Ordered Clustering
+// for Bounding Volume Hierarchy Construction", IEEE TVCG 2018).  Clusters
+// start as the leaves in Morton order.  Each pass pairs every cluster with
+// the neighbour within +-kPlocRadius array positions whose union box has the
+// least surface area, merges mutual pairs into a new inner node that takes
+// the lower position, and compacts the array in order.  Distance ties go to
+// the lexicographically smallest (lower, upper) position pair, a strict
+// order on pairs, so the globally closest pair is always mutual and every
+// pass merges at least one pair.  Inner node ids follow creation order; the
+// root is the last one (nf - 2).
+constexpr int kPlocRadius = 16;
+constexpr int kPlocBlock = 256;
+
+__global__ void kPlocInit(const float* __restrict__ vtx, const uint32_t* __restrict__ face, int nf,
+                          const uint32_t* __restrict__ order, int* cid, float* cbox) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+    uint32_t p = order[i];
+    const float* a = vtx + (size_t)face[p * 3] * 3;
+    const float* b = vtx + (size_t)face[p * 3 + 1] * 3;
+    const float* c = vtx + (size_t)face[p * 3 + 2] * 3;
+    for (int k = 0; k < 3; ++k) {
+      cbox[(size_t)i * 6 + k] = fminf(a[k], fminf(b[k], c[k]));
+      cbox[(size_t)i * 6 + 3 + k] = fmaxf(a[k], fmaxf(b[k], c[k]));
+    }
+    cid[i] = ~i;
+  }
+}
+
+__device__ __forceinline__ float unionArea(const float* a, const float* b) {
+  float dx = fmaxf(a[3], b[3]) - fminf(a[0], b[0]);
+  float dy = fmaxf(a[4], b[4]) - fminf(a[1], b[1]);
+  float dz = fmaxf(a[5], b[5]) - fminf(a[2], b[2]);
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// nn[i] = nearest neighbour of cluster i; the block's boxes plus a halo of
+// kPlocRadius on each side are staged in LDS.
+__global__ void __launch_bounds__(kPlocBlock) kPlocNearest(const float* __restrict__ cbox, int n,
+                                                            int* __restrict__ nn) {
+  constexpr int kSpan = kPlocBlock + 2 * kPlocRadius;
+  __shared__ float sb[kSpan * 6];
+  const int lo = blockIdx.x * kPlocBlock - kPlocRadius;
+  for (int t = threadIdx.x; t < kSpan * 6; t += kPlocBlock) {
+    int g = lo * 6 + t;
+    sb[t] = (g >= 0 && g < n * 6) ? cbox[g] : 0.f;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * kPlocBlock + threadIdx.x;
+  if (i >= n) return;
+  float me[6];
+  for (int k = 0; k < 6; ++k) me[k] = sb[(i - lo) * 6 + k];
+  const int jlo = max(i - kPlocRadius, 0), jhi = min(i + kPlocRadius, n - 1);
+  float best = kInfF;
+  int bj = -1;
+  // ascending j visits the candidate pairs in ascending (lower, upper) order,
+  // so a strict comparison keeps the smallest pair among equal areas
+  for (int j = jlo; j <= jhi; ++j) {
+    if (j == i) continue;
+    float a = unionArea(me, sb + (j - lo) * 6);
+    if (a < best || bj < 0) {
+      best = a;
+      bj = j;
+    }
+  }
+  nn[i] = bj;
+}
+
+// mflag[i]: cluster i starts a merge (mutual pair, lower position);
+// kflag[i]: cluster i keeps a position after the pass.
+__global__ void kPlocFlags(const int* __restrict__ nn, int n, uint32_t* mflag, uint32_t* kflag) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
+    if (i == n) {
+      mflag[n] = 0u;
+      kflag[n] = 0u;
+      continue;
+    }
+    int j = nn[i];
+    bool mutual = nn[j] == i;
+    mflag[i] = (mutual && i < j) ? 1u : 0u;
+    kflag[i] = (mutual && j < i) ? 0u : 1u;
+  }
+}
+
+__global__ void kPlocMerge(const int* __restrict__ nn, int n, const int* __restrict__ cid,
+                           const float* __restrict__ cbox, const uint32_t* __restrict__ mpos,
+                           const uint32_t* __restrict__ kpos, int node_base, int* cid2,
+                           float* cbox2, int* left, int* right, float* innerbox, int* icount) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int j = nn[i];
+    bool mutual = nn[j] == i;
+    if (mutual && j < i) continue;
+    uint32_t o = kpos[i];
+    if (mutual) {
+      int id = node_base + (int)mpos[i];
+      int a = cid[i], b = cid[j];
+      left[id] = a;
+      right[id] = b;
+      icount[id] = (a < 0 ? 1 : icount[a]) + (b < 0 ? 1 : icount[b]);
+      for (int k = 0; k < 3; ++k) {
+        float l = fminf(cbox[(size_t)i * 6 + k], cbox[(size_t)j * 6 + k]);
+        float h = fmaxf(cbox[(size_t)i * 6 + 3 + k], cbox[(size_t)j * 6 + 3 + k]);
+        innerbox[(size_t)id * 6 + k] = l;
+        innerbox[(size_t)id * 6 + 3 + k] = h;
+        cbox2[(size_t)o * 6 + k] = l;
+        cbox2[(size_t)o * 6 + 3 + k] = h;
+      }
+      cid2[o] = id;
+    } else {
+      for (int k = 0; k < 6; ++k) cbox2[(size_t)o * 6 + k] = cbox[(size_t)i * 6 + k];
+      cid2[o] = cid[i];
+    }
+  }
+}
+
+// Depth-first leaf ranges, top-down over the nodes one pass created (their
+// parents come from later passes and are already placed).  Leaf children are
+// renumbered to their depth-first position q, and perm[q] keeps the Morton
+// position they came from.
+__global__ void kPlocRanges(int id_lo, int id_hi, int* left, int* right,
+                            const int* __restrict__ icount, int* first, int* last, uint32_t* perm) {
+  for (int id = id_lo + blockIdx.x * blockDim.x + threadIdx.x; id < id_hi;
+       id += gridDim.x * blockDim.x) {
+    int f = first[id];
+    int a = left[id], b = right[id];
+    int fb = f + (a < 0 ? 1 : icount[a]);
+    if (a < 0) {
+      perm[f] = (uint32_t)(~a);
+      left[id] = ~f;
+    } else {
+      first[a] = f;
+    }
+    if (b < 0) {
+      perm[fb] = (uint32_t)(~b);
+      right[id] = ~fb;
+    } else {
+      first[b] = fb;
+    }
+    last[id] = f + icount[id] - 1;
+  }
+}
+
+__global__ void kComposeOrder(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ morton,
+                              int n, uint32_t* order) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    order[q] = morton[perm[q]];
 }
 
 // ---------------------------------------------------------------------------
@@ -501,6 +659,13 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
   scal.reserve(64);
   DevBuf adj;
   adj.reserve((size_t)maxf * 12 + 64);
+  DevBuf pc, pc2, pb, pb2, pnn, pcnt, pmf, pkf, pmp, pkp, pperm, pord;
+  if (c->bvh_builder == 1) {
+    for (DevBuf* b : {&pc, &pc2, &pnn, &pcnt, &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
+      b->reserve((size_t)maxf * 4 + 64);
+    pb.reserve((size_t)maxf * 24 + 64);
+    pb2.reserve((size_t)maxf * 24 + 64);
+  }
   P.bvh_depth.assign(nd, 0);
   for (int d = 0; d < nd; ++d) {
     DomainDesc& D = P.desc[d];
@@ -541,9 +706,9 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
     LTri* tris = P.tris.as<LTri>() + D.tri_off;
     BNode4* nodes = P.nodes.as<BNode4>() + D.node_off;
     float* leafbox = lb.as<float>();
-    kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris);
-    int node_cap = nf / 2 + 16;
+    const int node_cap = nf / 2 + 16;
     if (nf <= kLeafMax) {
+      kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris);
       kTinyRoot<<<1, 1, 0, s>>>(nf, leafbox, nodes);
       D.num_nodes = 1;
       P.bvh_depth[d] = 1;
@@ -553,38 +718,103 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
     int* R = rgt.as<int>();
     int* F = fst.as<int>();
     int* Lst = lst.as<int>();
-    int* PI = pin.as<int>();
-    int* PL = plf.as<int>();
-    SPRAY_HIP(hipMemsetAsync(PI, 0xff, (size_t)nf * 4, s));  // root parent = -1
-    kKarras<<<gridFor(nf, 256), 256, 0, s>>>(ka, nf, L, R, F, Lst, PI, PL);
-    SPRAY_HIP(hipMemsetAsync(flags.p, 0, (size_t)nf * 4, s));
-    kRefit<<<gridFor(nf, 256), 256, 0, s>>>(nf, L, R, PI, PL, leafbox, ib.as<float>(),
-                                            flags.as<uint32_t>());
-    // collapse
-    int* ctr = (int*)(scal.as<uint32_t>() + 8);  // [node_counter, nout, overflow]
-    int hinit[3] = {1, 0, 0};
-    SPRAY_HIP(hipMemcpyAsync(ctr, hinit, 12, hipMemcpyHostToDevice, s));
-    FrontierItem root{0, 0};
-    SPRAY_HIP(hipMemcpyAsync(fa.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
-    FrontierItem *fin = fa.as<FrontierItem>(), *fout = fb.as<FrontierItem>();
-    int nin = 1, depth = 0;
-    while (nin > 0) {
-      SPRAY_HIP(hipMemsetAsync(ctr + 1, 0, 4, s));
-      kCollapse<<<gridFor(nin, 256), 256, 0, s>>>(fin, nin, fout, ctr + 1, ctr, node_cap, L, R, F,
-                                                   Lst, leafbox, ib.as<float>(), nodes, ctr + 2);
-      int h[3];
-      SPRAY_HIP(hipMemcpyAsync(h, ctr, 12, hipMemcpyDeviceToHost, s));
-      SPRAY_HIP(hipStreamSynchronize(s));
-      if (h[2]) throw StatusError(SPRAY_ERR_CAPACITY, "BVH node capacity exceeded");
-      nin = h[1];
-      D.num_nodes = h[0];
-      std::swap(fin, fout);
-      ++depth;
+    float* inner = ib.as<float>();
+    // Collapse the binary tree under `root` into 4-wide nodes; returns the
+    // number of 4-wide levels.
+    auto collapse = [&](int root) {
+      int* ctr = (int*)(scal.as<uint32_t>() + 8);  // [node_counter, nout, overflow]
+      int hinit[3] = {1, 0, 0};
+      SPRAY_HIP(hipMemcpyAsync(ctr, hinit, 12, hipMemcpyHostToDevice, s));
+      FrontierItem item{root, 0};
+      SPRAY_HIP(hipMemcpyAsync(fa.p, &item, sizeof(item), hipMemcpyHostToDevice, s));
+      FrontierItem *fin = fa.as<FrontierItem>(), *fout = fb.as<FrontierItem>();
+      int nin = 1, depth = 0;
+      while (nin > 0) {
+        SPRAY_HIP(hipMemsetAsync(ctr + 1, 0, 4, s));
+        kCollapse<<<gridFor(nin, 256), 256, 0, s>>>(fin, nin, fout, ctr + 1, ctr, node_cap, L, R,
+                                                     F, Lst, leafbox, inner, nodes, ctr + 2);
+        int h[3];
+        SPRAY_HIP(hipMemcpyAsync(h, ctr, 12, hipMemcpyDeviceToHost, s));
+        SPRAY_HIP(hipStreamSynchronize(s));
+        if (h[2]) throw StatusError(SPRAY_ERR_CAPACITY, "BVH node capacity exceeded");
+        nin = h[1];
+        D.num_nodes = h[0];
+        std::swap(fin, fout);
+        ++depth;
+      }
+      return depth;
+    };
+    // worst-case stack need: three pushes per 4-wide level
+    auto fitsStack = [](int depth) { return 3 * depth + 1 <= kRing + kOverflow; };
+    int depth = -1;
+    if (c->bvh_builder == 1) {
+      // PLOC passes until one cluster is left
+      int* cid = pc.as<int>();
+      int* cid2 = pc2.as<int>();
+      float* cbox = pb.as<float>();
+      float* cbox2 = pb2.as<float>();
+      int* nn = pnn.as<int>();
+      int* icount = pcnt.as<int>();
+      uint32_t* mflag = pmf.as<uint32_t>();
+      uint32_t* kflag = pkf.as<uint32_t>();
+      uint32_t* mpos = pmp.as<uint32_t>();
+      uint32_t* kpos = pkp.as<uint32_t>();
+      kPlocInit<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, cid, cbox);
+      std::vector<int> pass_base{0};
+      int n = nf, made = 0;
+      bool ok = true;
+      while (n > 1) {
+        kPlocNearest<<<(n + kPlocBlock - 1) / kPlocBlock, kPlocBlock, 0, s>>>(cbox, n, nn);
+        kPlocFlags<<<gridFor(n + 1, 256), 256, 0, s>>>(nn, n, mflag, kflag);
+        exclusiveScan(mflag, mpos, n + 1, s);
+        exclusiveScan(kflag, kpos, n + 1, s);
+        kPlocMerge<<<gridFor(n, 256), 256, 0, s>>>(nn, n, cid, cbox, mpos, kpos, made, cid2, cbox2,
+                                                    L, R, inner, icount);
+        uint32_t h[2];
+        SPRAY_HIP(hipMemcpyAsync(&h[0], mpos + n, 4, hipMemcpyDeviceToHost, s));
+        SPRAY_HIP(hipMemcpyAsync(&h[1], kpos + n, 4, hipMemcpyDeviceToHost, s));
+        SPRAY_HIP(hipStreamSynchronize(s));
+        if (h[0] == 0 || (int)h[1] != n - (int)h[0]) {  // no progress: non-finite boxes
+          ok = false;
+          break;
+        }
+        made += (int)h[0];
+        pass_base.push_back(made);
+        n = (int)h[1];
+        std::swap(cid, cid2);
+        std::swap(cbox, cbox2);
+      }
+      if (ok) {
+        // depth-first leaf order: roots of later passes first
+        const int root = made - 1;
+        uint32_t* perm = pperm.as<uint32_t>();
+        SPRAY_HIP(hipMemsetAsync(F + root, 0, 4, s));
+        for (int p = (int)pass_base.size() - 1; p > 0; --p) {
+          int lo = pass_base[p - 1], hi = pass_base[p];
+          kPlocRanges<<<gridFor(hi - lo, 256), 256, 0, s>>>(lo, hi, L, R, icount, F, Lst, perm);
+        }
+        uint32_t* order = pord.as<uint32_t>();
+        kComposeOrder<<<gridFor(nf, 256), 256, 0, s>>>(perm, va, nf, order);
+        kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, order, leafbox, tris);
+        depth = collapse(root);
+        if (!fitsStack(depth)) depth = -1;
+      }
+    }
+    if (depth < 0) {
+      // Karras LBVH: leaves in Morton order, bottom-up refit
+      int* PI = pin.as<int>();
+      int* PL = plf.as<int>();
+      kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris);
+      SPRAY_HIP(hipMemsetAsync(PI, 0xff, (size_t)nf * 4, s));  // root parent = -1
+      kKarras<<<gridFor(nf, 256), 256, 0, s>>>(ka, nf, L, R, F, Lst, PI, PL);
+      SPRAY_HIP(hipMemsetAsync(flags.p, 0, (size_t)nf * 4, s));
+      kRefit<<<gridFor(nf, 256), 256, 0, s>>>(nf, L, R, PI, PL, leafbox, inner,
+                                              flags.as<uint32_t>());
+      depth = collapse(0);
+      if (!fitsStack(depth))
+        throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
     }
     P.bvh_depth[d] = depth;
-    // worst-case stack need: three pushes per level
-    if (3 * depth + 1 > kRing + kOverflow)
-      throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
   }
   SPRAY_HIP(hipStreamSynchronize(s));
   long long used = 0;
@@ -593,7 +823,8 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
   P.d_desc.reserve(sizeof(DomainDesc) * nd);
   SPRAY_HIP(hipMemcpy(P.d_desc.p, P.desc.data(), sizeof(DomainDesc) * nd, hipMemcpyHostToDevice));
   for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
-                    &plf, &flags, &fa, &fb, &cnts, &scal, &adj})
+                    &plf, &flags, &fa, &fb, &cnts, &scal, &adj, &pc, &pc2, &pb, &pb2, &pnn, &pcnt,
+                    &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
     b->release();
   P.build_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

@@ -402,6 +402,14 @@ int spray_set_tiles_in_flight(spray_ctx* ctx, int tiles) {
   });
 }
 
+int spray_set_bvh_builder(spray_ctx* ctx, int builder) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (builder < 0 || builder > 1) throw StatusError(SPRAY_ERR_INVALID, "builder must be 0 or 1");
+    ctx->c.bvh_builder = builder;
+  });
+}
+
 int spray_profile_kernels(spray_ctx* ctx, int enable) {
   if (!ctx) return SPRAY_ERR_INVALID;
   ctx->c.profile = enable != 0;

@@ -101,6 +101,7 @@ struct Ctx {
   Predictor* pred = nullptr;
   int batch_chunk = 16;  // refill threshold of the trace kernel (idle lanes per wave)
   int tiles_in_flight = 0;  // blocking tiles traced together (batched forms); 0 = automatic
+  int bvh_builder = 0;     // 0 = Karras LBVH, 1 = PLOC (next scene load)
   int batch_tile = 4;      // waves per block of the persistent kernel (block = 64 * tile)
   int param_version = 0;   // bumped when camera / config change (graph re-capture)
   bool profile = false;    // time every trace-kernel launch with hipEvents

@@ -31,6 +31,9 @@
 namespace spray {
 
 constexpr int kBlock = 256;
+// grid of kCount and kScatter (they must match: blocks own queue slots)
+constexpr int kQueueGrid = 1024;
+static_assert(kQueueGrid % kBlock == 0, "kBlockBases splits the grid evenly over its threads");
 
 __device__ __forceinline__ void flushCount(unsigned long long* dst, unsigned long long v) {
   v = waveSumULL(v);
@@ -149,8 +152,6 @@ __global__ void kClearQ(TraceParams P) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += gridDim.x * blockDim.x) {
     P.cnt[i] = 0;
     P.cnt[nd + i] = 0;
-    P.cursor[i] = 0;
-    P.cursor[nd + i] = 0;
     P.score[i] = 0;
     P.wctr[i] = 0;
   }
@@ -272,6 +273,10 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     }
   }
   __syncthreads();
+  // per-block counts, [kind][domain][block]: kScatter's blocks take the same
+  // rays and place their entries at the bases these counts give
+  for (int j = threadIdx.x; j < 2 * nd; j += blockDim.x)
+    P.bcnt[(size_t)j * kQueueGrid + blockIdx.x] = lc[j];
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
     if (lc[i]) atomicAdd(&P.cnt[i], lc[i]);
     if (lc[nd + i]) atomicAdd(&P.cnt[nd + i], lc[nd + i]);
@@ -345,9 +350,9 @@ __global__ void __launch_bounds__(1024) kSchedule(TraceParams P) {
   }
 }
 
-// The domains of one ray: kWrite = false counts them per domain (lc),
-// kWrite = true writes the queue entries at the block's reserved slots.
-template <bool kSafe, bool kWrite>
+// The queue entries of one ray, at its block's base plus the block-local
+// slot of each domain.
+template <bool kSafe>
 __device__ __forceinline__ void scatterDomains(const float4* box, int nd, const float o[3],
                                                const float inv[3], int spawn, uint32_t* lc_kind,
                                                const uint32_t* lb_kind, const TraceParams& P,
@@ -355,32 +360,58 @@ __device__ __forceinline__ void scatterDomains(const float4* box, int nd, const 
   for (int dd = 0; dd < nd; ++dd) {
     float tmin;
     if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, kSafe, &tmin) || dd == spawn) continue;
-    const uint32_t slot = atomicAdd(&lc_kind[dd], 1u);
-    if (kWrite) {
-      uint32_t dst = P.dbase[kind * nd + dd] + lb_kind[dd] + slot;
-      if (dst < P.q_cap[kind]) {
-        P.q[kind][dst] = QEntry{r, tmin};
-        P.qdom[kind][dst] = (uint16_t)dd;
-      }
+    const uint32_t dst = lb_kind[dd] + atomicAdd(&lc_kind[dd], 1u);
+    if (dst < P.q_cap[kind]) {
+      P.q[kind][dst] = QEntry{r, tmin};
+      P.qdom[kind][dst] = (uint16_t)dd;
     }
   }
 }
 
-// Scatter the queue entries (block-aggregated slot reservation).
+// Queue base of every (kind, domain, block of kCount/kScatter): the
+// domain's base in schedule order plus the entries of the lower blocks.
+// One block per (kind, domain).
+__global__ void __launch_bounds__(kBlock) kBlockBases(TraceParams P) {
+  __shared__ uint32_t wsum[16 + 1];
+  if (!P.ctr->active) return;
+  const int j = blockIdx.x;  // kind * nd + domain
+  constexpr int kPer = kQueueGrid / kBlock;
+  const uint32_t* src = P.bcnt + (size_t)j * kQueueGrid + threadIdx.x * kPer;
+  uint32_t v[kPer], sum = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    v[k] = src[k];
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t pre = P.dbase[j] + blockScanExcl(sum, wsum, &total);
+  uint32_t* dst = P.bbase + (size_t)j * kQueueGrid + threadIdx.x * kPer;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    dst[k] = pre;
+    pre += v[k];
+  }
+}
+
+// Scatter the queue entries: the same grid and ray-to-block mapping as
+// kCount, so each block owns the slots kBlockBases reserved for it.
 __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
   const int nd = P.ndomains;
   float4* box = (float4*)sm;                 // 2 * nd: (lo, 0), (hi, 0)
-  uint32_t* lc = (uint32_t*)(box + 2 * nd);  // 2 * nd local counts
+  uint32_t* lc = (uint32_t*)(box + 2 * nd);  // 2 * nd block-local slots
   uint32_t* lb = lc + 2 * nd;                // 2 * nd block bases
-  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
+  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) {
+    box[i] = P.dbox[i];
+    lc[i] = 0;
+    lb[i] = P.bbase[(size_t)i * kQueueGrid + blockIdx.x];
+  }
+  __syncthreads();
   const DevCounters* c = P.ctr;
   const long long total = (long long)(c->r_end - c->r_beg) + (c->s_end - c->s_beg);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long base = blockIdx.x * (long long)blockDim.x; base < total; base += stride) {
-    for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) lc[i] = 0;
-    __syncthreads();
     long long i = base + threadIdx.x;
     int kind = 0;
     uint32_t r = 0;
@@ -402,28 +433,12 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
       }
     }
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
-    // pass 1: local counts
     if (valid) {
       if (safe)
-        scatterDomains<true, false>(box, nd, o, inv, spawn, lc + kind * nd, nullptr, P, kind, r);
+        scatterDomains<true>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
       else
-        scatterDomains<false, false>(box, nd, o, inv, spawn, lc + kind * nd, nullptr, P, kind, r);
+        scatterDomains<false>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
     }
-    __syncthreads();
-    for (int j = threadIdx.x; j < 2 * nd; j += blockDim.x) {
-      uint32_t n = lc[j];
-      lb[j] = n ? atomicAdd(&P.cursor[j], n) : 0u;
-      lc[j] = 0;
-    }
-    __syncthreads();
-    // pass 2: write
-    if (valid) {
-      if (safe)
-        scatterDomains<true, true>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
-      else
-        scatterDomains<false, true>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
-    }
-    __syncthreads();
   }
 }
 
@@ -968,7 +983,7 @@ struct TracerState {
   long long S = 0;  // samples of the largest tile
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
-  DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, cursor,
+  DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
       wctr, score, order, q[2], qdom[2], rcand, dbox, ovf, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa,
       qb;
   unsigned long long log_cap = 0;
@@ -1021,7 +1036,7 @@ void tracerRelease(Ctx* c) {
   for (auto* arr : {T->R, T->Pn, T->Sh})
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
-                    &T->off, &T->dbase, &T->cursor, &T->wctr, &T->score, &T->order, &T->q[0],
+                    &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->score, &T->order, &T->q[0],
                     &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
                     &T->sh_tuv, &T->sh_prim, &T->log_ctr, &T->qa, &T->qb})
     b->release();
@@ -1118,7 +1133,8 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->cnt, (size_t)2 * nd * 4 + 64);
   need(T, T->off, (size_t)2 * (nd + 1) * 4 + 64);
   need(T, T->dbase, (size_t)2 * nd * 4 + 64);
-  need(T, T->cursor, (size_t)2 * nd * 4 + 64);
+  need(T, T->bcnt, (size_t)2 * nd * kQueueGrid * 4 + 64);
+  need(T, T->bbase, (size_t)2 * nd * kQueueGrid * 4 + 64);
   need(T, T->wctr, (size_t)(nd + 3) * 4 + 64);
   need(T, T->qa, (size_t)T->q_cap[0] * 4);
   need(T, T->qb, (size_t)T->q_cap[0] * 4);
@@ -1216,7 +1232,8 @@ static TraceParams makeParams(Ctx* c) {
   P.cnt = T->cnt.as<uint32_t>();
   P.off = T->off.as<uint32_t>();
   P.dbase = T->dbase.as<uint32_t>();
-  P.cursor = T->cursor.as<uint32_t>();
+  P.bcnt = T->bcnt.as<uint32_t>();
+  P.bbase = T->bbase.as<uint32_t>();
   P.wctr = T->wctr.as<uint32_t>();
   P.score = T->score.as<unsigned long long>();
   P.order = T->order.as<int>();
@@ -1247,7 +1264,7 @@ static TraceParams makeParams(Ctx* c) {
 }
 
 static size_t countShmem(int nd) { return (size_t)nd * (8 + 3) * 4; }
-static size_t scatterShmem(int nd) { return (size_t)nd * (8 + 4) * 4; }
+static size_t scatterShmem(int nd) { return (size_t)nd * (8 + 4) * 4; }  // boxes, slots, bases
 
 static void ensureOverflow(TracerState* T, size_t lanes) { need(T, T->ovf, lanes * kOverflow * 4); }
 
@@ -1258,9 +1275,10 @@ static void passQueues(Ctx* c, const TraceParams& P) {
   int nd = P.ndomains;
   kPassBegin<<<1, 1, 0, s>>>(P);
   kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
-  kCount<<<1024, kBlock, countShmem(nd), s>>>(P);
+  kCount<<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
   kSchedule<<<1, 1024, 0, s>>>(P);
-  kScatter<<<1024, kBlock, scatterShmem(nd), s>>>(P);
+  kBlockBases<<<2 * nd, kBlock, 0, s>>>(P);
+  kScatter<<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);
 }
 
 static void passFinish(Ctx* c, const TraceParams& P) {

@@ -76,7 +76,8 @@ struct TraceParams {
   uint32_t* cnt;      // [2][nd] entries per domain (radiance, shadow)
   uint32_t* off;      // [2][nd + 1] queue offsets in schedule order
   uint32_t* dbase;    // [2][nd] queue offset of each domain (by id)
-  uint32_t* cursor;   // [2][nd]
+  uint32_t* bcnt;     // [2][nd][queue grid] entries per block (kCount)
+  uint32_t* bbase;    // [2][nd][queue grid] queue base per block (kScatter)
   uint32_t* wctr;     // [nd + 3] work counters: per domain (plain), phase A / B, local shadows
   unsigned long long* score;  // [nd]
   int* order;         // [nd] schedule

@@ -24,23 +24,28 @@ constexpr float kSlabInflate = 1.0000003576278687f;
 constexpr uint32_t kMaxNodeVisits = 1u << 22;
 
 // The stack state is one packed word per lane: ring head (bits 0-3), ring
-// fill (4-8), overflow fill (9-15), overflow error (16).
+// fill (4-8), overflow fill (9-15), overflow error (16).  The ring pointer
+// carries the LDS address space, so ring accesses stay ds_read/ds_write and
+// are never merged with the overflow path into flat accesses.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 struct LaneStack {
-  uint32_t* lds;       // this block's ring base
+  lds_u32* lds;        // this block's ring base
   uint32_t* ovf;       // overflow areas of the grid (kOverflow words per thread)
   uint32_t w;
   __device__ void init(uint32_t* lds_base, uint32_t* ovf_base) {
-    lds = lds_base;
+    lds = (lds_u32*)lds_base;
     ovf = ovf_base;
     w = 0;
   }
   __device__ void reset() { w &= 1u << 16; }
   __device__ bool overflowed() const { return (w >> 16) & 1u; }
+  __device__ uint32_t count() const { return (w >> 4) & 31u; }
   __device__ uint32_t* lane_ovf() const {  // only on the rare spill path
     return ovf + (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * kOverflow;
   }
-  __device__ uint32_t& slot(uint32_t s) const {
-    return lds[(((threadIdx.x >> 6) * kRing + s) << 6) + (threadIdx.x & 63)];
+  __device__ lds_u32& slot(uint32_t s) const {
+    return lds[(((threadIdx.x >> 6) * kRing + (s & (kRing - 1))) << 6) + (threadIdx.x & 63)];
   }
   __device__ void push(uint32_t x) {
     uint32_t head = w & 15u, count = (w >> 4) & 31u, spilled = (w >> 9) & 127u, err = w >> 16;
@@ -53,16 +58,32 @@ struct LaneStack {
       head = (head + 1) & (kRing - 1);
       --count;
     }
-    slot((head + count) & (kRing - 1)) = x;
+    slot(head + count) = x;
     ++count;
     w = head | (count << 4) | (spilled << 9) | (err << 16);
+  }
+  // Pushes the first n (<= 3) of x0, x1, x2 (x0 deepest).  With room for
+  // three entries all three slots above the top are written (the unused
+  // ones hold nothing live), so the common case has no branch; a nearly
+  // full ring takes the spilling path.
+  __device__ void push3(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t n) {
+    uint32_t head = w & 15u, count = (w >> 4) & 31u;
+    if (count + 3u <= (uint32_t)kRing) {
+      slot(head + count) = x0;
+      slot(head + count + 1u) = x1;
+      slot(head + count + 2u) = x2;
+      w += n << 4;
+    } else {
+      if (n >= 1u) push(x0);
+      if (n >= 2u) push(x1);
+      if (n >= 3u) push(x2);
+    }
   }
   __device__ uint32_t pop() {
     uint32_t head = w & 15u, count = (w >> 4) & 31u, spilled = (w >> 9) & 127u;
     if (count > 0) {
-      --count;
-      w = (w & ~(31u << 4)) | (count << 4);
-      return slot((head + count) & (kRing - 1));
+      w -= 1u << 4;
+      return slot(head + count - 1u);
     }
     if (spilled > 0) {
       --spilled;
@@ -218,9 +239,10 @@ __device__ __forceinline__ void travNode(TravState& s, LaneStack& st,
     s.cur = st.pop();
     return;
   }
-  if (m > 3) st.push((uint32_t)i3);
-  if (m > 2) st.push((uint32_t)i2);
-  if (m > 1) st.push((uint32_t)i1);
+  // farther children below nearer ones: i3 (if hit) deepest, i1 on top
+  uint32_t x0 = (uint32_t)(m > 3 ? i3 : (m > 2 ? i2 : i1));
+  uint32_t x1 = (uint32_t)(m > 3 ? i2 : i1);
+  st.push3(x0, x1, (uint32_t)i1, (uint32_t)(m - 1));
   s.cur = (uint32_t)i0;
 }
 

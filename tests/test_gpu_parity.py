@@ -97,6 +97,31 @@ def test_any_hit(ctx_c1, ref_c1):
         assert np.array_equal(occ, rocc)
 
 
+def test_ploc_builder(c1, ctx_c1, ref_c1):
+    """The PLOC builder gives the same hits as the default LBVH tree and the
+    oracle: hits must not depend on the tree."""
+    import spray_amd as sa
+    rng = np.random.default_rng(11)
+    n = 100_000
+    org = rng.uniform(-0.5, 4.5, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    dirs = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    tfar = rng.uniform(0.01, 2.0, n).astype(np.float32)
+    with sa.Context(ctx_c1.cfg) as ctx:
+        ctx.set_bvh_builder(1)
+        ctx.load_scene(os.path.join(c1, "c1.spray"), c1)
+        for dom in (5, 42):
+            t, u, v, p = ctx.trace_rays(org, dirs, dom)
+            t0, u0, v0, p0 = ctx_c1.trace_rays(org, dirs, dom)
+            rt, ru, rv, rp = ref_c1.closest(dom, org, dirs)
+            assert (p == rp).mean() >= 0.9999
+            assert np.array_equal(p, p0)
+            both = (p >= 0) & (p == rp)
+            assert np.array_equal(t[both], rt[both])
+            occ = ctx.trace_rays(org, dirs, dom, mode=1, tfar=tfar)
+            assert np.array_equal(occ, ref_c1.any_hit(dom, org, dirs, tfar))
+
+
 def test_domain_list(ctx_c1, ref_c1):
     rng = np.random.default_rng(11)
     n = 100_000
