@@ -72,6 +72,22 @@ __global__ void kVertNormal(const float* __restrict__ vtx, const uint32_t* __res
   }
 }
 
+// Shading record per face: the three vertex normals and colours of the
+// face in one 48-byte record, so shading reads one record per hit instead
+// of the face and then six vertex attributes.
+__global__ void kFaceShade(const uint32_t* __restrict__ face, long long nf,
+                           const float* __restrict__ nrm, const uint32_t* __restrict__ rgb,
+                           float4* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nf;
+       i += (long long)gridDim.x * blockDim.x) {
+    for (int k = 0; k < 3; ++k) {
+      uint32_t v = face[i * 3 + k];
+      out[i * 3 + k] = make_float4(nrm[(size_t)v * 3], nrm[(size_t)v * 3 + 1],
+                                   nrm[(size_t)v * 3 + 2], __uint_as_float(rgb[v]));
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Centroid bounds and Morton codes
 __device__ __forceinline__ uint32_t orderedBits(float f) {
@@ -625,6 +641,7 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
   P.rgb.reserve(tv * 4);
   P.nrm.reserve(tv * 12);
   P.tris.reserve(tf * sizeof(LTri));
+  P.fshade.reserve(tf * 48 + 64);
   P.nodes.reserve(tn * sizeof(BNode4));
   for (int d = 0; d < nd; ++d) {
     const DomainDesc& D = P.desc[d];
@@ -685,6 +702,9 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
                                                               adj.as<uint32_t>());
     kVertNormal<<<gridFor(nv, 256), 256, 0, s>>>(vtx, face, nv, off, cnt, adj.as<uint32_t>(),
                                                   P.nrm.as<float>() + D.vtx_off * 3);
+    kFaceShade<<<gridFor(nf, 256), 256, 0, s>>>(face, nf, P.nrm.as<float>() + D.vtx_off * 3,
+                                                 P.rgb.as<uint32_t>() + D.vtx_off,
+                                                 P.fshade.as<float4>() + D.face_off * 3);
     // Morton codes
     uint32_t* bmm = scal.as<uint32_t>();
     uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};

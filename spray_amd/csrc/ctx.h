@@ -67,6 +67,7 @@ struct DomainPool {
   DevBuf vtx, face, rgb, nrm;    // float3 / uint3 / uint32 / float3
   DevBuf nodes;                  // BNode4
   DevBuf tris;                   // LTri (leaf order)
+  DevBuf fshade;                 // float4[3] per face: vertex normal xyz + vertex rgb bits
   std::vector<int> bvh_depth;
   double build_ms = 0;
   void release() {
@@ -77,6 +78,7 @@ struct DomainPool {
     nrm.release();
     nodes.release();
     tris.release();
+    fshade.release();
     desc.clear();
     ndomains = 0;
   }

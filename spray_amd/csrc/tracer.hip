@@ -33,11 +33,37 @@ namespace spray {
 constexpr int kBlock = 256;
 // grid of kCount and kScatter (they must match: blocks own queue slots)
 constexpr int kQueueGrid = 1024;
+constexpr int kShadeRec = 5;  // float4s per shade record
+constexpr int kShadeBlock = 512;  // threads per shading block (one pool atomic per block)
+// The trace kernel's waves reserve queue entries kRange at a time (one
+// atomic per range, not per refill); the ranges still follow queue order,
+// so the waves of a launch work through the domains together.
+constexpr uint32_t kRange = 256;
+// Shade slots are taken kShadeChunk at a time per wave.
+constexpr uint32_t kShadeChunk = 64;
 static_assert(kQueueGrid % kBlock == 0, "kBlockBases splits the grid evenly over its threads");
 
 __device__ __forceinline__ void flushCount(unsigned long long* dst, unsigned long long v) {
   v = waveSumULL(v);
   if (__lane_id() == 0 && v) atomicAdd(dst, v);
+}
+
+// Block-wide flush of N wave-uniform counts: one atomic per counter and
+// block instead of per wave (waves that end together would otherwise queue
+// on the same device-scope atomics).  Every thread of the block calls it.
+template <int N>
+__device__ __forceinline__ void blockFlush(unsigned long long* const (&dst)[N],
+                                           const unsigned long long (&v)[N]) {
+  __shared__ unsigned long long part[16][N];
+  const int w = threadIdx.x >> 6;
+  if (__lane_id() == 0)
+    for (int k = 0; k < N; ++k) part[w][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    unsigned long long sum = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) sum += part[i][threadIdx.x];
+    if (sum) atomicAdd(dst[threadIdx.x], sum);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -153,9 +179,9 @@ __global__ void kClearQ(TraceParams P) {
     P.cnt[i] = 0;
     P.cnt[nd + i] = 0;
     P.score[i] = 0;
-    P.wctr[i] = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x < 3) P.wctr[nd + threadIdx.x] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd + 3; i += gridDim.x * blockDim.x)
+    P.wctr[i] = 0;
 }
 
 // Ray of the current pass: index into the radiance range, then the shadow range.
@@ -459,10 +485,42 @@ __device__ __forceinline__ void childHistory(const float ph[4], int depth, float
   for (int k = 0; k < kHistory; ++k) h[k] = (k < depth) ? ph[k] : (k == depth ? t : kInfF);
 }
 
-__device__ __forceinline__ void spawnShadow(const TraceParams& P, bool pred, f3 pos, int pixid,
-                                            f3 wi, int samid, f3 Lr, int depth, int dom,
-                                            const float hist[4], float t, float tmax) {
-  uint32_t idx = waveAppend(pred, &P.ctr->s_count, P.s_cap, &P.ctr->err);
+// Appends to a ray pool with one device-scope atomic per block and call
+// (same-address atomics from every wave queue up at the device level).
+// Every thread of the block makes the same sequence of calls; the two LDS
+// buffers alternate, so a call never overwrites words the previous call's
+// readers may still need.
+struct BlockAppend {
+  uint32_t* sm;  // 2 x 17 words of LDS
+  uint32_t flip;
+  __device__ uint32_t operator()(bool pred, uint32_t* counter, uint32_t cap, uint32_t* err) {
+    uint32_t* b = sm + 17 * flip;
+    flip ^= 1u;
+    const unsigned long long m = __ballot(pred);
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    if (lane == 0) b[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t acc = 0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        const uint32_t x = b[i];
+        b[i] = acc;
+        acc += x;
+      }
+      const uint32_t base = acc ? atomicAdd(counter, acc) : 0u;
+      b[16] = base;
+      if (acc && base + acc > cap) atomicOr(err, 1u);
+    }
+    __syncthreads();
+    const uint32_t idx = b[16] + b[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    return (pred && idx < cap) ? idx : 0xffffffffu;
+  }
+};
+
+__device__ __forceinline__ void spawnShadow(const TraceParams& P, BlockAppend& app, bool pred,
+                                            f3 pos, int pixid, f3 wi, int samid, f3 Lr, int depth,
+                                            int dom, const float hist[4], float t, float tmax) {
+  uint32_t idx = app(pred, &P.ctr->s_count, P.s_cap, &P.ctr->err);
   if (idx == 0xffffffffu) return;
   float h[4];
   childHistory(hist, depth, t, h);
@@ -471,13 +529,13 @@ __device__ __forceinline__ void spawnShadow(const TraceParams& P, bool pred, f3 
   P.Sx.occluded[idx] = 0u;
 }
 
-__device__ __forceinline__ void spawnBounce(const TraceParams& P, bool pred, bool pending, f3 pos,
-                                            int pixid, f3 wi, int samid, f3 Lr, int depth,
-                                            const float hist[4], float t) {
+__device__ __forceinline__ void spawnBounce(const TraceParams& P, BlockAppend& app, bool pred,
+                                            bool pending, f3 pos, int pixid, f3 wi, int samid,
+                                            f3 Lr, int depth, const float hist[4], float t) {
   // rays whose next virtual depth is 4 go to the pending pool (genR2,
   // ooc_shader_pt.h:73-90) with their parent's full history
-  uint32_t ir = waveAppend(pred && !pending, &P.ctr->r_count, P.r_cap, &P.ctr->err);
-  uint32_t ip = waveAppend(pred && pending, &P.ctr->p_count, P.p_cap, &P.ctr->err);
+  uint32_t ir = app(pred && !pending, &P.ctr->r_count, P.r_cap, &P.ctr->err);
+  uint32_t ip = app(pred && pending, &P.ctr->p_count, P.p_cap, &P.ctr->err);
   float h[4];
   childHistory(hist, depth, t, h);
   if (ir != 0xffffffffu) writeRay(P.R, ir, pos, pixid, wi, samid, Lr, depth + 1, h);
@@ -527,7 +585,8 @@ __device__ __forceinline__ float frDielectric(float cosI, float etaI, float etaT
 
 // All lanes of a wave call this together (the spawn appends are ballots);
 // `act` marks lanes whose hit was accepted by the history buffer.
-__device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int dom, float4 op,
+__device__ __forceinline__ void shadeSpawn(const TraceParams& P, BlockAppend& app, bool act,
+                                           int dom, float4 op,
                                            float4 ds, float4 wd, const float hist[4], int depth,
                                            float t, float u, float v, int prim, int ray_depth) {
   const DomainDesc& D = P.desc[dom];
@@ -537,9 +596,9 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
   uint32_t color = 0;
   f3 normal = mk(0, 0, 1);
   if (act) {  // render/trimesh_buffer.cc:327-360
-    const uint32_t* fc = P.face + (size_t)(D.face_off + prim) * 3;
-    uint32_t i0 = fc[0], i1 = fc[1], i2 = fc[2];
-    uint32_t c0 = P.rgb[D.vtx_off + i0], c1 = P.rgb[D.vtx_off + i1], c2 = P.rgb[D.vtx_off + i2];
+    const float4* fs = P.fshade + (size_t)(D.face_off + prim) * 3;
+    const float4 a0 = fs[0], a1 = fs[1], a2 = fs[2];
+    uint32_t c0 = __float_as_uint(a0.w), c1 = __float_as_uint(a1.w), c2 = __float_as_uint(a2.w);
     float w = 1.f - u - v;
     uint32_t r = (uint32_t)((float)((c0 >> 16) & 0xff) * w + (float)((c1 >> 16) & 0xff) * u +
                             (float)((c2 >> 16) & 0xff) * v);
@@ -547,11 +606,8 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
                             (float)((c2 >> 8) & 0xff) * v);
     uint32_t b = (uint32_t)((float)(c0 & 0xff) * w + (float)(c1 & 0xff) * u + (float)(c2 & 0xff) * v);
     color = (r << 16) | (g << 8) | b;
-    const float* n0 = P.nrm + (size_t)(D.vtx_off + i0) * 3;
-    const float* n1 = P.nrm + (size_t)(D.vtx_off + i1) * 3;
-    const float* n2 = P.nrm + (size_t)(D.vtx_off + i2) * 3;
-    normal = mk((n0[0] * w) + (n1[0] * u) + (n2[0] * v), (n0[1] * w) + (n1[1] * u) + (n2[1] * v),
-                (n0[2] * w) + (n1[2] * u) + (n2[2] * v));
+    normal = mk((a0.x * w) + (a1.x * u) + (a2.x * v), (a0.y * w) + (a1.y * u) + (a2.y * v),
+                (a0.z * w) + (a1.z * u) + (a2.z * v));
   }
   f3 surf = unpackColor(color);
   f3 wo = neg(dir);
@@ -573,7 +629,7 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
       cosineSample(u1, u2, nff, &wi, &pdf);
       costheta = clamp01(dot(nff, wi));
       f3 Lr = scl(mul(Lin, surf), ((kOneOverPi * costheta) * ao_weight) / pdf);
-      spawnShadow(P, act && hasPositive(Lr), pos, pixid, wi, samid, Lr, depth, dom, hist, t,
+      spawnShadow(P, app, act && hasPositive(Lr), pos, pixid, wi, samid, Lr, depth, dom, hist, t,
                   P.ao_max_dist);
     }
   } else {
@@ -599,13 +655,18 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
           float sc = (L.type == kPointLight) ? (1.0f / pdf) : (1.0f / (pdf * (float)nsamp));
           Lr = scl(mul(Lin, bp), sc);
         }
-        spawnShadow(P, ok && hasPositive(Lr), pos, pixid, wi, samid, Lr, depth, dom, hist, t, tmax);
+        spawnShadow(P, app, ok && hasPositive(Lr), pos, pixid, wi, samid, Lr, depth, dom, hist, t,
+                    tmax);
       }
     }
   }
   // bounce (ooc_shader_pt.h:177-226)
   bool bounce = act && next_actual < P.bounces;
   wo = normalize(wo);
+  // up to two bounce rays (reflection, transmission), spawned after the
+  // branch so that every thread of the block appends at the same points
+  bool b1 = false, b2 = false;
+  f3 d1 = mk(0, 0, 0), d2 = mk(0, 0, 0), L1 = mk(0, 0, 0), L2 = mk(0, 0, 0);
   if (delta) {
     bool refl = false, trans = false;
     f3 wr = mk(0, 0, 0), wt = mk(0, 0, 0), Lr1 = mk(0, 0, 0), Lr2 = mk(0, 0, 0);
@@ -647,8 +708,12 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
         Lr2 = scl(Lin, (1.0f - fr) / ac);
       }
     }
-    spawnBounce(P, refl && hasPositive(Lr1), pending, pos, pixid, wr, samid, Lr1, depth, hist, t);
-    spawnBounce(P, trans && hasPositive(Lr2), pending, pos, pixid, wt, samid, Lr2, depth, hist, t);
+    b1 = refl && hasPositive(Lr1);
+    d1 = wr;
+    L1 = Lr1;
+    b2 = trans && hasPositive(Lr2);
+    d2 = wt;
+    L2 = Lr2;
   } else {
     f3 Lr = mk(0, 0, 0);
     bool go = false;
@@ -669,8 +734,12 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
         if (go) Lr = dvs(Lr, p);
       }
     }
-    spawnBounce(P, go, pending, pos, pixid, wi, samid, Lr, depth, hist, t);
+    b1 = go;
+    d1 = wi;
+    L1 = Lr;
   }
+  spawnBounce(P, app, b1, pending, pos, pixid, d1, samid, L1, depth, hist, t);
+  spawnBounce(P, app, b2, pending, pos, pixid, d2, samid, L2, depth, hist, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -685,9 +754,13 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, bool act, int d
 // traversals do not hold up the rest of the wave.
 constexpr int kTravBlock = 256;
 
-// Adds a wave-uniform count once per wave.
-__device__ __forceinline__ void flushWave(unsigned long long* dst, unsigned long long v) {
-  if (__lane_id() == 0 && v) atomicAdd(dst, v);
+
+// Marks shade slots [cur, end) (fewer than 64) as holding no record.
+__device__ __forceinline__ void shadeSlotsEmpty(const TraceParams& P, uint32_t cur, uint32_t end,
+                                                int lane) {
+  const uint32_t i = cur + (uint32_t)lane;
+  if (i < end && i < P.sh_cap)
+    P.sh_rec[(size_t)i * kShadeRec + 4] = make_float4(kInfF, 0.f, 0.f, __int_as_float(-1));
 }
 
 __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain_dom, int phase) {
@@ -744,19 +817,32 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   bool active = false, exhausted = false;
   int kind = 0, dom = 0;
   uint32_t ray = 0;
+  // wave-uniform: entries left in this wave's range; shade slots left in its chunk
+  uint32_t r_cur = 0, r_end = 0, sh_cur = 0, sh_end = 0;
+  // small launches keep ranges at one refill's worth so every wave gets work
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t range = (unsigned long long)total >= (unsigned long long)nwaves * 1024u ? kRange : 64u;
+  // one launch per domain (plain) takes exactly the slots it fills
+  const uint32_t sh_chunk = plain_dom >= 0 ? 0u : kShadeChunk;
   while (true) {
     unsigned long long idle = __ballot(!active);
     int nidle = __popcll(idle);
     if (!exhausted && (nidle >= P.refill || nidle == 64)) {
-      int leader = __builtin_ctzll(idle);
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(wc, (uint32_t)nidle);
-      base = __shfl(base, leader, 64);
-      if (base + (uint32_t)nidle >= total) exhausted = true;
+      if (r_cur >= r_end) {
+        const int leader = __builtin_ctzll(idle);
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(wc, range);
+        b = __builtin_amdgcn_readlane(b, leader);
+        r_cur = min(b, total);
+        r_end = min(b + range, total);
+        if (r_cur >= total) exhausted = true;
+      }
+      const uint32_t base = r_cur, limit = r_end;
+      r_cur = min(r_cur + (uint32_t)nidle, r_end);
       bool go = false, fetched = false;
       if (!active) {
         uint32_t e = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-        if (e < total) {
+        if (e < limit) {
           fetched = true;
           kind = e < nr ? 0 : 1;
           QEntry qe;
@@ -805,15 +891,18 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     }
     travNodePhase(s, st, tc, active, P.nodes);
     travLeafPhase(s, st, tc, active, P.tris);
+    bool upd = false;
+    float4 wd = make_float4(0.f, 0.f, 0.f, 0.f), hh = wd;
     if (active && travDone(s)) {
       active = false;
       if (kind == 0) {
         // the history prefix the buffer update checks is fixed for this pass
-        const float4 wd = P.R.wd[ray], hh = P.R.h[ray];
+        wd = P.R.wd[ray];
+        hh = P.R.h[ray];
         const int depth = __float_as_int(wd.w) & 0xff;
         const int samid = __float_as_int(P.R.ds[ray].w);
         float hist[4] = {hh.x, hh.y, hh.z, hh.w};
-        bool upd = s.prim >= 0 && vbufUpdate(P.vbuf, samid, hist, depth, s.tlim);
+        upd = s.prim >= 0 && vbufUpdate(P.vbuf, samid, hist, depth, s.tlim);
         if (P.log_on) {  // visit log for the predictor: (ray, domain, t)
           uint32_t i = atomicAdd(&c->vlog_count, 1u);
           if (i < P.vlog_cap) {
@@ -823,69 +912,89 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
           }
         }
         if (upd) atomicMin((uint32_t*)&P.R.h[ray] + depth, __float_as_uint(s.tlim));
-        uint32_t idx = waveAppend(upd, &c->shade_count, P.sh_cap, &c->err);
-        if (idx != 0xffffffffu) {
-          P.sh_ray[idx] = ray;
-          P.sh_dom[idx] = (uint32_t)dom;
-          P.sh_tuv[(size_t)idx * 3] = s.tlim;
-          P.sh_tuv[(size_t)idx * 3 + 1] = s.u;
-          P.sh_tuv[(size_t)idx * 3 + 2] = s.v;
-          P.sh_prim[idx] = s.prim;
-        }
       } else if (s.prim > 0) {
         __hip_atomic_store(&P.Sx.occluded[ray], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+    // shade records: slots come from the wave's chunk (one atomic per
+    // kShadeChunk slots); the rest of a chunk that cannot hold this round's
+    // records is marked empty
+    const unsigned long long sm = __ballot(upd);
+    if (sm != 0ull) {
+      const uint32_t n = (uint32_t)__popcll(sm);
+      if (sh_cur + n > sh_end) {
+        shadeSlotsEmpty(P, sh_cur, sh_end, lane);
+        const int leader = __builtin_ctzll(sm);
+        uint32_t b = 0;
+        const uint32_t take = max(n, sh_chunk);
+        if (lane == leader) b = atomicAdd(&c->shade_count, take);
+        sh_cur = __builtin_amdgcn_readlane(b, leader);
+        sh_end = sh_cur + take;
+        if (sh_end > P.sh_cap && lane == leader) atomicOr(&c->err, 1u);
+      }
+      const uint32_t idx = sh_cur + (uint32_t)__popcll(sm & ((1ull << lane) - 1ull));
+      sh_cur += n;
+      if (upd && idx < P.sh_cap) {
+        // the whole ray goes into the shade record, written in order, so
+        // shading reads no ray record at random; the history prefix
+        // [0, depth) is fixed for the pass and entry `depth` is replaced
+        // by the hit's t in the children
+        float4* rec = P.sh_rec + (size_t)idx * kShadeRec;
+        rec[0] = P.R.op[ray];
+        rec[1] = P.R.ds[ray];
+        // radiance rays keep only their depth in wd.w: the domain joins it
+        rec[2] = make_float4(wd.x, wd.y, wd.z, __int_as_float(__float_as_int(wd.w) | (dom << 8)));
+        rec[3] = hh;
+        rec[4] = make_float4(s.tlim, s.u, s.v, __int_as_float(s.prim));
+      }
+    }
   }
-  flushWave(&c->radiance_visits, rvis);
-  flushWave(&c->shadow_visits, svis);
-  flushWave(&c->pruned, pruned);
-  flushWave(&c->node_visits, tc.nodes);
-  flushCount(&c->tri_tests, tc.tris);
+  shadeSlotsEmpty(P, sh_cur, sh_end, lane);
+  blockFlush<5>({&c->radiance_visits, &c->shadow_visits, &c->pruned, &c->node_visits,
+                 &c->tri_tests},
+                {rvis, svis, pruned, tc.nodes, waveSumULL(tc.tris)});
   if (st.overflowed()) atomicOr(&c->err, 4u);
 }
 
 // Shading after the trace step: every hit whose history entry is still the
 // buffer's final entry for this pass is shaded once (stale speculative hits
 // would only spawn rays that can never retire, so they are skipped).
-__global__ void __launch_bounds__(kBlock, 4) kShade(TraceParams P) {
+__global__ void __launch_bounds__(kShadeBlock, 4) kShade(TraceParams P) {
+  __shared__ uint32_t app_sm[2 * 17];
   DevCounters* c = P.ctr;
   if (!c->active) return;
+  BlockAppend app{app_sm, 0u};
   const uint32_t n = min(c->shade_count, P.sh_cap);
   const int ray_depth = P.tile_rec[2];
   unsigned long long shades = 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
-    uint32_t ray = valid ? P.sh_ray[i] : 0u;
-    int dom = valid ? (int)P.sh_dom[i] : 0;
-    float t = valid ? P.sh_tuv[(size_t)i * 3] : kInfF;
-    float u = valid ? P.sh_tuv[(size_t)i * 3 + 1] : 0.f;
-    float v = valid ? P.sh_tuv[(size_t)i * 3 + 2] : 0.f;
-    int prim = valid ? P.sh_prim[i] : -1;
     float4 op = make_float4(0, 0, 0, 0), ds = make_float4(1, 0, 0, 0), wd = make_float4(0, 0, 0, 0);
-    float hist[4] = {kInfF, kInfF, kInfF, kInfF};
-    int depth = 0;
-    bool act = false;
+    float4 hh = make_float4(kInfF, kInfF, kInfF, kInfF), tuv = make_float4(kInfF, 0, 0, 0);
+    int dom = 0;
     if (valid) {
-      op = P.R.op[ray];
-      ds = P.R.ds[ray];
-      wd = P.R.wd[ray];
-      float4 hh = P.R.h[ray];
-      hist[0] = hh.x;
-      hist[1] = hh.y;
-      hist[2] = hh.z;
-      hist[3] = hh.w;
-      depth = __float_as_int(wd.w) & 0xff;
-      int samid = __float_as_int(ds.w);
-      uint32_t parent = depth > 0 ? __float_as_uint(pick4(hist, depth - 1)) : 0u;
-      act = vbufCorrect(P.vbuf, samid, hist, depth) &&
-            ldW(P.vbuf + (size_t)samid * kHistory + depth) == mkW(t, parent);
+      const float4* rec = P.sh_rec + (size_t)i * kShadeRec;
+      tuv = rec[4];
+      valid = __float_as_int(tuv.w) >= 0;  // slots left empty by the trace kernel
     }
+    if (valid) {
+      const float4* rec = P.sh_rec + (size_t)i * kShadeRec;
+      op = rec[0];
+      ds = rec[1];
+      wd = rec[2];
+      hh = rec[3];
+      dom = __float_as_int(wd.w) >> 8;
+    }
+    float hist[4] = {hh.x, hh.y, hh.z, hh.w};
+    const int depth = __float_as_int(wd.w) & 0xff;
+    const float t = tuv.x, u = tuv.y, v = tuv.z;
+    const int prim = __float_as_int(tuv.w);
+    const bool act = valid && vbufIsFinal(P.vbuf, __float_as_int(ds.w), hist, depth, t);
     shades += act ? 1 : 0;
-    shadeSpawn(P, act, dom, op, ds, wd, hist, depth, t, u, v, prim, ray_depth);
+    shadeSpawn(P, app, act, dom, op, ds, wd, hist, depth, t, u, v, prim, ray_depth);
   }
-  flushCount(&c->shades, shades);
+  blockFlush<1>({&c->shades}, {waveSumULL(shades)});
 }
 
 // Retire (ooc_tcontext.inl:123-135): shadow rays tested in this pass.
@@ -909,7 +1018,7 @@ __global__ void kRetire(TraceParams P) {
     atomicAdd(px + 1, (float)(P.inv_spp * (double)wd.y));
     atomicAdd(px + 2, (float)(P.inv_spp * (double)wd.z));
   }
-  flushCount(&c->useful, useful);
+  blockFlush<1>({&c->useful}, {waveSumULL(useful)});
 }
 
 // Window end: census of non-stale spawned radiance rays and final hits.
@@ -984,7 +1093,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
-      wctr, score, order, q[2], qdom[2], rcand, dbox, ovf, vlog, log, sh_ray, sh_dom, sh_tuv, sh_prim, qa,
+      wctr, score, order, q[2], qdom[2], rcand, dbox, ovf, vlog, log, sh_rec, qa,
       qb;
   unsigned long long log_cap = 0;
   bool logging = false;
@@ -1037,8 +1146,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_ray, &T->sh_dom,
-                    &T->sh_tuv, &T->sh_prim, &T->log_ctr, &T->qa, &T->qb})
+                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1138,10 +1246,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->wctr, (size_t)(nd + 3) * 4 + 64);
   need(T, T->qa, (size_t)T->q_cap[0] * 4);
   need(T, T->qb, (size_t)T->q_cap[0] * 4);
-  need(T, T->sh_ray, (size_t)T->q_cap[0] * 4);
-  need(T, T->sh_dom, (size_t)T->q_cap[0] * 4);
-  need(T, T->sh_tuv, (size_t)T->q_cap[0] * 12);
-  need(T, T->sh_prim, (size_t)T->q_cap[0] * 4);
+  need(T, T->sh_rec, (size_t)T->q_cap[0] * kShadeRec * 16);
   need(T, T->score, (size_t)nd * 8 + 64);
   need(T, T->order, (size_t)nd * 4 + 64);
   need(T, T->q[0], (size_t)T->q_cap[0] * sizeof(QEntry));
@@ -1208,6 +1313,7 @@ static TraceParams makeParams(Ctx* c) {
   P.face = pool.face.as<uint32_t>();
   P.rgb = pool.rgb.as<uint32_t>();
   P.nrm = pool.nrm.as<float>();
+  P.fshade = pool.fshade.as<float4>();
   auto pool4 = [](DevBuf* b) {
     RayPool r;
     r.op = b[0].as<float4>();
@@ -1245,10 +1351,7 @@ static TraceParams makeParams(Ctx* c) {
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
   P.refill = std::min(64, std::max(1, c->batch_chunk));
-  P.sh_ray = T->sh_ray.as<uint32_t>();
-  P.sh_dom = T->sh_dom.as<uint32_t>();
-  P.sh_tuv = T->sh_tuv.as<float>();
-  P.sh_prim = T->sh_prim.as<int>();
+  P.sh_rec = T->sh_rec.as<float4>();
   P.sh_cap = T->q_cap[0];
   P.qa = T->qa.as<uint32_t>();
   P.qb = T->qb.as<uint32_t>();
@@ -1284,7 +1387,7 @@ static void passQueues(Ctx* c, const TraceParams& P) {
 static void passFinish(Ctx* c, const TraceParams& P) {
   TracerState* T = c->tracer;
   hipStream_t s = c->stream;
-  kShade<<<1024, kBlock, 0, s>>>(P);
+  kShade<<<1024, kShadeBlock, 0, s>>>(P);
   // new shadow rays: any hit in the spawning domain first (procShads2)
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();

@@ -63,6 +63,7 @@ struct TraceParams {
   const uint32_t* face;
   const uint32_t* rgb;
   const float* nrm;
+  const float4* fshade;  // per face: 3 x (vertex normal, vertex rgb bits)
   // tracer state
   RayPool R, Pn;  // radiance (window) and pending pools
   RayPool S;
@@ -88,10 +89,10 @@ struct TraceParams {
   int refill;         // a wave refetches work once this many lanes are idle
   uint32_t* ovf;      // traversal stack overflow areas
   // shade records of the pass (accepted hits), SoA
-  uint32_t* sh_ray;
-  uint32_t* sh_dom;
-  float* sh_tuv;      // t, u, v
-  int* sh_prim;
+  // shade records, kShadeRec float4 each: the ray's op, ds, wd (w: depth |
+  // domain << 8) and history, then (t, u, v, prim); prim < 0 marks an
+  // empty slot
+  float4* sh_rec;
   uint32_t sh_cap;
   // learned form
   uint32_t* qa;       // learned form: radiance queue indices of phase A
@@ -135,13 +136,33 @@ __device__ __forceinline__ unsigned long long ldW(const unsigned long long* p) {
 __device__ __forceinline__ bool vbufCorrect(const unsigned long long* vb, int samid,
                                             const float h[4], int n) {
   const unsigned long long* w = vb + (size_t)samid * kHistory;
+  // all words are read up front: independent loads, not a chain of
+  // loads each waiting on the previous comparison
+  unsigned long long x[kHistory];
+#pragma unroll
+  for (int i = 0; i < kHistory; ++i) x[i] = ldW(&w[i]);
   bool ok = true;
 #pragma unroll
-  for (int i = 0; i < kHistory; ++i) {  // constant indices keep h[] in registers
-    if (ok && i < n) {
-      unsigned long long x = ldW(&w[i]);
-      ok = wT(x) == h[i] && (i == 0 || wP(x) == __float_as_uint(h[i > 0 ? i - 1 : 0]));
-    }
+  for (int i = 0; i < kHistory; ++i)  // constant indices keep h[] in registers
+    if (i < n)
+      ok = ok && wT(x[i]) == h[i] && (i == 0 || wP(x[i]) == __float_as_uint(h[i > 0 ? i - 1 : 0]));
+  return ok;
+}
+
+// Shading test: the ray's history prefix is the buffer's chain and the
+// buffer's entry at `depth` is this hit (t with the parent's t).
+__device__ __forceinline__ bool vbufIsFinal(const unsigned long long* vb, int samid,
+                                            const float h[4], int depth, float t) {
+  const unsigned long long* w = vb + (size_t)samid * kHistory;
+  unsigned long long x[kHistory];
+#pragma unroll
+  for (int i = 0; i < kHistory; ++i) x[i] = ldW(&w[i]);
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < kHistory; ++i) {
+    const uint32_t parent = i > 0 ? __float_as_uint(h[i > 0 ? i - 1 : 0]) : 0u;
+    if (i < depth) ok = ok && wT(x[i]) == h[i] && (i == 0 || wP(x[i]) == parent);
+    if (i == depth) ok = ok && x[i] == mkW(t, parent);
   }
   return ok;
 }
