@@ -205,23 +205,35 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
 
 // One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
 // entries sorted (registers) and the number of domains hit.
+// The domains hit are also recorded as a bit mask (one word per 64 domains)
+// so that the scatter re-tests only those.
 __device__ __forceinline__ int rayDomains(const float4* box, int nd, const float o[3],
                                           const float inv[3], bool safe, int spawn, uint32_t* lc_kind,
-                                          float tt[kTopDepths], int ti[kTopDepths], int* kept_o) {
+                                          float tt[kTopDepths], int ti[kTopDepths], int* kept_o,
+                                          unsigned long long* mask_out) {
   int kept = 0, hits = 0;
+  unsigned long long mask = 0ull;
   for (int dd = 0; dd < nd; ++dd) {
     float tmin;
     // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
     // spawning domain was tested right after shading
-    if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
+    const bool hit = slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) && dd != spawn;
+    mask |= (hit ? 1ull : 0ull) << (dd & 63);
+    if ((dd & 63) == 63 || dd == nd - 1) {  // dd is wave-uniform
+      mask_out[dd >> 6] = mask;
+      mask = 0ull;
+    }
+    if (!hit) continue;
     ++hits;
     atomicAdd(&lc_kind[dd], 1u);
-    // insertion into the (t, id)-sorted top list (unrolled: stays in registers)
+    // insertion into the (t, id)-sorted top list (unrolled: stays in
+    // registers); steps past every active lane's list end are skipped
     if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
       float ct = tmin;
       int ci = dd;
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k) {
+        if (__ballot(k <= kept) == 0ull) break;
         if (k < kept) {
           if (ct < tt[k] || (ct == tt[k] && ci < ti[k])) {
             float xt = tt[k];
@@ -291,7 +303,8 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     // NaN-checked slab tests only in waves with an axis-parallel ray
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
     if (valid) {
-      hits = rayDomains(box, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti, &kept);
+      hits = rayDomains(box, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti, &kept,
+                        P.dmask + (size_t)i * ((nd + 63) >> 6));
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
         if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
@@ -377,19 +390,25 @@ __global__ void __launch_bounds__(1024) kSchedule(TraceParams P) {
 }
 
 // The queue entries of one ray, at its block's base plus the block-local
-// slot of each domain.
-template <bool kSafe>
+// slot of each domain; only the domains kCount marked are tested again
+// (for their entry distance).
 __device__ __forceinline__ void scatterDomains(const float4* box, int nd, const float o[3],
-                                               const float inv[3], int spawn, uint32_t* lc_kind,
+                                               const float inv[3], bool safe,
+                                               const unsigned long long* mask, uint32_t* lc_kind,
                                                const uint32_t* lb_kind, const TraceParams& P,
                                                int kind, uint32_t r) {
-  for (int dd = 0; dd < nd; ++dd) {
-    float tmin;
-    if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, kSafe, &tmin) || dd == spawn) continue;
-    const uint32_t dst = lb_kind[dd] + atomicAdd(&lc_kind[dd], 1u);
-    if (dst < P.q_cap[kind]) {
-      P.q[kind][dst] = QEntry{r, tmin};
-      P.qdom[kind][dst] = (uint16_t)dd;
+  for (int w = 0; w < ((nd + 63) >> 6); ++w) {
+    unsigned long long m = mask[w];
+    while (m) {
+      const int dd = (w << 6) + __builtin_ctzll(m);
+      m &= m - 1ull;
+      float tmin;
+      slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin);
+      const uint32_t dst = lb_kind[dd] + atomicAdd(&lc_kind[dd], 1u);
+      if (dst < P.q_cap[kind]) {
+        P.q[kind][dst] = QEntry{r, tmin};
+        P.qdom[kind][dst] = (uint16_t)dd;
+      }
     }
   }
 }
@@ -443,7 +462,6 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
     uint32_t r = 0;
     bool valid = passRay(P, i, &kind, &r);
     float o[3] = {0, 0, 0}, inv[3] = {1, 1, 1};
-    int spawn = -1;
     if (valid) {
       const RayPool& pool = kind ? P.S : P.R;
       float4 op = pool.op[r], ds = pool.ds[r];
@@ -453,18 +471,13 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
       inv[0] = 1.0f / ds.x;
       inv[1] = 1.0f / ds.y;
       inv[2] = 1.0f / ds.z;
-      if (kind) {
-        spawn = __float_as_int(pool.wd[r].w) >> 8;
-        if (P.Sx.occluded[r]) valid = false;  // occluded in its spawning domain
-      }
+      if (kind && P.Sx.occluded[r]) valid = false;  // occluded in its spawning domain
     }
+    // the same wave-uniform slab variant as kCount (same rays per wave)
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
-    if (valid) {
-      if (safe)
-        scatterDomains<true>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
-      else
-        scatterDomains<false>(box, nd, o, inv, spawn, lc + kind * nd, lb + kind * nd, P, kind, r);
-    }
+    if (valid)
+      scatterDomains(box, nd, o, inv, safe, P.dmask + (size_t)i * ((nd + 63) >> 6),
+                     lc + kind * nd, lb + kind * nd, P, kind, r);
   }
 }
 
@@ -1093,7 +1106,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
-      wctr, score, order, q[2], qdom[2], rcand, dbox, ovf, vlog, log, sh_rec, qa,
+      wctr, score, order, q[2], qdom[2], rcand, dmask, dbox, ovf, vlog, log, sh_rec, qa,
       qb;
   unsigned long long log_cap = 0;
   bool logging = false;
@@ -1146,7 +1159,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb})
+                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1235,6 +1248,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->s_occ, (size_t)T->s_cap * 4);
   need(T, T->vbuf, (size_t)S * kHistory * 8);
   need(T, T->rcand, (size_t)T->r_cap + 64);
+  need(T, T->dmask, ((size_t)T->r_cap + T->s_cap) * ((nd + 63) / 64) * 8 + 64);
   need(T, T->dbox, (size_t)nd * 32 + 64);
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
@@ -1348,6 +1362,7 @@ static TraceParams makeParams(Ctx* c) {
   P.qdom[0] = T->qdom[0].as<uint16_t>();
   P.qdom[1] = T->qdom[1].as<uint16_t>();
   P.rcand = T->rcand.as<uint8_t>();
+  P.dmask = T->dmask.as<unsigned long long>();
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
   P.refill = std::min(64, std::max(1, c->batch_chunk));
