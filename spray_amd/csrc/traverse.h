@@ -246,36 +246,29 @@ __device__ __forceinline__ void travNode(TravState& s, LaneStack& st,
   s.cur = (uint32_t)i0;
 }
 
-// Tests the triangles of one leaf; returns true when an any-hit query is
-// answered.  Returns the number of triangles tested in *ntested.
-__device__ __forceinline__ bool travLeaf(TravState& s, uint32_t leaf, const LTri* __restrict__ tris,
-                                         int* ntested) {
-  const uint32_t first = s.tri_base + (leaf & 0x00ffffffu);
-  const int n = (int)((leaf >> 24) & 0x7fu);
-  *ntested = n;
+// Tests one triangle (index relative to the domain's leaf-ordered array);
+// returns true when an any-hit query is answered.
+__device__ __forceinline__ bool travTri(TravState& s, uint32_t tri, const LTri* __restrict__ tris) {
   const RayPre rp{(int)(s.kk & 3u), (int)((s.kk >> 2) & 3u), (int)((s.kk >> 4) & 3u), s.Sx, s.Sy,
                   s.Sz};
   const bool any = (s.kk >> 8) & 1u;
-  for (int k = 0; k < n; ++k) {
-    const float4* tq = (const float4*)&tris[first + k];
-    float4 a = tq[0], b = tq[1], c = tq[2];
-    float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
-    // closest: accept t <= tlim once a hit exists (ties go to the lower id)
-    float ub = (any || s.prim < 0) ? s.tlim : __uint_as_float(__float_as_uint(s.tlim) + 1u);
-    float t, u, v;
-    if (triHit(rp, s.org, v0, v1, v2, s.tnear, ub, &t, &u, &v)) {
-      if (any) {
-        s.prim = 1;
-        *ntested = k + 1;
-        return true;
-      }
-      int p = __float_as_int(a.w);
-      if (s.prim < 0 || t < s.tlim || p < s.prim) {
-        s.tlim = t;
-        s.u = u;
-        s.v = v;
-        s.prim = p;
-      }
+  const float4* tq = (const float4*)&tris[s.tri_base + tri];
+  float4 a = tq[0], b = tq[1], c = tq[2];
+  float v0[3] = {a.x, a.y, a.z}, v1[3] = {b.x, b.y, b.z}, v2[3] = {c.x, c.y, c.z};
+  // closest: accept t <= tlim once a hit exists (ties go to the lower id)
+  float ub = (any || s.prim < 0) ? s.tlim : __uint_as_float(__float_as_uint(s.tlim) + 1u);
+  float t, u, v;
+  if (triHit(rp, s.org, v0, v1, v2, s.tnear, ub, &t, &u, &v)) {
+    if (any) {
+      s.prim = 1;
+      return true;
+    }
+    int p = __float_as_int(a.w);
+    if (s.prim < 0 || t < s.tlim || p < s.prim) {
+      s.tlim = t;
+      s.u = u;
+      s.v = v;
+      s.prim = p;
     }
   }
   return false;
@@ -303,26 +296,37 @@ __device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravC
   }
 }
 
-// Leaf phase over the wave: postponed leaf first, then consecutive leaves
-// from the stack.
+// Leaf phase over the wave, one triangle per lane and step: each lane
+// walks a leaf cursor (the leaf word with its count decremented and its
+// first triangle advanced as triangles are tested), taking its postponed
+// leaf first and then consecutive leaves from the stack, so lanes with
+// short leaves move on instead of waiting for the longest leaf of the wave.
 __device__ __forceinline__ void travLeafPhase(TravState& s, LaneStack& st, TravCount& cnt,
                                               bool active, const LTri* __restrict__ tris) {
+  uint32_t lf = 0;  // leaf cursor (bit 31 | remaining << 24 | next triangle), 0 = none
   while (true) {
-    bool leaf = active && (s.pend != kStackEmpty || isLeafRef(s.cur));
-    if (__ballot(leaf) == 0ull) break;
-    int nt = 0;
-    if (leaf) {
-      uint32_t L;
+    if (active && lf == 0) {
       if (s.pend != kStackEmpty) {
-        L = s.pend;
+        lf = s.pend;
         s.pend = kStackEmpty;
-      } else {
-        L = s.cur;
+      } else if (isLeafRef(s.cur)) {
+        lf = s.cur;
         s.cur = st.pop();
       }
-      if (travLeaf(s, L, tris, &nt)) s.cur = s.pend = kStackEmpty;
+      if (((lf >> 24) & 0x7fu) == 0u) lf = 0;
     }
-    cnt.tris += (uint32_t)nt;
+    const bool tri = lf != 0;
+    if (__ballot(tri) == 0ull) break;
+    if (tri) {
+      cnt.tris += 1;
+      if (travTri(s, lf & 0x00ffffffu, tris)) {
+        s.cur = s.pend = kStackEmpty;
+        lf = 0;
+      } else {
+        lf += 1u - (1u << 24);
+        if (((lf >> 24) & 0x7fu) == 0u) lf = 0;
+      }
+    }
   }
 }
 
