@@ -275,7 +275,8 @@ __device__ __forceinline__ bool travTri(TravState& s, uint32_t tri, const LTri* 
 }
 
 // Node phase over the wave: lanes with an inner `cur` descend until every
-// busy lane holds a postponed leaf (or none can descend further).
+// busy lane holds a postponed leaf, none can descend further, or fewer than
+// a third of the wave's rays would take a step.
 __device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravCount& cnt,
                                               bool active, const BNode4* __restrict__ nodes) {
   // wave-uniform: NaN-checked box tests only when some ray is axis-parallel
@@ -292,7 +293,11 @@ __device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravC
     }
     bool more = active && isInnerRef(s.cur);
     bool hungry = active && s.pend == kStackEmpty && s.cur != kStackEmpty;
-    if (__ballot(more) == 0ull || __ballot(hungry) == 0ull) break;
+    // leave when no lane lacks a leaf, or when fewer than a third of the
+    // wave's rays would step: the rest go to the leaf phase instead of
+    // idling behind a few deep lanes (they resume in the next round)
+    const int nmore = __popcll(__ballot(more));
+    if (nmore == 0 || __ballot(hungry) == 0ull || nmore * 3 < __popcll(__ballot(active))) break;
   }
 }
 
