@@ -225,6 +225,30 @@ SPRAY_HD bool slabBox4(const float4& lo, const float4& hi, const float org[3], c
   return !out && tmin <= tmax && tmin < kInfF && tmax > kRayEps;
 }
 
+// Closed variant for culling boxes that contain other boxes: no half-open
+// rejection, NaN axes unbounded.  A box that contains B passes whenever
+// slabBox4 accepts B (the slab arithmetic is monotone in the planes).
+SPRAY_HD bool slabBoxClosed(const float4& lo, const float4& hi, const float org[3],
+                            const float inv[3], bool kSafe) {
+  const float ax = (lo.x - org[0]) * inv[0], bx = (hi.x - org[0]) * inv[0];
+  const float ay = (lo.y - org[1]) * inv[1], by = (hi.y - org[1]) * inv[1];
+  const float az = (lo.z - org[2]) * inv[2], bz = (hi.z - org[2]) * inv[2];
+  float nx = fminf(ax, bx), fx = fmaxf(ax, bx);
+  float ny = fminf(ay, by), fy = fmaxf(ay, by);
+  float nz = fminf(az, bz), fz = fmaxf(az, bz);
+  if (kSafe) {
+    const bool qx = ax != ax || bx != bx, qy = ay != ay || by != by, qz = az != az || bz != bz;
+    nx = qx ? -kInfF : nx;
+    fx = qx ? kInfF : fx;
+    ny = qy ? -kInfF : ny;
+    fy = qy ? kInfF : fy;
+    nz = qz ? -kInfF : nz;
+    fz = qz ? kInfF : fz;
+  }
+  const float tmin = fmaxf(fmaxf(nx, ny), nz), tmax = fminf(fminf(fx, fy), fz);
+  return tmin <= tmax && tmin < kInfF && tmax > kRayEps;
+}
+
 // True when 1/dir has an infinite component (dir component 0 or denormal).
 SPRAY_HD bool hasInfInv(const float inv[3]) {
   return fabsf(inv[0]) == kInfF || fabsf(inv[1]) == kInfF || fabsf(inv[2]) == kInfF;

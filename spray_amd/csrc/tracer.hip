@@ -34,6 +34,24 @@ constexpr int kBlock = 256;
 // grid of kCount and kScatter (they must match: blocks own queue slots)
 constexpr int kQueueGrid = 1024;
 constexpr int kShadeRec = 5;  // float4s per shade record
+constexpr int kDomGroup = 8;  // domains per culling group of the domain-list pass
+
+// kCount's dynamic LDS (byte offsets): domain boxes, group boxes, counts,
+// scores, the block's hit-mask words ([word][thread]), domain permutation.
+struct CountLds {
+  size_t gbox, lc, ls, mask, perm, total;
+};
+__host__ __device__ inline CountLds countLds(int nd, int block) {
+  const int ng = (nd + kDomGroup - 1) / kDomGroup, nw = (nd + 63) / 64;
+  CountLds L;
+  L.gbox = (size_t)nd * 32;
+  L.lc = L.gbox + (size_t)ng * 32;
+  L.ls = L.lc + (size_t)nd * 8;
+  L.mask = (L.ls + (size_t)nd * 4 + 7) & ~(size_t)7;
+  L.perm = L.mask + (size_t)nw * block * 8;
+  L.total = L.perm + (size_t)nd * 2;
+  return L;
+}
 constexpr int kShadeBlock = 512;  // threads per shading block (one pool atomic per block)
 // The trace kernel's waves reserve queue entries kRange at a time (one
 // atomic per range, not per refill); the ranges still follow queue order,
@@ -204,51 +222,55 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
 }
 
 // One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
-// entries sorted (registers) and the number of domains hit.
-// The domains hit are also recorded as a bit mask (one word per 64 domains)
-// so that the scatter re-tests only those.
-__device__ __forceinline__ int rayDomains(const float4* box, int nd, const float o[3],
+// entries sorted (registers), the domains hit as a bit mask (one word per 64
+// domains, this lane's words at mask[w * stride]) so that the scatter
+// re-tests only those, and the number of domains hit.  Domains are visited
+// in groups of kDomGroup (Morton order of their centres); a group's members
+// are tested only when some lane of the wave hits the group's box.  The
+// member tests are the exact ones, so the list does not depend on grouping.
+__device__ __forceinline__ int rayDomains(const float4* box, const float4* gbox,
+                                          const uint16_t* perm, int nd, const float o[3],
                                           const float inv[3], bool safe, int spawn, uint32_t* lc_kind,
                                           float tt[kTopDepths], int ti[kTopDepths], int* kept_o,
-                                          unsigned long long* mask_out) {
+                                          unsigned long long* mask, int stride) {
   int kept = 0, hits = 0;
-  unsigned long long mask = 0ull;
-  for (int dd = 0; dd < nd; ++dd) {
-    float tmin;
-    // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
-    // spawning domain was tested right after shading
-    const bool hit = slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) && dd != spawn;
-    mask |= (hit ? 1ull : 0ull) << (dd & 63);
-    if ((dd & 63) == 63 || dd == nd - 1) {  // dd is wave-uniform
-      mask_out[dd >> 6] = mask;
-      mask = 0ull;
-    }
-    if (!hit) continue;
-    ++hits;
-    atomicAdd(&lc_kind[dd], 1u);
-    // insertion into the (t, id)-sorted top list (unrolled: stays in
-    // registers); steps past every active lane's list end are skipped
-    if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
-      float ct = tmin;
-      int ci = dd;
+  const int ng = (nd + kDomGroup - 1) / kDomGroup;
+  for (int g = 0; g < ng; ++g) {
+    if (__ballot(slabBoxClosed(gbox[2 * g], gbox[2 * g + 1], o, inv, safe)) == 0ull) continue;
+    const int k1 = min(nd, (g + 1) * kDomGroup);
+    for (int k = g * kDomGroup; k < k1; ++k) {
+      const int dd = perm[k];  // wave-uniform
+      float tmin;
+      // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
+      // spawning domain was tested right after shading
+      if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
+      ++hits;
+      atomicAdd(&lc_kind[dd], 1u);
+      mask[(dd >> 6) * stride] |= 1ull << (dd & 63);
+      // insertion into the (t, id)-sorted top list (unrolled: stays in
+      // registers); steps past every active lane's list end are skipped
+      if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
+        float ct = tmin;
+        int ci = dd;
 #pragma unroll
-      for (int k = 0; k < kTopDepths; ++k) {
-        if (__ballot(k <= kept) == 0ull) break;
-        if (k < kept) {
-          if (ct < tt[k] || (ct == tt[k] && ci < ti[k])) {
-            float xt = tt[k];
-            int xi = ti[k];
-            tt[k] = ct;
-            ti[k] = ci;
-            ct = xt;
-            ci = xi;
+        for (int j = 0; j < kTopDepths; ++j) {
+          if (__ballot(j <= kept) == 0ull) break;
+          if (j < kept) {
+            if (ct < tt[j] || (ct == tt[j] && ci < ti[j])) {
+              float xt = tt[j];
+              int xi = ti[j];
+              tt[j] = ct;
+              ti[j] = ci;
+              ct = xt;
+              ci = xi;
+            }
+          } else if (j == kept) {
+            tt[j] = ct;
+            ti[j] = ci;
           }
-        } else if (k == kept) {
-          tt[k] = ct;
-          ti[k] = ci;
         }
+        if (kept < kTopDepths) ++kept;
       }
-      if (kept < kTopDepths) ++kept;
     }
   }
   *kept_o = kept;
@@ -261,15 +283,23 @@ __device__ __forceinline__ int rayDomains(const float4* box, int nd, const float
 __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
-  const int nd = P.ndomains;
-  float4* box = (float4*)sm;               // 2 * nd: (lo, 0), (hi, 0)
-  uint32_t* lc = (uint32_t*)(box + 2 * nd);  // 2 * nd counts
-  uint32_t* ls = lc + 2 * nd;                // nd score (32-bit per block)
+  const int nd = P.ndomains, ng = (nd + kDomGroup - 1) / kDomGroup, nw = (nd + 63) >> 6;
+  const CountLds L = countLds(nd, blockDim.x);
+  unsigned char* smc = (unsigned char*)sm;
+  float4* box = (float4*)smc;                                 // 2 * nd: (lo, 0), (hi, 0)
+  float4* gbox = (float4*)(smc + L.gbox);                     // 2 * ng
+  uint32_t* lc = (uint32_t*)(smc + L.lc);                     // 2 * nd counts
+  uint32_t* ls = (uint32_t*)(smc + L.ls);                     // nd score (32-bit per block)
+  unsigned long long* lmask = (unsigned long long*)(smc + L.mask) + threadIdx.x;
+  uint16_t* perm = (uint16_t*)(smc + L.perm);                 // nd
   for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
+  for (int i = threadIdx.x; i < 2 * ng; i += blockDim.x) box[2 * nd + i] = P.dbox[2 * nd + i];
+  const uint16_t* dperm = (const uint16_t*)(P.dbox + 2 * nd + 2 * ng);
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
     lc[i] = 0;
     lc[nd + i] = 0;
     ls[i] = 0;
+    perm[i] = dperm[i];
   }
   __syncthreads();
   const DevCounters* c = P.ctr;
@@ -303,8 +333,10 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     // NaN-checked slab tests only in waves with an axis-parallel ray
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
     if (valid) {
-      hits = rayDomains(box, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti, &kept,
-                        P.dmask + (size_t)i * ((nd + 63) >> 6));
+      for (int w = 0; w < nw; ++w) lmask[w * blockDim.x] = 0ull;
+      hits = rayDomains(box, gbox, perm, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti, &kept,
+                        lmask, blockDim.x);
+      for (int w = 0; w < nw; ++w) P.dmask[(size_t)i * nw + w] = lmask[w * blockDim.x];
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
         if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
@@ -1205,6 +1237,8 @@ static int tilesPerBatch(const Ctx* c, const std::vector<int>& hs) {
   return (int)((n + batches - 1) / batches);
 }
 
+static size_t countShmem(int nd);
+
 static void ensureState(Ctx* c, double grow) {
   const spray_config& cfg = c->cfg;
   std::vector<int> ys, hs;
@@ -1222,6 +1256,9 @@ static void ensureState(Ctx* c, double grow) {
   TracerState* T = c->tracer;
   int nd = c->pool.ndomains;
   if (nd > 1024) throw StatusError(SPRAY_ERR_CAPACITY, "the tracer supports at most 1024 domains");
+  if (countShmem(nd) > 65536)  // many domains: kCount's LDS beyond the default limit
+    SPRAY_HIP(hipFuncSetAttribute((const void*)kCount, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)countShmem(nd)));
   int depths = std::min(cfg.bounces, kHistory);
   double spec = 2.0 * grow;  // headroom for speculative re-shading
   auto cap = [](double v) { return (uint32_t)std::min<double>(3.9e9, v + 4096); };
@@ -1249,7 +1286,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->vbuf, (size_t)S * kHistory * 8);
   need(T, T->rcand, (size_t)T->r_cap + 64);
   need(T, T->dmask, ((size_t)T->r_cap + T->s_cap) * ((nd + 63) / 64) * 8 + 64);
-  need(T, T->dbox, (size_t)nd * 32 + 64);
+  need(T, T->dbox, (size_t)nd * 32 + (size_t)(nd + kDomGroup - 1) / kDomGroup * 32 + nd * 2 + 64);
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
   need(T, T->cnt, (size_t)2 * nd * 4 + 64);
@@ -1381,7 +1418,60 @@ static TraceParams makeParams(Ctx* c) {
   return P;
 }
 
-static size_t countShmem(int nd) { return (size_t)nd * (8 + 3) * 4; }
+static size_t countShmem(int nd) { return countLds(nd, kBlock).total; }
+
+// Domain boxes for the ray/domain slab tests, plus the culling groups of
+// kCount: domains sorted by the Morton code of their box centres (ties by
+// id), kDomGroup consecutive ones per group, each group with its union box.
+static void domainBoxesUpload(Ctx* c, hipStream_t s) {
+  TracerState* T = c->tracer;
+  const int nd = c->pool.ndomains, ng = (nd + kDomGroup - 1) / kDomGroup;
+  const auto& D = c->pool.desc;
+  float smin[3] = {kInfF, kInfF, kInfF}, smax[3] = {-kInfF, -kInfF, -kInfF};
+  for (int d = 0; d < nd; ++d)
+    for (int k = 0; k < 3; ++k) {
+      smin[k] = std::min(smin[k], 0.5f * (D[d].lo[k] + D[d].hi[k]));
+      smax[k] = std::max(smax[k], 0.5f * (D[d].lo[k] + D[d].hi[k]));
+    }
+  std::vector<std::pair<uint32_t, int>> key(nd);
+  for (int d = 0; d < nd; ++d) {
+    uint32_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      const float ext = smax[k] - smin[k];
+      const float x = ext > 0.f ? (0.5f * (D[d].lo[k] + D[d].hi[k]) - smin[k]) / ext : 0.f;
+      const uint32_t q = (uint32_t)std::min(1023.f, std::max(0.f, x * 1023.f));
+      for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + (2 - k));
+    }
+    key[d] = {code, d};
+  }
+  std::sort(key.begin(), key.end());
+  const size_t bytes = (size_t)nd * 32 + (size_t)ng * 32 + (size_t)nd * 2;
+  T->h_box.assign((bytes + 3) / 4, 0.f);
+  float* box = T->h_box.data();
+  float* gbox = box + (size_t)nd * 8;
+  uint16_t* perm = (uint16_t*)(gbox + (size_t)ng * 8);
+  for (int d = 0; d < nd; ++d)
+    for (int k = 0; k < 3; ++k) {
+      box[(size_t)d * 8 + k] = D[d].lo[k];
+      box[(size_t)d * 8 + 4 + k] = D[d].hi[k];
+    }
+  for (int g = 0; g < ng; ++g) {
+    float* gb = gbox + (size_t)g * 8;
+    for (int k = 0; k < 3; ++k) {
+      gb[k] = kInfF;
+      gb[4 + k] = -kInfF;
+    }
+    for (int j = g * kDomGroup; j < std::min(nd, (g + 1) * kDomGroup); ++j) {
+      const int d = key[j].second;
+      perm[j] = (uint16_t)d;
+      for (int k = 0; k < 3; ++k) {
+        gb[k] = std::min(gb[k], D[d].lo[k]);
+        gb[4 + k] = std::max(gb[4 + k], D[d].hi[k]);
+      }
+    }
+  }
+  SPRAY_HIP(hipMemcpyAsync(T->dbox.p, T->h_box.data(), bytes, hipMemcpyHostToDevice, s));
+}
 static size_t scatterShmem(int nd) { return (size_t)nd * (8 + 4) * 4; }  // boxes, slots, bases
 
 static void ensureOverflow(TracerState* T, size_t lanes) { need(T, T->ovf, lanes * kOverflow * 4); }
@@ -1514,16 +1604,7 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     }
     TraceParams P = makeParams(c);
     P.ovf = T->ovf.as<uint32_t>();
-    {  // domain boxes for the ray/domain slab tests (scalar-loaded by the kernels)
-      const int nd = c->pool.ndomains;
-      T->h_box.assign((size_t)nd * 8, 0.f);
-      for (int d = 0; d < nd; ++d)
-        for (int k = 0; k < 3; ++k) {
-          T->h_box[(size_t)d * 8 + k] = c->pool.desc[d].lo[k];
-          T->h_box[(size_t)d * 8 + 4 + k] = c->pool.desc[d].hi[k];
-        }
-      SPRAY_HIP(hipMemcpyAsync(T->dbox.p, T->h_box.data(), (size_t)nd * 32, hipMemcpyHostToDevice, s));
-    }
+    domainBoxesUpload(c, s);
     SPRAY_HIP(hipMemsetAsync(P.image, 0, (size_t)cfg.image_w * cfg.image_h * 16, s));
     SPRAY_HIP(hipMemsetAsync(P.ctr, 0, sizeof(DevCounters), s));
     int windows = (cfg.bounces + kHistory - 1) / kHistory;

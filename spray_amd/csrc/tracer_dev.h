@@ -56,7 +56,9 @@ struct TraceParams {
   DevLight lights[kMaxLights];
   // pool
   const DomainDesc* desc;
-  const float4* dbox;  // per domain: (lo, 0), (hi, 0)
+  // per domain (lo, 0), (hi, 0); then per culling group of kDomGroup
+  // domains the union box; then the domain ids in group order (uint16)
+  const float4* dbox;
   const BNode4* nodes;
   const LTri* tris;
   const float* vtx;

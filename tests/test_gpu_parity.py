@@ -225,6 +225,32 @@ def test_long_domain_lists(tmp_path):
     assert rmse <= 1e-4 and mx <= 1e-3
 
 
+def test_partial_domain_groups(tmp_path):
+    """27 domains (3x3x3): the domain-list pass culls in groups of 8 in Morton
+    order, so here the groups straddle the grid and the last one is partial.
+    Path-traced frames (batched and plain) match the oracle."""
+    import spray_amd as sa
+    d = str(tmp_path)
+    sa.scenegen_write(d, "g27", 3, 3, 3, 24, rgb=0xB3B3B3,
+                      extra_lines="light rect 0.5 3.4 0.5 2 0 0 0 0 2 4 4 4")
+    cam = dict(pos=[1.5, 1.5, 1.5 + 3.0], lookat=[1.5, 1.5, 1.5], up=[0.0, 1.0, 0.0])
+    kw = dict(image_w=64, image_h=64, pixel_samples=2, ao_mode=0, ao_samples=1, bounces=4,
+              rr_start=3, max_samples_per_rank=4096)
+    ref = orc.Scene(os.path.join(d, "g27.spray"), d)
+    rimg, rst = ref.render(nthreads=8, cam_pos=cam["pos"], cam_lookat=cam["lookat"], cam_up=cam["up"], **kw)
+    for schedule in (1, 0):
+        cfg = sa.default_config(has_camera=1, camera_pos=cam["pos"], camera_lookat=cam["lookat"],
+                                camera_up=cam["up"], nthreads=8, schedule=schedule, **kw)
+        with sa.Context(cfg) as ctx:
+            ctx.load_scene(os.path.join(d, "g27.spray"), d)
+            img, st = ctx.trace_frame()
+        rmse, mx = _img_close(img, rimg)
+        print(f"27 domains, schedule {schedule}: rmse {rmse:.3g} max {mx:.3g} useful {st['useful_rays']}")
+        assert st["useful_rays"] == rst["useful_rays"]
+        assert st["final_hits"] == rst["final_hits"]
+        assert rmse <= 1e-4 and mx <= 1e-3
+
+
 def test_axis_parallel_rays(ctx_c1, ref_c1):
     """Rays with zero direction components and origins on grid planes (0 * inf
     in slab tests): closest hits, any hits and domain lists equal the oracle's."""
