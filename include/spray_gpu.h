@@ -82,6 +82,9 @@ typedef struct spray_stats {
   unsigned long long tri_tests;       /* triangles tested by all traversals */
   double trace_ms;                    /* device time of the frame (hipEvents) */
   double build_ms;                    /* domain-pool build time of the last load */
+  unsigned long long cache_hits;      /* domain loads served by the HBM cache (paged pool) */
+  unsigned long long cache_misses;    /* domain loads copied from host memory */
+  unsigned long long cache_bytes;     /* bytes copied host -> HBM by those loads */
 } spray_stats;
 
 typedef struct spray_rays_soa { /* n entries each; host memory */
@@ -168,6 +171,24 @@ int spray_set_tiles_in_flight(spray_ctx* ctx, int tiles);
  * (render/trimesh_buffer.cc:112, RTC_SCENE_DYNAMIC); hits do not depend on
  * the choice, only traversal cost does. */
 int spray_set_bvh_builder(spray_ctx* ctx, int builder);
+
+/* Domain cache size for the next spray_load_scene, in domains
+ * (render/config.cc:200-202 --cache-size; lru_cache.cc:41-171): < 0 or
+ * >= the domain count keeps every domain in HBM (the infinite cache); a
+ * smaller value keeps that many domains in HBM, least recently used evicted,
+ * every domain's image in pinned host memory.  A partial cache supports the
+ * plain schedule (the reference's domain sweep); the batched forms need the
+ * whole scene resident. */
+int spray_set_cache_size(spray_ctx* ctx, int domains);
+
+/* Domain cache state (render/lru_cache.h:50-57, getCacheSize and
+ * getBlocks): *capacity = domains the cache holds; lru_domains (capacity
+ * entries) = the resident domains from least to most recently used, -1
+ * past the last; counters = hits, misses and bytes copied into HBM since the
+ * scene was loaded.  A scene held whole reports every domain resident and
+ * zero counters. */
+int spray_cache_state(spray_ctx* ctx, int* capacity, int* lru_domains,
+                      unsigned long long counters[3]);
 
 /* Kernel timing (new): when enabled, every trace-kernel launch (the
  * per-domain kernel of the plain form, the persistent kernel of the batched

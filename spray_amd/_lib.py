@@ -55,6 +55,7 @@ class spray_stats(C.Structure):
         ("pruned_entries", C.c_ulonglong),
         ("node_visits", C.c_ulonglong), ("tri_tests", C.c_ulonglong),
         ("trace_ms", C.c_double), ("build_ms", C.c_double),
+        ("cache_hits", C.c_ulonglong), ("cache_misses", C.c_ulonglong), ("cache_bytes", C.c_ulonglong),
     ]
 
 
@@ -103,6 +104,8 @@ PROTOTYPES = {
     "spray_set_batch_params": (C.c_int, [_P, C.c_int, C.c_int]),
     "spray_set_tiles_in_flight": (C.c_int, [_P, C.c_int]),
     "spray_set_bvh_builder": (C.c_int, [_P, C.c_int]),
+    "spray_set_cache_size": (C.c_int, [_P, C.c_int]),
+    "spray_cache_state": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_ulonglong)]),
     "spray_profile_kernels": (C.c_int, [_P, C.c_int]),
     "spray_kernel_time": (C.c_int, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.c_int]),
     "spray_predictor_log": (C.c_int, [_P, C.c_int, C.c_longlong]),

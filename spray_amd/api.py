@@ -135,6 +135,19 @@ class Context:
     def set_tiles_in_flight(self, tiles):
         self._check(self.lib.spray_set_tiles_in_flight(self.h, int(tiles)))
 
+    def set_cache_size(self, domains):
+        """Domains kept in HBM from the next load_scene (< 0: all, the default)."""
+        self._check(self.lib.spray_set_cache_size(self.h, int(domains)))
+
+    def cache_state(self):
+        """(capacity, resident domains least to most recently used, hits, misses, bytes)."""
+        cap = C.c_int(0)
+        self._check(self.lib.spray_cache_state(self.h, C.byref(cap), None, None))
+        doms = (C.c_int * cap.value)()
+        cnt = (C.c_ulonglong * 3)()
+        self._check(self.lib.spray_cache_state(self.h, C.byref(cap), doms, cnt))
+        return cap.value, [d for d in doms if d >= 0], int(cnt[0]), int(cnt[1]), int(cnt[2])
+
     def set_bvh_builder(self, builder):
         """0 = Karras LBVH (default), 1 = PLOC; applies to the next load_scene."""
         self._check(self.lib.spray_set_bvh_builder(self.h, int(builder)))

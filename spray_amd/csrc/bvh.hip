@@ -607,18 +607,22 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
   int nd = (int)meshes.size();
   P.ndomains = nd;
   P.desc.resize(nd);
+  // a cache smaller than the scene pages domains (raw meshes are staged one
+  // domain at a time, images kept in pinned host memory)
+  const bool paged = c->cache_size > 0 && c->cache_size < nd;
   long long tv = 0, tf = 0, tn = 0;
   int maxf = 0;
-  long long maxv = 0;
+  long long maxv = 0, max_nodes = 0;
   for (int d = 0; d < nd; ++d) {
     const HostDomain& hd = c->scene.domains[d];
     DomainDesc& D = P.desc[d];
     long long nv = (long long)meshes[d].v.size() / 3, nf = (long long)meshes[d].f.size() / 3;
     if (nf >= (1 << 24)) throw StatusError(SPRAY_ERR_CAPACITY, "domain has more than 2^24 faces");
-    D.vtx_off = tv;
-    D.face_off = tf;
+    D.vtx_off = paged ? 0 : tv;
+    D.face_off = paged ? 0 : tf;
     D.tri_off = tf;
     D.node_off = tn;
+    D.shade_off = D.face_off;
     D.num_tris = (int)nf;
     D.num_nodes = 0;
     for (int k = 0; k < 3; ++k) {
@@ -633,17 +637,42 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
     tn += nf / 2 + 16;  // 4-wide node bound for leaves of <= 4
     maxf = std::max(maxf, (int)nf);
     maxv = std::max(maxv, nv);
+    max_nodes = std::max(max_nodes, nf / 2 + 16);
   }
   P.total_verts = tv;
   P.total_tris = tf;
-  P.vtx.reserve(tv * 12);
-  P.face.reserve(tf * 12);
-  P.rgb.reserve(tv * 4);
-  P.nrm.reserve(tv * 12);
-  P.tris.reserve(tf * sizeof(LTri));
-  P.fshade.reserve(tf * 48 + 64);
-  P.nodes.reserve(tn * sizeof(BNode4));
-  for (int d = 0; d < nd; ++d) {
+  if (paged) {
+    P.paged = true;
+    P.capacity = c->cache_size;
+    P.block_nodes = max_nodes;
+    P.block_tris = maxf;
+    P.vtx.reserve(maxv * 12 + 64);
+    P.face.reserve((size_t)maxf * 12 + 64);
+    P.rgb.reserve(maxv * 4 + 64);
+    P.nrm.reserve(maxv * 12 + 64);
+    P.tris.reserve((size_t)P.capacity * maxf * sizeof(LTri) + 64);
+    P.fshade.reserve((size_t)P.capacity * maxf * 48 + 64);
+    P.nodes.reserve((size_t)P.capacity * max_nodes * sizeof(BNode4) + 64);
+    size_t bytes = 0;
+    P.h_image_off.resize(nd);
+    for (int d = 0; d < nd; ++d) {
+      P.h_image_off[d] = (long long)bytes;
+      const size_t nf = (size_t)P.desc[d].num_tris;
+      bytes += ((nf / 2 + 16) * sizeof(BNode4) + nf * (sizeof(LTri) + 48) + 255) & ~(size_t)255;
+    }
+    SPRAY_HIP(hipHostMalloc(&P.h_image, bytes, hipHostMallocDefault));
+    P.h_nrm.resize(nd);
+    SPRAY_HIP(hipStreamCreate(&P.copy_stream));
+  } else {
+    P.vtx.reserve(tv * 12);
+    P.face.reserve(tf * 12);
+    P.rgb.reserve(tv * 4);
+    P.nrm.reserve(tv * 12);
+    P.tris.reserve(tf * sizeof(LTri));
+    P.fshade.reserve(tf * 48 + 64);
+    P.nodes.reserve(tn * sizeof(BNode4));
+  }
+  for (int d = 0; d < nd && !paged; ++d) {
     const DomainDesc& D = P.desc[d];
     SPRAY_HIP(hipMemcpyAsync(P.vtx.as<float>() + D.vtx_off * 3, meshes[d].v.data(),
                              meshes[d].v.size() * 4, hipMemcpyHostToDevice, s));
@@ -688,6 +717,17 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
     DomainDesc& D = P.desc[d];
     const int nf = D.num_tris;
     const long long nv = (long long)meshes[d].v.size() / 3;
+    if (paged) {  // build into block d % capacity from the staged mesh
+      const int b = d % P.capacity;
+      D.node_off = (long long)b * P.block_nodes;
+      D.tri_off = D.shade_off = (long long)b * P.block_tris;
+      SPRAY_HIP(hipMemcpyAsync(P.vtx.p, meshes[d].v.data(), meshes[d].v.size() * 4,
+                               hipMemcpyHostToDevice, s));
+      SPRAY_HIP(hipMemcpyAsync(P.face.p, meshes[d].f.data(), meshes[d].f.size() * 4,
+                               hipMemcpyHostToDevice, s));
+      SPRAY_HIP(hipMemcpyAsync(P.rgb.p, meshes[d].rgb.data(), meshes[d].rgb.size() * 4,
+                               hipMemcpyHostToDevice, s));
+    }
     const float* vtx = P.vtx.as<float>() + D.vtx_off * 3;
     const uint32_t* face = P.face.as<uint32_t>() + D.face_off * 3;
     // normals
@@ -704,7 +744,7 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
                                                   P.nrm.as<float>() + D.vtx_off * 3);
     kFaceShade<<<gridFor(nf, 256), 256, 0, s>>>(face, nf, P.nrm.as<float>() + D.vtx_off * 3,
                                                  P.rgb.as<uint32_t>() + D.vtx_off,
-                                                 P.fshade.as<float4>() + D.face_off * 3);
+                                                 P.fshade.as<float4>() + D.shade_off * 3);
     // Morton codes
     uint32_t* bmm = scal.as<uint32_t>();
     uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
@@ -727,11 +767,26 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
     BNode4* nodes = P.nodes.as<BNode4>() + D.node_off;
     float* leafbox = lb.as<float>();
     const int node_cap = nf / 2 + 16;
+    // paged pool: the finished domain's image goes to pinned host memory
+    // (and its normals to the host, for the normals query)
+    auto pageOut = [&]() {
+      if (!paged) return;
+      char* dst = (char*)P.h_image + P.h_image_off[d];
+      const size_t nb = (size_t)D.num_nodes * sizeof(BNode4), tb = (size_t)nf * sizeof(LTri);
+      SPRAY_HIP(hipMemcpyAsync(dst, nodes, nb, hipMemcpyDeviceToHost, s));
+      SPRAY_HIP(hipMemcpyAsync(dst + nb, tris, tb, hipMemcpyDeviceToHost, s));
+      SPRAY_HIP(hipMemcpyAsync(dst + nb + tb, P.fshade.as<float4>() + D.shade_off * 3,
+                               (size_t)nf * 48, hipMemcpyDeviceToHost, s));
+      P.h_nrm[d].resize((size_t)nv * 3);
+      SPRAY_HIP(hipMemcpyAsync(P.h_nrm[d].data(), P.nrm.p, (size_t)nv * 12, hipMemcpyDeviceToHost, s));
+      SPRAY_HIP(hipStreamSynchronize(s));
+    };
     if (nf <= kLeafMax) {
       kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris);
       kTinyRoot<<<1, 1, 0, s>>>(nf, leafbox, nodes);
       D.num_nodes = 1;
       P.bvh_depth[d] = 1;
+      pageOut();
       continue;
     }
     int* L = lft.as<int>();
@@ -835,13 +890,25 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
         throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
     }
     P.bvh_depth[d] = depth;
+    pageOut();
   }
   SPRAY_HIP(hipStreamSynchronize(s));
   long long used = 0;
   for (auto& D : P.desc) used += D.num_nodes;
   P.total_nodes = used;
-  P.d_desc.reserve(sizeof(DomainDesc) * nd);
-  SPRAY_HIP(hipMemcpy(P.d_desc.p, P.desc.data(), sizeof(DomainDesc) * nd, hipMemcpyHostToDevice));
+  if (paged) {
+    // the last `capacity` domains built stay in their blocks (least recently
+    // built first); the raw-mesh staging arrays are no longer needed
+    std::vector<std::pair<int, int>> resident;
+    for (int d = std::max(0, nd - P.capacity); d < nd; ++d) resident.push_back({d, d % P.capacity});
+    for (DevBuf* b : {&P.vtx, &P.face, &P.rgb, &P.nrm}) b->release();
+    P.d_desc.reserve(sizeof(DomainDesc) * nd);
+    SPRAY_HIP(hipMemcpy(P.d_desc.p, P.desc.data(), sizeof(DomainDesc) * nd, hipMemcpyHostToDevice));
+    poolInitPaging(c, resident);
+  } else {
+    P.d_desc.reserve(sizeof(DomainDesc) * nd);
+    SPRAY_HIP(hipMemcpy(P.d_desc.p, P.desc.data(), sizeof(DomainDesc) * nd, hipMemcpyHostToDevice));
+  }
   for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
                     &plf, &flags, &fa, &fb, &cnts, &scal, &adj, &pc, &pc2, &pb, &pb2, &pnn, &pcnt,
                     &pmf, &pkf, &pmp, &pkp, &pperm, &pord})

@@ -250,6 +250,10 @@ int spray_domain_normals(spray_ctx* ctx, int id, float* out) {
     needScene(&ctx->c);
     const DomainPool& P = ctx->c.pool;
     if (id < 0 || id >= P.ndomains) throw StatusError(SPRAY_ERR_INVALID, "bad domain id");
+    if (P.paged) {  // normals of a paged pool stay on the host
+      std::copy(P.h_nrm[id].begin(), P.h_nrm[id].end(), out);
+      return;
+    }
     long long nv = (id + 1 < P.ndomains ? P.desc[id + 1].vtx_off : P.total_verts) - P.desc[id].vtx_off;
     SPRAY_HIP(hipMemcpy(out, P.nrm.as<float>() + P.desc[id].vtx_off * 3, (size_t)nv * 12,
                         hipMemcpyDeviceToHost));
@@ -312,7 +316,9 @@ int spray_trace_rays(spray_ctx* ctx, const spray_rays_soa* rays, size_t n, spray
     float* d_u = d_t + n;
     float* d_v = d_u + n;
     int* d_p = (int*)(d_v + n);
+    poolAcquire(c, domain, c->stream);  // paged pool: the domain's block first
     traceRaysDevice(c, d_org, d_dir, d_tn, d_tf, n, mode, domain, d_t, d_u, d_v, d_p);
+    poolRelease(c, domain, c->stream);
     if (mode == SPRAY_QUERY_CLOSEST) {
       SPRAY_HIP(hipMemcpy(hits->t, d_t, n * 4, hipMemcpyDeviceToHost));
       SPRAY_HIP(hipMemcpy(hits->u, d_u, n * 4, hipMemcpyDeviceToHost));
@@ -399,6 +405,44 @@ int spray_set_tiles_in_flight(spray_ctx* ctx, int tiles) {
   return guarded(ctx, [&] {
     if (tiles < 0 || tiles > 4096) throw StatusError(SPRAY_ERR_INVALID, "tiles must be in [0, 4096]");
     ctx->c.tiles_in_flight = tiles;
+  });
+}
+
+int spray_set_cache_size(spray_ctx* ctx, int domains) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    // render/config.cc:200-202 takes any value; lru_cache.cc:44 rejects 0
+    if (domains == 0) throw StatusError(SPRAY_ERR_INVALID, "cache size must not be 0");
+    ctx->c.cache_size = domains;
+  });
+}
+
+int spray_cache_state(spray_ctx* ctx, int* capacity, int* lru_domains,
+                      unsigned long long counters[3]) {
+  if (!ctx || !capacity) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    needScene(&ctx->c);
+    const DomainPool& P = ctx->c.pool;
+    if (!P.paged) {
+      *capacity = P.ndomains;
+      if (lru_domains)
+        for (int d = 0; d < P.ndomains; ++d) lru_domains[d] = d;
+      if (counters) counters[0] = counters[1] = counters[2] = 0;
+      return;
+    }
+    *capacity = P.capacity;
+    if (lru_domains) {
+      std::vector<std::pair<unsigned long long, int>> order;
+      for (int b = 0; b < P.capacity; ++b)
+        if (P.block_domain[b] >= 0) order.push_back({P.block_tick[b], P.block_domain[b]});
+      std::sort(order.begin(), order.end());
+      for (int i = 0; i < P.capacity; ++i) lru_domains[i] = i < (int)order.size() ? order[i].second : -1;
+    }
+    if (counters) {
+      counters[0] = P.hits;
+      counters[1] = P.misses;
+      counters[2] = P.bytes_in;
+    }
   });
 }
 

@@ -70,6 +70,24 @@ struct DomainPool {
   DevBuf fshade;                 // float4[3] per face: vertex normal xyz + vertex rgb bits
   std::vector<int> bvh_depth;
   double build_ms = 0;
+  // LRU paging (lru_cache.cc:41-171): with a cache smaller than the scene,
+  // HBM holds `capacity` blocks of block_nodes nodes and block_tris
+  // triangles + shading records, and every domain's image (nodes, leaf-ordered
+  // triangles, shading records) stays in pinned host memory.
+  bool paged = false;
+  int capacity = 0;
+  long long block_nodes = 0, block_tris = 0;
+  void* h_image = nullptr;                // pinned: per domain nodes, tris, shading records
+  std::vector<long long> h_image_off;     // byte offset of each domain's image
+  std::vector<std::vector<float>> h_nrm;  // per-domain vertex normals (query API)
+  DomainDesc* h_desc_pinned = nullptr;    // staging for descriptor uploads
+  std::vector<int> block_domain, domain_block;
+  std::vector<unsigned long long> block_tick;
+  std::vector<hipEvent_t> block_free, block_ready;
+  std::vector<char> pending;              // prefetched, access not yet made
+  unsigned long long tick = 0, hits = 0, misses = 0, bytes_in = 0;
+  hipStream_t copy_stream = nullptr;
+  void releasePaging();
   void release() {
     d_desc.release();
     vtx.release();
@@ -79,6 +97,7 @@ struct DomainPool {
     nodes.release();
     tris.release();
     fshade.release();
+    releasePaging();
     desc.clear();
     ndomains = 0;
   }
@@ -104,6 +123,7 @@ struct Ctx {
   int batch_chunk = 16;  // refill threshold of the trace kernel (idle lanes per wave)
   int tiles_in_flight = 0;  // blocking tiles traced together (batched forms); 0 = automatic
   int bvh_builder = 0;     // 0 = Karras LBVH, 1 = PLOC (next scene load)
+  int cache_size = -1;     // domains resident in HBM (next scene load); < 0 or >= ndomains: all
   int batch_tile = 4;      // waves per block of the persistent kernel (block = 64 * tile)
   int param_version = 0;   // bumped when camera / config change (graph re-capture)
   bool profile = false;    // time every trace-kernel launch with hipEvents
@@ -114,6 +134,13 @@ struct Ctx {
 
 // bvh.hip
 void buildPool(Ctx* c, std::vector<HostMesh>& meshes);
+// paged pool (pool_cache.cc): make domain d resident and have stream s wait
+// for it; start loading d ahead (never evicting `keep`); record that the
+// kernels on s that use d's block have been issued; cache state after a build
+void poolAcquire(Ctx* c, int d, hipStream_t s);
+void poolPrefetch(Ctx* c, int d, int keep);
+void poolRelease(Ctx* c, int d, hipStream_t s);
+void poolInitPaging(Ctx* c, const std::vector<std::pair<int, int>>& resident);
 void traceRaysDevice(Ctx* c, const float* d_org, const float* d_dir, const float* d_tnear,
                      const float* d_tfar, size_t n, int mode, int domain, float* d_t, float* d_u,
                      float* d_v, int* d_prim);

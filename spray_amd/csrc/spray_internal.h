@@ -89,6 +89,7 @@ struct DomainDesc {
   long long tri_off;   // into the pool's leaf-ordered triangle array
   long long vtx_off;   // into vertex / normal / colour arrays
   long long face_off;  // into the face array (3 uint32 per face)
+  long long shade_off; // into the per-face shading records (the domain's block when paged)
   int num_tris;
   int num_nodes;
   float lo[3], hi[3];

@@ -186,6 +186,8 @@ __global__ void kPassBegin(TraceParams P) {
   if (c->active) c->passes++;
   c->vlog_count = 0;
   c->shade_count = 0;
+  c->sh_mark = 0;
+  c->s_local = c->s_mark;
   c->n_phase[0] = 0;
   c->n_phase[1] = 0;
 }
@@ -641,7 +643,7 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, BlockAppend& ap
   uint32_t color = 0;
   f3 normal = mk(0, 0, 1);
   if (act) {  // render/trimesh_buffer.cc:327-360
-    const float4* fs = P.fshade + (size_t)(D.face_off + prim) * 3;
+    const float4* fs = P.fshade + (size_t)(D.shade_off + prim) * 3;
     const float4 a0 = fs[0], a1 = fs[1], a2 = fs[2];
     uint32_t c0 = __float_as_uint(a0.w), c1 = __float_as_uint(a1.w), c2 = __float_as_uint(a2.w);
     float w = 1.f - u - v;
@@ -830,7 +832,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     ns = P.cnt[nd + plain_dom];
     wc = &P.wctr[plain_dom];
   } else if (local) {
-    local_base = c->s_mark;
+    local_base = c->s_local;
     uint32_t end = min(c->s_count, P.s_cap);
     ns = end > local_base ? end - local_base : 0u;
     wc = &P.wctr[nd + 2];
@@ -1009,10 +1011,10 @@ __global__ void __launch_bounds__(kShadeBlock, 4) kShade(TraceParams P) {
   DevCounters* c = P.ctr;
   if (!c->active) return;
   BlockAppend app{app_sm, 0u};
-  const uint32_t n = min(c->shade_count, P.sh_cap);
+  const uint32_t n = min(c->shade_count, P.sh_cap), first = c->sh_mark;
   const int ray_depth = P.tile_rec[2];
   unsigned long long shades = 0;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+  for (uint32_t base = first + blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
     float4 op = make_float4(0, 0, 0, 0), ds = make_float4(1, 0, 0, 0), wd = make_float4(0, 0, 0, 0);
@@ -1040,6 +1042,16 @@ __global__ void __launch_bounds__(kShadeBlock, 4) kShade(TraceParams P) {
     shadeSpawn(P, app, act, dom, op, ds, wd, hist, depth, t, u, v, prim, ray_depth);
   }
   blockFlush<1>({&c->shades}, {waveSumULL(shades)});
+}
+
+// After one domain of the paged sweep: its shade records and spawned shadow
+// rays are done, and the local-shadow work counter starts again at zero.
+__global__ void kDomainDone(TraceParams P) {
+  DevCounters* c = P.ctr;
+  if (!c->active) return;
+  c->sh_mark = min(c->shade_count, P.sh_cap);
+  c->s_local = min(c->s_count, P.s_cap);
+  P.wctr[P.ndomains + 2] = 0;
 }
 
 // Retire (ooc_tcontext.inl:123-135): shadow rays tested in this pass.
@@ -1489,19 +1501,29 @@ static void passQueues(Ctx* c, const TraceParams& P) {
   kScatter<<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);
 }
 
-static void passFinish(Ctx* c, const TraceParams& P) {
+// Shading of the pass's accepted hits (those not yet shaded), then the new
+// shadow rays' any-hit test in their spawning domain (procShads2).
+static void passShade(Ctx* c, const TraceParams& P) {
   TracerState* T = c->tracer;
   hipStream_t s = c->stream;
   kShade<<<1024, kShadeBlock, 0, s>>>(P);
-  // new shadow rays: any hit in the spawning domain first (procShads2)
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
   int block = 64 * std::max(1, c->batch_tile);
   int ei = c->profile ? T->evBegin(s) : -1;
   kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -2, 0);
   if (ei >= 0) T->evEnd(ei, s);
+}
+
+static void passRetire(Ctx* c, const TraceParams& P) {
+  hipStream_t s = c->stream;
   kRetire<<<1024, 256, 0, s>>>(P);
   if (P.log_on) kLogFinalize<<<1024, 256, 0, s>>>(P);
+}
+
+static void passFinish(Ctx* c, const TraceParams& P) {
+  passShade(c, P);
+  passRetire(c, P);
 }
 
 // One pass of the plain schedule: the host reads the per-domain counts and
@@ -1524,16 +1546,34 @@ static bool passPlain(Ctx* c, const TraceParams& P) {
   ensureOverflow(T, (size_t)max_grid * kTravBlock);
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
+  std::vector<int> doms;  // domains with entries, in schedule order
   for (int k = 0; k < nd; ++k) {
     int d = T->h_order[k];
+    if (T->h_cnt[d] + T->h_cnt[nd + d] > 0) doms.push_back(d);
+  }
+  // with a paged pool every domain is loaded in turn (the next one ahead, on
+  // the copy stream), and its hits are shaded and its new shadow rays tested
+  // while it is resident
+  const bool paged = c->pool.paged;
+  for (size_t j = 0; j < doms.size(); ++j) {
+    const int d = doms[j];
+    poolAcquire(c, d, s);
+    if (j + 1 < doms.size()) poolPrefetch(c, doms[j + 1], d);
     uint32_t n = T->h_cnt[d] + T->h_cnt[nd + d];
-    if (n == 0) continue;
     int grid = (int)std::min<uint32_t>(max_grid, (n + kTravBlock - 1) / kTravBlock);
     int ei = c->profile ? T->evBegin(s) : -1;
     kTraverse<<<grid, kTravBlock, travShmem(kTravBlock), s>>>(Q, d, 0);
     if (ei >= 0) T->evEnd(ei, s);
+    if (paged) {
+      passShade(c, P);
+      kDomainDone<<<1, 1, 0, s>>>(P);
+    }
+    poolRelease(c, d, s);
   }
-  passFinish(c, P);
+  if (paged)
+    passRetire(c, P);
+  else
+    passFinish(c, P);
   return true;
 }
 
@@ -1587,8 +1627,13 @@ static void tileBody(Ctx* c, const TraceParams& P, bool batched, int windows, in
 
 void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
   const spray_config& cfg = c->cfg;
+  if (c->pool.paged && cfg.schedule != SPRAY_SCHEDULE_PLAIN)
+    throw StatusError(SPRAY_ERR_STATE,
+                      "a domain cache smaller than the scene needs the plain schedule");
   std::vector<int> ys, hs;
   tileList(cfg, &ys, &hs);
+  const unsigned long long hits0 = c->pool.hits, misses0 = c->pool.misses,
+                           bytes0 = c->pool.bytes_in;
   double grow = 1.0;
   for (int attempt = 0; attempt < 4; ++attempt) {
     ensureState(c, grow);
@@ -1668,6 +1713,9 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     st->trace_ms = ms;
     st->node_visits = h.node_visits;
     st->tri_tests = h.tri_tests;
+    st->cache_hits = c->pool.hits - hits0;
+    st->cache_misses = c->pool.misses - misses0;
+    st->cache_bytes = c->pool.bytes_in - bytes0;
     return;
   }
   throw StatusError(SPRAY_ERR_CAPACITY, "ray pools exceeded after growing 8x");

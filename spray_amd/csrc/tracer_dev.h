@@ -38,6 +38,8 @@ struct DevCounters {
   uint32_t q_total[2];                  // entries in this pass's queues
   uint32_t vlog_count;                  // visit-log entries of this pass
   uint32_t shade_count;                 // accepted hits of this pass (shade records)
+  uint32_t sh_mark, s_local;            // shade records / spawned shadow rays already processed
+                                        // (advanced per domain by the paged domain sweep)
   uint32_t n_phase[2];                  // learned form: radiance entries in phase A / B
   unsigned long long useful, radiance_visits, shadow_visits, shades, final_hits, queue_entries,
       pruned, deferred, passes, node_visits, tri_tests, log_count;
