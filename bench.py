@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-tiles", type=int, default=0,
                     help="blocking tiles timed on the CPU (0: the workload's default)")
+    ap.add_argument("--cache-domains", type=int, default=-1,
+                    help="domains kept in HBM (< 0: all); a partial cache pages domains (plain schedule)")
     ap.add_argument("--bvh", default="lbvh", choices=["ploc", "lbvh"], help="per-domain BVH builder")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default batch parameters")
     ap.add_argument("--no-graph", action="store_true",
@@ -130,6 +132,7 @@ def main():
     cfg = sa.default_config(nthreads=min(16, os.cpu_count() or 1), **camkw, **kw)
     ctx = sa.Context(cfg)
     ctx.set_bvh_builder({"lbvh": 0, "ploc": 1}[args.bvh])
+    ctx.set_cache_size(args.cache_domains)
     t0 = time.time()
     ctx.load_scene(os.path.join(tmp, args.workload + ".spray"), tmp)
     load_s = time.time() - t0
@@ -161,8 +164,9 @@ def main():
         except sa.SprayError as e:
             results[name] = {"error": str(e)}
             continue
-        for _ in range(args.warmup):
+        for k in range(args.warmup):
             ctx.trace_frame(want_image=False)
+            print(f"# {name}: warmup frame {k + 1}/{args.warmup}", file=sys.stderr, flush=True)
         frame_ms, walls = [], []
         st = None
         t_start = time.perf_counter()
@@ -171,6 +175,8 @@ def main():
             img, st = ctx.trace_frame(want_image=False)
             walls.append(time.perf_counter() - tw)
             frame_ms.append(st["trace_ms"])
+            print(f"# {name}: frame {len(walls)}/{args.steps} {1000.0 * walls[-1]:.1f} ms", file=sys.stderr,
+                  flush=True)
         total_s = time.perf_counter() - t_start
         img, st_img = ctx.trace_frame(want_image=True)
         images[name] = img
@@ -193,6 +199,9 @@ def main():
             node_visits=st["node_visits"], tri_tests=st["tri_tests"],
             trace_kernel_launches=launches, trace_kernel_ms=kern_ms,
             trace_kernel_bytes=algorithmic_bytes(st_prof),
+            cache_hits=st["cache_hits"], cache_misses=st["cache_misses"],
+            cache_bytes=st["cache_bytes"],
+            cache_h2d_gbs=st["cache_bytes"] / (st["trace_ms"] * 1e6) if st["trace_ms"] > 0 else 0.0,
         )
     ref_name = "plain" if "plain" in images else next(iter(images), None)
     for name, img in images.items():

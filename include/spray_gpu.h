@@ -176,9 +176,11 @@ int spray_set_bvh_builder(spray_ctx* ctx, int builder);
  * (render/config.cc:200-202 --cache-size; lru_cache.cc:41-171): < 0 or
  * >= the domain count keeps every domain in HBM (the infinite cache); a
  * smaller value keeps that many domains in HBM, least recently used evicted,
- * every domain's image in pinned host memory.  A partial cache supports the
- * plain schedule (the reference's domain sweep); the batched forms need the
- * whole scene resident. */
+ * every domain's image in pinned host memory.  With a partial cache the
+ * plain schedule sweeps the domains one by one (the reference's domain
+ * loop) and the batched schedule traces chunks of consecutive scheduled
+ * domains that fit in half the cache while the next chunk loads; the learned
+ * schedule needs the whole scene resident. */
 int spray_set_cache_size(spray_ctx* ctx, int domains);
 
 /* Domain cache state (render/lru_cache.h:50-57, getCacheSize and

@@ -85,6 +85,7 @@ struct DomainPool {
   std::vector<unsigned long long> block_tick;
   std::vector<hipEvent_t> block_free, block_ready;
   std::vector<char> pending;              // prefetched, access not yet made
+  std::vector<char> pinned;               // in use by the chunk being traced: never evicted
   unsigned long long tick = 0, hits = 0, misses = 0, bytes_in = 0;
   hipStream_t copy_stream = nullptr;
   void releasePaging();

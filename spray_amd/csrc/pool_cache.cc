@@ -34,6 +34,7 @@ void DomainPool::releasePaging() {
   domain_block.clear();
   block_tick.clear();
   pending.clear();
+  pinned.clear();
   paged = false;
   capacity = 0;
   tick = hits = misses = bytes_in = 0;
@@ -80,7 +81,7 @@ static int lruLoad(Ctx* c, int d, int keep) {
       victim = i;
       break;
     }
-    if (keep >= 0 && P.block_domain[i] == keep) continue;
+    if ((keep >= 0 && P.block_domain[i] == keep) || P.pinned[P.block_domain[i]]) continue;
     if (victim < 0 || P.block_tick[i] < P.block_tick[victim]) victim = i;
   }
   if (victim < 0) throw StatusError(SPRAY_ERR_STATE, "no evictable cache block");
@@ -136,6 +137,7 @@ void poolInitPaging(Ctx* c, const std::vector<std::pair<int, int>>& resident) {
   P.domain_block.assign(nd, -1);
   P.block_tick.assign(P.capacity, 0);
   P.pending.assign(nd, 0);
+  P.pinned.assign(nd, 0);
   P.block_free.resize(P.capacity);
   P.block_ready.resize(P.capacity);
   for (int b = 0; b < P.capacity; ++b) {

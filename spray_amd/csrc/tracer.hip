@@ -821,6 +821,8 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   //     form, phase 0 takes the phase-A radiance list, phase 1 the deferred list
   //   plain_dom == -2: shadow rays spawned in this pass, in their spawning
   //     domain (procShads2, ooc_tcontext.inl:51-70)
+  //   plain_dom == -3: the entry ranges P.rng of both queues (the chunks of
+  //     consecutive domains a paged pool traces while they are resident)
   uint32_t nr = 0, ns = 0, rbase = 0, sbase = 0, local_base = 0;
   const uint32_t* rlist = nullptr;
   uint32_t* wc;
@@ -831,6 +833,12 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     nr = P.cnt[plain_dom];
     ns = P.cnt[nd + plain_dom];
     wc = &P.wctr[plain_dom];
+  } else if (plain_dom == -3) {
+    rbase = P.rng[0];
+    nr = P.rng[1];
+    sbase = P.rng[2];
+    ns = P.rng[3];
+    wc = &P.wctr[nd];
   } else if (local) {
     local_base = c->s_local;
     uint32_t end = min(c->s_count, P.s_cap);
@@ -1044,13 +1052,15 @@ __global__ void __launch_bounds__(kShadeBlock, 4) kShade(TraceParams P) {
   blockFlush<1>({&c->shades}, {waveSumULL(shades)});
 }
 
-// After one domain of the paged sweep: its shade records and spawned shadow
-// rays are done, and the local-shadow work counter starts again at zero.
+// After one domain (or chunk of domains) of a paged pool: its shade records
+// and spawned shadow rays are done, and the work counters of the range and
+// local-shadow launches start again at zero.
 __global__ void kDomainDone(TraceParams P) {
   DevCounters* c = P.ctr;
   if (!c->active) return;
   c->sh_mark = min(c->shade_count, P.sh_cap);
   c->s_local = min(c->s_count, P.s_cap);
+  P.wctr[P.ndomains] = 0;
   P.wctr[P.ndomains + 2] = 0;
 }
 
@@ -1161,6 +1171,7 @@ struct TracerState {
   std::vector<uint32_t> h_cnt;
   std::vector<float> h_box;
   std::vector<int> h_order;
+  unsigned long long paged_passes = 0;  // alternates the chunk direction of paged passes
   // kernel timing of the trace launches (hipEvents on the tracer's stream)
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
@@ -1580,6 +1591,81 @@ static bool passPlain(Ctx* c, const TraceParams& P) {
 // One pass of the batched schedule: every queue of the pass in one launch
 // (two with the learned form: deferred entries run after the others);
 // sizes stay on the device so the pass can be captured in a graph.
+// Batched pass over a paged pool.  The queues are laid out in schedule
+// order, so consecutive scheduled domains own contiguous entry ranges: the
+// pass traces them in chunks of half the cache, one persistent launch per
+// chunk, while the next chunk's domains load on the copy stream.  Each
+// chunk's hits are shaded and its new shadow rays tested while it is
+// resident.  Chunks run forward and backward in alternate passes, so the
+// last chunk of a pass is still resident for the first of the next.
+static bool passPagedBatched(Ctx* c, const TraceParams& P) {
+  TracerState* T = c->tracer;
+  DomainPool& pool = c->pool;
+  hipStream_t s = c->stream;
+  const int nd = P.ndomains;
+  passQueues(c, P);
+  DevCounters h;
+  SPRAY_HIP(hipMemcpyAsync(&h, P.ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+  SPRAY_HIP(hipMemcpyAsync(T->h_cnt.data(), P.cnt, 2 * nd * 4, hipMemcpyDeviceToHost, s));
+  SPRAY_HIP(hipMemcpyAsync(T->h_order.data(), P.order, nd * 4, hipMemcpyDeviceToHost, s));
+  SPRAY_HIP(hipStreamSynchronize(s));
+  if (!h.active) return false;
+  // entry offsets by schedule rank, and chunks of ranks holding `per` domains with entries
+  std::vector<uint32_t> off0(nd + 1, 0), off1(nd + 1, 0);
+  for (int k = 0; k < nd; ++k) {
+    off0[k + 1] = off0[k] + T->h_cnt[T->h_order[k]];
+    off1[k + 1] = off1[k] + T->h_cnt[nd + T->h_order[k]];
+  }
+  const int per = std::max(1, pool.capacity / 2);
+  std::vector<std::pair<int, int>> chunks;  // [k0, k1) in schedule rank
+  for (int k = 0, k0 = 0, n = 0; k < nd; ++k) {
+    const int d = T->h_order[k];
+    n += (T->h_cnt[d] + T->h_cnt[nd + d] > 0) ? 1 : 0;
+    if (n == per || k == nd - 1) {
+      if (n > 0) chunks.push_back({k0, k + 1});
+      k0 = k + 1;
+      n = 0;
+    }
+  }
+  if (T->paged_passes++ & 1) std::reverse(chunks.begin(), chunks.end());
+  auto chunkDomains = [&](const std::pair<int, int>& ch) {
+    std::vector<int> v;
+    for (int k = ch.first; k < ch.second; ++k) {
+      const int d = T->h_order[k];
+      if (T->h_cnt[d] + T->h_cnt[nd + d] > 0) v.push_back(d);
+    }
+    return v;
+  };
+  const int block = 64 * std::max(1, c->batch_tile);
+  TraceParams Q = P;
+  Q.ovf = T->ovf.as<uint32_t>();
+  for (size_t j = 0; j < chunks.size(); ++j) {
+    const std::vector<int> cur = chunkDomains(chunks[j]);
+    for (int d : cur) {
+      poolAcquire(c, d, s);
+      pool.pinned[d] = 1;
+    }
+    if (j + 1 < chunks.size())
+      for (int d : chunkDomains(chunks[j + 1])) poolPrefetch(c, d, -1);
+    const int k0 = chunks[j].first, k1 = chunks[j].second;
+    Q.rng[0] = off0[k0];
+    Q.rng[1] = off0[k1] - off0[k0];
+    Q.rng[2] = off1[k0];
+    Q.rng[3] = off1[k1] - off1[k0];
+    int ei = c->profile ? T->evBegin(s) : -1;
+    kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -3, 0);
+    if (ei >= 0) T->evEnd(ei, s);
+    passShade(c, P);
+    kDomainDone<<<1, 1, 0, s>>>(P);
+    for (int d : cur) {
+      poolRelease(c, d, s);
+      pool.pinned[d] = 0;
+    }
+  }
+  passRetire(c, P);
+  return true;
+}
+
 static void passBatched(Ctx* c, const TraceParams& P) {
   TracerState* T = c->tracer;
   hipStream_t s = c->stream;
@@ -1613,7 +1699,9 @@ static void tileBody(Ctx* c, const TraceParams& P, bool batched, int windows, in
     }
     kVbufReset<<<eye_grid, 256, 0, s>>>(P);
     for (int p = 0; p <= kHistory; ++p) {
-      if (batched) {
+      if (batched && c->pool.paged) {
+        if (!passPagedBatched(c, P)) break;
+      } else if (batched) {
         passBatched(c, P);
       } else {
         if (!passPlain(c, P)) break;
@@ -1627,9 +1715,9 @@ static void tileBody(Ctx* c, const TraceParams& P, bool batched, int windows, in
 
 void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
   const spray_config& cfg = c->cfg;
-  if (c->pool.paged && cfg.schedule != SPRAY_SCHEDULE_PLAIN)
+  if (c->pool.paged && cfg.schedule == SPRAY_SCHEDULE_LEARNED)
     throw StatusError(SPRAY_ERR_STATE,
-                      "a domain cache smaller than the scene needs the plain schedule");
+                      "the learned schedule needs the whole scene resident (no partial domain cache)");
   std::vector<int> ys, hs;
   tileList(cfg, &ys, &hs);
   const unsigned long long hits0 = c->pool.hits, misses0 = c->pool.misses,
@@ -1659,7 +1747,8 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     SPRAY_HIP(hipEventRecord(e0, s));
     int key[6] = {(int)batched, windows, (int)P.learned, c->batch_chunk, c->batch_tile,
                   P.log_on + 2 * c->param_version};
-    const bool use_graph = batched && !c->profile;  // profiling times each launch instead
+    // profiling times each launch instead; a paged pool is driven from the host
+    const bool use_graph = batched && !c->profile && !c->pool.paged;
     if (use_graph && (T->graph == nullptr || std::memcmp(key, T->graph_key, sizeof(key)) != 0)) {
       T->releaseGraph();
       hipGraph_t g;

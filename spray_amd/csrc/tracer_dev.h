@@ -89,6 +89,7 @@ struct TraceParams {
   QEntry* q[2];       // queues (radiance, shadow)
   uint16_t* qdom[2];  // domain of each queue entry
   uint8_t* rcand;     // radiance rays: queue entries of this pass (saturating)
+  uint32_t rng[4];    // range launches: radiance base, count, shadow base, count
   unsigned long long* dmask;  // [pass ray][(nd + 63) / 64]: domains hit (kCount -> kScatter)
   uint32_t q_cap[2];
   int refill;         // a wave refetches work once this many lanes are idle

@@ -100,8 +100,24 @@ def test_paged_normals(scene, ref):
             assert np.array_equal(ctx.normals(dom), ref.mesh(dom)[3])
 
 
-def test_paged_needs_plain_schedule(scene):
+@pytest.mark.parametrize("cache", [8, 3])
+def test_paged_batched_frame_matches_oracle(scene, ref, cache):
+    """The batched schedule over a partial cache traces chunks of consecutive
+    scheduled domains (half the cache each, the next chunk loading meanwhile,
+    alternate passes in reverse) and gives the oracle's image and counts."""
+    rimg, rst = ref.render(nthreads=8, cam_pos=CAM["pos"], cam_lookat=CAM["lookat"], cam_up=CAM["up"], **PT)
+    with _ctx(scene, cache, schedule=1) as ctx:
+        img, st = ctx.trace_frame()
+    err = np.abs(img[..., :3] - rimg[..., :3])
+    rmse, mx = float(np.sqrt((err ** 2).mean())), float(err.max())
+    assert st["useful_rays"] == rst["useful_rays"]
+    assert st["final_hits"] == rst["final_hits"]
+    assert rmse <= 1e-4 and mx <= 1e-3
+    assert st["cache_misses"] > 0
+
+
+def test_paged_learned_needs_whole_scene(scene):
     import spray_amd as sa
-    with _ctx(scene, 8, schedule=1) as ctx:
+    with _ctx(scene, 8, schedule=2) as ctx:
         with pytest.raises(sa.SprayError):
             ctx.trace_frame()
