@@ -72,6 +72,8 @@ def parse():
                     help="blocking tiles timed on the CPU (0: the workload's default)")
     ap.add_argument("--cache-domains", type=int, default=-1,
                     help="domains kept in HBM (< 0: all); a partial cache pages domains (plain schedule)")
+    ap.add_argument("--tiles-in-flight", type=int, default=0,
+                    help="blocking tiles per batch of the batched forms (0: about 8M samples)")
     ap.add_argument("--bvh", default="lbvh", choices=["ploc", "lbvh"], help="per-domain BVH builder")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default batch parameters")
     ap.add_argument("--no-graph", action="store_true",
@@ -133,6 +135,7 @@ def main():
     ctx = sa.Context(cfg)
     ctx.set_bvh_builder({"lbvh": 0, "ploc": 1}[args.bvh])
     ctx.set_cache_size(args.cache_domains)
+    ctx.set_tiles_in_flight(args.tiles_in_flight)
     t0 = time.time()
     ctx.load_scene(os.path.join(tmp, args.workload + ".spray"), tmp)
     load_s = time.time() - t0

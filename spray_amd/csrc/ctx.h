@@ -107,7 +107,8 @@ struct DomainPool {
 struct TracerState;  // tracer.hip
 struct Predictor;    // mlp.hip
 
-// Samples in flight the batched forms aim for when they group blocking tiles.
+// Smallest sample budget of a batch of blocking tiles (the budget grows with
+// the HBM left after the scene is loaded).
 constexpr long long kBatchSamples = 8ll << 20;
 
 struct Ctx {
@@ -123,6 +124,7 @@ struct Ctx {
   Predictor* pred = nullptr;
   int batch_chunk = 16;  // refill threshold of the trace kernel (idle lanes per wave)
   int tiles_in_flight = 0;  // blocking tiles traced together (batched forms); 0 = automatic
+  long long batch_samples = 0;  // automatic batches: samples in flight (set from free HBM)
   int bvh_builder = 0;     // 0 = Karras LBVH, 1 = PLOC (next scene load)
   int cache_size = -1;     // domains resident in HBM (next scene load); < 0 or >= ndomains: all
   int batch_tile = 4;      // waves per block of the persistent kernel (block = 64 * tile)

@@ -1157,7 +1157,7 @@ __global__ void kLogFinalize(TraceParams P) {
 // Host side
 struct TracerState {
   long long S = 0;  // samples of the largest tile
-  uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0};
+  uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0;
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
       wctr, score, order, q[2], qdom[2], rcand, dmask, dbox, ovf, vlog, log, sh_rec, qa,
@@ -1172,6 +1172,16 @@ struct TracerState {
   std::vector<float> h_box;
   std::vector<int> h_order;
   unsigned long long paged_passes = 0;  // alternates the chunk direction of paged passes
+  size_t heldBytes() const {
+    size_t b = log_ctr.bytes;
+    for (const DevBuf* x : {R, Pn, Sh})
+      for (int k = 0; k < 4; ++k) b += x[k].bytes;
+    for (const DevBuf* x : {&s_tmax, &s_occ, &vbuf, &image, &ctr, &tile_rec, &cnt, &off, &dbase, &bcnt,
+                            &bbase, &wctr, &score, &order, &q[0], &q[1], &qdom[0], &qdom[1], &rcand,
+                            &dmask, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb})
+      b += x->bytes;
+    return b;
+  }
   // kernel timing of the trace launches (hipEvents on the tracer's stream)
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
@@ -1255,17 +1265,39 @@ static int tilesPerBatch(const Ctx* c, const std::vector<int>& hs) {
   const long long n = (long long)hs.size();
   if (c->tiles_in_flight > 0) return (int)std::min<long long>(c->tiles_in_flight, n);
   // as many tiles as fit the sample budget, spread evenly over the batches
-  const long long kmax = std::max<long long>(1, std::min<long long>(n, kBatchSamples / per_tile));
+  const long long budget = std::max(kBatchSamples, c->batch_samples);
+  const long long kmax = std::max<long long>(1, std::min<long long>(n, budget / per_tile));
   const long long batches = (n + kmax - 1) / kmax;
   return (int)((n + batches - 1) / batches);
 }
 
 static size_t countShmem(int nd);
 
+// Device bytes of tracer state per sample in flight: the capacities that
+// ensureState sets at grow = 1, per sample.
+static double stateBytesPerSample(const Ctx* c) {
+  const spray_config& cfg = c->cfg;
+  const int nd = c->pool.ndomains;
+  const double spec = 2.0, depths = std::min(cfg.bounces, kHistory), spv = spawnPerVertex(c);
+  const double per_ray = std::min(nd, 12), words = (nd + 63) / 64;
+  const double r = 1.0 + spec * depths, s = spv * spec * depths, p = spec;
+  const double q0 = spec * per_ray, q1 = spv * spec * per_ray * 0.5;
+  return r * 64 + s * 72 + p * 64 + q0 * (8 + 2 + 8) + std::min(q0, r) * kShadeRec * 16 + q1 * 10 +
+         (r + s) * 8 * words + 32 + r;
+}
+
 static void ensureState(Ctx* c, double grow) {
   const spray_config& cfg = c->cfg;
   std::vector<int> ys, hs;
   tileList(cfg, &ys, &hs);
+  {  // automatic batches: samples whose state takes at most 40% of the HBM the
+     // tracer can use (free plus what it already holds), so that one grown
+     // retry still fits
+    size_t free_b = 0, total_b = 0;
+    SPRAY_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double held = c->tracer ? (double)c->tracer->heldBytes() : 0.0;
+    c->batch_samples = (long long)(0.40 * ((double)free_b + held) / stateBytesPerSample(c));
+  }
   const int per_batch = cfg.schedule != SPRAY_SCHEDULE_PLAIN ? tilesPerBatch(c, hs) : 1;
   long long rows = 0;
   for (size_t b0 = 0; b0 < hs.size(); b0 += per_batch) {
@@ -1320,7 +1352,10 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->wctr, (size_t)(nd + 3) * 4 + 64);
   need(T, T->qa, (size_t)T->q_cap[0] * 4);
   need(T, T->qb, (size_t)T->q_cap[0] * 4);
-  need(T, T->sh_rec, (size_t)T->q_cap[0] * kShadeRec * 16);
+  // shade records: one per accepted hit of a pass, far fewer than queue
+  // entries (an overflow grows and re-renders like any other pool)
+  T->sh_cap = std::min(T->q_cap[0], T->r_cap);
+  need(T, T->sh_rec, (size_t)T->sh_cap * kShadeRec * 16);
   need(T, T->score, (size_t)nd * 8 + 64);
   need(T, T->order, (size_t)nd * 4 + 64);
   need(T, T->q[0], (size_t)T->q_cap[0] * sizeof(QEntry));
@@ -1427,7 +1462,7 @@ static TraceParams makeParams(Ctx* c) {
   P.q_cap[1] = T->q_cap[1];
   P.refill = std::min(64, std::max(1, c->batch_chunk));
   P.sh_rec = T->sh_rec.as<float4>();
-  P.sh_cap = T->q_cap[0];
+  P.sh_cap = T->sh_cap;
   P.qa = T->qa.as<uint32_t>();
   P.qb = T->qb.as<uint32_t>();
   P.learned = (cfg.schedule == SPRAY_SCHEDULE_LEARNED) ? 1 : 0;
