@@ -210,8 +210,9 @@ def main():
     for name, img in images.items():
         if ref_name and name != ref_name:
             diff = np.abs(img[..., :3] - images[ref_name][..., :3])
-            results[name]["image_rmse_vs_plain"] = float(np.sqrt((diff ** 2).mean()))
-            results[name]["image_max_vs_plain"] = float(diff.max())
+            results[name]["image_vs"] = ref_name  # plain when it ran, else the first schedule
+            results[name]["image_rmse"] = float(np.sqrt((diff ** 2).mean()))
+            results[name]["image_max"] = float(diff.max())
 
     ok = {k: v for k, v in results.items() if "mrays_s" in v}
     best = max(ok, key=lambda k: ok[k]["mrays_s"]) if ok else None

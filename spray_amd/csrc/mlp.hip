@@ -7,8 +7,8 @@
 // Training data are (origin, direction, domain, label) samples logged by
 // the tracer.  Inference orders speculative domain visits: per radiance ray,
 // the domains entered no later than the most likely hit domain run in phase
-// A of a pass; the rest are deferred to phase B, where the tdom <= history
-// filter prunes most of them.  Exact traversal stays the judge, so the image
+// A of a pass; the rest are deferred to the end of the pass's launch, where
+// the tdom <= history filter prunes most of them.  Exact traversal stays the judge, so the image
 // does not depend on the network.
 #include <hip/hip_runtime.h>
 

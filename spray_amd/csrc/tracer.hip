@@ -815,15 +815,21 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   DevCounters* c = P.ctr;
   if (!c->active) return;
   const int nd = P.ndomains;
-  // entry space of this launch: radiance entries [0, nr) then shadow [nr, nr + ns)
+  // entry space of this launch: radiance entries [0, nr), shadow entries
+  // [nr, nr + ns), then (learned form) deferred radiance entries [nr + ns, total)
   //   plain_dom >= 0: one domain's queues (plain schedule)
   //   plain_dom == -1: every queue of the pass (batched); with the learned
-  //     form, phase 0 takes the phase-A radiance list, phase 1 the deferred list
+  //     form the phase-A list first and the deferred list last, so that the
+  //     deferred entries meet the histories phase A has updated.  When phase
+  //     A and the shadow queues hold fewer than 4 entries per lane of the
+  //     grid, every wave would start on deferred entries at once: then phase
+  //     0 stops after the shadow queues and phase 1 (a second launch) takes
+  //     the deferred list; otherwise phase 1 has nothing to do
   //   plain_dom == -2: shadow rays spawned in this pass, in their spawning
   //     domain (procShads2, ooc_tcontext.inl:51-70)
   //   plain_dom == -3: the entry ranges P.rng of both queues (the chunks of
   //     consecutive domains a paged pool traces while they are resident)
-  uint32_t nr = 0, ns = 0, rbase = 0, sbase = 0, local_base = 0;
+  uint32_t nr = 0, ns = 0, nb = 0, rbase = 0, sbase = 0, local_base = 0;
   const uint32_t* rlist = nullptr;
   uint32_t* wc;
   const bool local = plain_dom == -2;
@@ -849,12 +855,20 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     ns = c->q_total[1];
     wc = &P.wctr[nd];
   } else {
-    // learned: phase 0 takes the phase-A radiance list and the shadow
-    // queues; phase 1 (launched after it) the deferred list, filtered
-    // against the histories phase A updated
-    nr = min(phase == 0 ? c->n_phase[0] : c->n_phase[1], P.q_cap[0]);
-    ns = phase == 0 ? c->q_total[1] : 0u;
-    rlist = phase == 0 ? P.qa : P.qb;
+    // learned: the phase-A radiance list, the shadow queues, then the
+    // deferred list, filtered against the histories phase A updated
+    const uint32_t na = min(c->n_phase[0], P.q_cap[0]), nd2 = min(c->n_phase[1], P.q_cap[0]);
+    const bool split = (unsigned long long)na + c->q_total[1] <
+                       4ull * gridDim.x * blockDim.x;
+    if (phase == 0) {
+      nr = na;
+      ns = c->q_total[1];
+      nb = split ? 0u : nd2;
+      rlist = P.qa;
+    } else {
+      nr = split ? nd2 : 0u;
+      rlist = P.qb;
+    }
     wc = &P.wctr[nd + phase];
   }
   // bounded by the queue capacities even after an overflow
@@ -862,7 +876,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     nr = rbase < P.q_cap[0] ? min(nr, P.q_cap[0] - rbase) : 0u;
     ns = sbase < P.q_cap[1] ? min(ns, P.q_cap[1] - sbase) : 0u;
   }
-  const uint32_t total = nr + ns;
+  const uint32_t total = nr + ns + nb;
   LaneStack st;
   st.init(ring, P.ovf);
   TravCount tc;
@@ -899,12 +913,14 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
         uint32_t e = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         if (e < limit) {
           fetched = true;
-          kind = e < nr ? 0 : 1;
+          kind = (e < nr || e >= nr + ns) ? 0 : 1;
           QEntry qe;
           if (local) {
             qe = QEntry{local_base + e, 0.f};
           } else {
-            uint32_t q = kind == 1 ? sbase + (e - nr) : (rlist ? rlist[e] : rbase + e);
+            const uint32_t q = kind == 1 ? sbase + (e - nr)
+                             : e >= nr + ns ? P.qb[e - nr - ns]
+                                            : (rlist ? rlist[e] : rbase + e);
             qe = P.q[kind][q];
             dom = plain_dom >= 0 ? plain_dom : (int)P.qdom[kind][q];
           }
@@ -1623,9 +1639,6 @@ static bool passPlain(Ctx* c, const TraceParams& P) {
   return true;
 }
 
-// One pass of the batched schedule: every queue of the pass in one launch
-// (two with the learned form: deferred entries run after the others);
-// sizes stay on the device so the pass can be captured in a graph.
 // Batched pass over a paged pool.  The queues are laid out in schedule
 // order, so consecutive scheduled domains own contiguous entry ranges: the
 // pass traces them in chunks of half the cache, one persistent launch per
@@ -1701,6 +1714,10 @@ static bool passPagedBatched(Ctx* c, const TraceParams& P) {
   return true;
 }
 
+// One pass of the batched schedule: every queue of the pass in one launch
+// (with the learned form the deferred entries come last in it, or, in small
+// passes, in a second launch); sizes stay on the device so the pass can be
+// captured in a graph.
 static void passBatched(Ctx* c, const TraceParams& P) {
   TracerState* T = c->tracer;
   hipStream_t s = c->stream;
@@ -1712,7 +1729,7 @@ static void passBatched(Ctx* c, const TraceParams& P) {
   int ei = c->profile ? T->evBegin(s) : -1;
   kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -1, 0);
   if (ei >= 0) T->evEnd(ei, s);
-  if (P.learned) {
+  if (P.learned) {  // the deferred list, when phase 0 left it (small passes)
     ei = c->profile ? T->evBegin(s) : -1;
     kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -1, 1);
     if (ei >= 0) T->evEnd(ei, s);

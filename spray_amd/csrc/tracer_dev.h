@@ -102,7 +102,7 @@ struct TraceParams {
   uint32_t sh_cap;
   // learned form
   uint32_t* qa;       // learned form: radiance queue indices of phase A
-  uint32_t* qb;       //   and of the deferred phase B
+  uint32_t* qb;       //   and of the deferred entries (traced last)
   int learned;
   // predictor logging: visit log of the pass (ray, domain, t) and samples
   int log_on;
@@ -115,7 +115,7 @@ struct TraceParams {
 
 struct Ctx;
 // mlp.hip: assigns every radiance queue entry of the pass to phase A or the
-// deferred phase B (learned form)
+// deferred list (learned form)
 void predictorPhase(Ctx* c, const TraceParams& P);
 bool predictorReady(Ctx* c);
 void predictorPrepare(Ctx* c, size_t r_cap);
