@@ -125,7 +125,8 @@ int spray_num_domains(const spray_ctx* ctx);
 /* Domain record: counts and world AABB (render/domain.h:32-44). */
 int spray_domain_info(const spray_ctx* ctx, int id, long long* nverts, long long* nfaces,
                       float lo[3], float hi[3]);
-/* Copies a domain's vertex normals (3 floats per vertex) computed on the GPU. */
+/* Copies a domain's vertex normals (3 floats per vertex, as many vertices
+ * as spray_domain_mesh reports) computed on the GPU. */
 int spray_domain_normals(spray_ctx* ctx, int id, float* out);
 
 /* Camera::init (render/camera.h:128-166). */
@@ -226,6 +227,43 @@ int spray_mlp_logits(spray_ctx* ctx, const float* samples, int n, const float* p
  * (dir/name.spray); extra_lines (e.g. light lines) go to the header. */
 int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const char* name,
                          const char* extra_lines, long long* triangles_out);
+
+/* ---- on-disk formats and remeshing on the GPU (new) ----
+ * spray_load_scene decodes binary little-endian PLY bodies on the GPU (the
+ * file bytes go to HBM unparsed; io/ply_loader.cc:179-324 semantics and
+ * errors); ASCII PLY stays on the host parser. */
+
+/* The scene spray_scenegen_write + spray_load_scene would give (the same
+ * meshes, domain bounds and .spray description, domain files named
+ * name_dNNNN.ply), generated by marching-cubes kernels straight into the
+ * HBM domain pool with no files.  Stands in for the reference's scene input
+ * path (Scene::init, render/scene.inl:30-100) for in-situ data. */
+int spray_scenegen_load(spray_ctx* ctx, const spray_scenegen_params* p, const char* name,
+                        const char* extra_lines);
+/* Remeshes a scene made by spray_scenegen_load at a new field phase (a
+ * time-varying field remeshed every frame): meshes, domain bounds, normals
+ * and BVHs are rebuilt on the GPU; camera, schedule, batch parameters and
+ * predictor stay. */
+int spray_remesh(spray_ctx* ctx, double phase);
+/* Domain id's mesh as held in HBM (TriMeshBuffer's vertex, face and colour
+ * buffers, render/trimesh_buffer.cc:52-110).  *nverts / *nfaces (when not
+ * NULL) receive the loaded counts (the scene file's vertex/face lines are
+ * not checked against the PLY); with v, f and rgb all NULL nothing else
+ * happens, else v gets nverts*3 floats, f nfaces*3 indices and rgb nverts
+ * packed colours (r<<16|g<<8|b), each pointer may be NULL.  Copying needs
+ * the whole scene resident (no partial domain cache). */
+int spray_domain_mesh(spray_ctx* ctx, int id, float* v, unsigned int* f, unsigned int* rgb,
+                      long long* nverts, long long* nfaces);
+/* Writes the loaded scene (remeshed frames included) as binary PLY files
+ * dir/name_dNNNN.ply plus dir/name.spray (lights, materials, world bounds,
+ * counts; vertices in world space), readable by spray_load_scene and the
+ * reference's loaders (io/scene_loader.cc:315-358, io/ply_loader.cc). */
+int spray_write_scene(spray_ctx* ctx, const char* dir, const char* name);
+/* ply_header_reader (apps/ply_header_reader.cc:36-108): vertex and face
+ * counts of a PLY file and the bounds of its vertices (binary bodies decoded
+ * and reduced on the GPU). */
+int spray_ply_info(spray_ctx* ctx, const char* path, long long* nverts, long long* nfaces,
+                   float lo[3], float hi[3]);
 
 /* HdrImage::writePpm (display/image.h:183-204): writes a W*H*4 rgba float
  * image (as returned by spray_trace_frame) as an ASCII P3 file with maximum

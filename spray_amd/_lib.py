@@ -119,6 +119,12 @@ PROTOTYPES = {
     "spray_mlp_logits": (C.c_int, [_P, _FP, C.c_int, _FP, _FP]),
     "spray_write_ppm": (C.c_int, [_FP, C.c_int, C.c_int, C.c_char_p]),
     "spray_scenegen_write": (C.c_int, [C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_longlong)]),
+    "spray_scenegen_load": (C.c_int, [_P, C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p]),
+    "spray_remesh": (C.c_int, [_P, C.c_double]),
+    "spray_domain_mesh": (C.c_int, [_P, C.c_int, _FP, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                    C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "spray_write_scene": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
+    "spray_ply_info": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _FP, _FP]),
 }
 
 _lib = None

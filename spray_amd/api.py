@@ -39,19 +39,34 @@ def default_config(**kw):
     return cfg
 
 
+def _gen_params(nx, ny, nz, res, a=6.0, b=17.0, phase=0.0, rgb=0xFFFFFF, per_domain_random=False,
+                seed=0, tri_total=0.0):
+    return L.spray_scenegen_params(nx=nx, ny=ny, nz=nz, res=res, a=a, b=b, phase=phase,
+                                   per_domain_random=int(per_domain_random), seed=seed,
+                                   tri_total=tri_total, rgb=rgb)
+
+
 def scenegen_write(directory, name, nx, ny, nz, res, a=6.0, b=17.0, phase=0.0, rgb=0xFFFFFF,
                    per_domain_random=False, seed=0, tri_total=0.0, extra_lines=""):
     """Writes a marching-cubes scene (name.spray + PLYs) and returns the triangle count."""
     lib = L.load()
-    p = L.spray_scenegen_params(nx=nx, ny=ny, nz=nz, res=res, a=a, b=b, phase=phase,
-                                per_domain_random=int(per_domain_random), seed=seed,
-                                tri_total=tri_total, rgb=rgb)
+    p = _gen_params(nx, ny, nz, res, a, b, phase, rgb, per_domain_random, seed, tri_total)
     tris = C.c_longlong(0)
     rc = lib.spray_scenegen_write(C.byref(p), str(directory).encode(), name.encode(),
                                   extra_lines.encode(), C.byref(tris))
     if rc != L.SPRAY_OK:
         raise SprayError(rc, lib.spray_last_error(None).decode())
     return tris.value
+
+
+def write_ply_info(infile, outfile, info):
+    """The ply_header_reader's output file (apps/ply_header_reader.cc:90-104):
+    file, vertex and face lines and the bound line with 16 significant
+    digits (std::setprecision(digits10 + 1) on the float bounds)."""
+    nv, nf, lo, hi = info
+    b = " ".join("%.16g" % float(x) for x in (lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]))
+    with open(outfile, "w") as fp:
+        fp.write(f"file {infile}\nvertex {nv}\nface {nf}\nbound {b}\n")
 
 
 def write_ppm(path, rgba):
@@ -101,6 +116,37 @@ class Context:
     def load_scene(self, spray_file, ply_path=""):
         self._check(self.lib.spray_load_scene(self.h, str(spray_file).encode(), str(ply_path).encode()))
 
+    def scenegen_load(self, name, nx, ny, nz, res, a=6.0, b=17.0, phase=0.0, rgb=0xFFFFFF,
+                      per_domain_random=False, seed=0, tri_total=0.0, extra_lines=""):
+        """Generates the marching-cubes scene on the GPU straight into the domain pool."""
+        p = _gen_params(nx, ny, nz, res, a, b, phase, rgb, per_domain_random, seed, tri_total)
+        self._check(self.lib.spray_scenegen_load(self.h, C.byref(p), name.encode(), extra_lines.encode()))
+
+    def remesh(self, phase):
+        self._check(self.lib.spray_remesh(self.h, float(phase)))
+
+    def domain_mesh(self, i):
+        """(vertices (nv, 3) float32, faces (nf, 3) uint32, colours (nv,) uint32) of domain i."""
+        nv, nf = self.mesh_size(i)
+        v = np.zeros((nv, 3), np.float32)
+        f = np.zeros((nf, 3), np.uint32)
+        rgb = np.zeros(nv, np.uint32)
+        up = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint))
+        self._check(self.lib.spray_domain_mesh(self.h, i, _fp(v), up(f), up(rgb), None, None))
+        return v, f, rgb
+
+    def write_scene(self, directory, name):
+        self._check(self.lib.spray_write_scene(self.h, str(directory).encode(), name.encode()))
+
+    def ply_info(self, path):
+        """(vertex count, face count, lo[3], hi[3]) of a PLY file."""
+        nv, nf = C.c_longlong(), C.c_longlong()
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        self._check(self.lib.spray_ply_info(self.h, str(path).encode(), C.byref(nv), C.byref(nf),
+                                            _fp(lo), _fp(hi)))
+        return nv.value, nf.value, lo, hi
+
     @property
     def num_domains(self):
         return self.lib.spray_num_domains(self.h)
@@ -112,8 +158,14 @@ class Context:
         self._check(self.lib.spray_domain_info(self.h, i, C.byref(nv), C.byref(nf), _fp(lo), _fp(hi)))
         return nv.value, nf.value, lo, hi
 
+    def mesh_size(self, i):
+        """(vertices, faces) of domain i as loaded."""
+        nv, nf = C.c_longlong(), C.c_longlong()
+        self._check(self.lib.spray_domain_mesh(self.h, i, None, None, None, C.byref(nv), C.byref(nf)))
+        return nv.value, nf.value
+
     def normals(self, i):
-        nv = self.domain_info(i)[0]
+        nv = self.mesh_size(i)[0]
         out = np.zeros((nv, 3), np.float32)
         self._check(self.lib.spray_domain_normals(self.h, i, _fp(out)))
         return out

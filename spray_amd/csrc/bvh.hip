@@ -599,24 +599,37 @@ static int gridFor(long long n, int block) {
   return (int)g;
 }
 
-void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
+void HostMeshSource::fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) {
+  (void)c;
+  SPRAY_HIP(hipMemcpyAsync(v, m[d].v.data(), m[d].v.size() * 4, hipMemcpyHostToDevice, s));
+  SPRAY_HIP(hipMemcpyAsync(f, m[d].f.data(), m[d].f.size() * 4, hipMemcpyHostToDevice, s));
+  SPRAY_HIP(hipMemcpyAsync(rgb, m[d].rgb.data(), m[d].rgb.size() * 4, hipMemcpyHostToDevice, s));
+}
+
+void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
   auto t_start = std::chrono::steady_clock::now();
   DomainPool& P = c->pool;
-  P.release();
   hipStream_t s = c->stream;
-  int nd = (int)meshes.size();
-  P.ndomains = nd;
-  P.desc.resize(nd);
+  int nd = (int)c->scene.domains.size();
   // a cache smaller than the scene pages domains (raw meshes are staged one
   // domain at a time, images kept in pinned host memory)
   const bool paged = c->cache_size > 0 && c->cache_size < nd;
+  if (keep_buffers && !paged && !P.paged) {
+    P.desc.clear();  // device buffers stay; they grow when too small
+  } else {
+    P.release();
+  }
+  P.ndomains = nd;
+  P.desc.resize(nd);
+  P.nverts.resize(nd);
   long long tv = 0, tf = 0, tn = 0;
   int maxf = 0;
   long long maxv = 0, max_nodes = 0;
   for (int d = 0; d < nd; ++d) {
     const HostDomain& hd = c->scene.domains[d];
     DomainDesc& D = P.desc[d];
-    long long nv = (long long)meshes[d].v.size() / 3, nf = (long long)meshes[d].f.size() / 3;
+    long long nv = src.numVerts(d), nf = src.numFaces(d);
+    P.nverts[d] = nv;
     if (nf >= (1 << 24)) throw StatusError(SPRAY_ERR_CAPACITY, "domain has more than 2^24 faces");
     D.vtx_off = paged ? 0 : tv;
     D.face_off = paged ? 0 : tf;
@@ -672,14 +685,13 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
     P.fshade.reserve(tf * 48 + 64);
     P.nodes.reserve(tn * sizeof(BNode4));
   }
-  for (int d = 0; d < nd && !paged; ++d) {
-    const DomainDesc& D = P.desc[d];
-    SPRAY_HIP(hipMemcpyAsync(P.vtx.as<float>() + D.vtx_off * 3, meshes[d].v.data(),
-                             meshes[d].v.size() * 4, hipMemcpyHostToDevice, s));
-    SPRAY_HIP(hipMemcpyAsync(P.face.as<uint32_t>() + D.face_off * 3, meshes[d].f.data(),
-                             meshes[d].f.size() * 4, hipMemcpyHostToDevice, s));
-    SPRAY_HIP(hipMemcpyAsync(P.rgb.as<uint32_t>() + D.vtx_off, meshes[d].rgb.data(),
-                             meshes[d].rgb.size() * 4, hipMemcpyHostToDevice, s));
+  if (!paged) {
+    for (int d = 0; d < nd; ++d) {
+      const DomainDesc& D = P.desc[d];
+      src.fill(c, d, P.vtx.as<float>() + D.vtx_off * 3, P.face.as<uint32_t>() + D.face_off * 3,
+               P.rgb.as<uint32_t>() + D.vtx_off, s);
+    }
+    src.check(c);
   }
   // build scratch sized for the largest domain
   int nblk = (maxf + kRsTile - 1) / kRsTile;
@@ -716,17 +728,13 @@ void buildPool(Ctx* c, std::vector<HostMesh>& meshes) {
   for (int d = 0; d < nd; ++d) {
     DomainDesc& D = P.desc[d];
     const int nf = D.num_tris;
-    const long long nv = (long long)meshes[d].v.size() / 3;
+    const long long nv = src.numVerts(d);
     if (paged) {  // build into block d % capacity from the staged mesh
       const int b = d % P.capacity;
       D.node_off = (long long)b * P.block_nodes;
       D.tri_off = D.shade_off = (long long)b * P.block_tris;
-      SPRAY_HIP(hipMemcpyAsync(P.vtx.p, meshes[d].v.data(), meshes[d].v.size() * 4,
-                               hipMemcpyHostToDevice, s));
-      SPRAY_HIP(hipMemcpyAsync(P.face.p, meshes[d].f.data(), meshes[d].f.size() * 4,
-                               hipMemcpyHostToDevice, s));
-      SPRAY_HIP(hipMemcpyAsync(P.rgb.p, meshes[d].rgb.data(), meshes[d].rgb.size() * 4,
-                               hipMemcpyHostToDevice, s));
+      src.fill(c, d, P.vtx.as<float>(), P.face.as<uint32_t>(), P.rgb.as<uint32_t>(), s);
+      src.check(c);
     }
     const float* vtx = P.vtx.as<float>() + D.vtx_off * 3;
     const uint32_t* face = P.face.as<uint32_t>() + D.face_off * 3;

@@ -193,35 +193,67 @@ int spray_load_scene(spray_ctx* ctx, const char* spray_file, const char* ply_pat
     std::string err;
     int rc = parseSceneFile(spray_file, ply_path ? ply_path : "", &scene, &err);
     if (rc != SPRAY_OK) throw StatusError(rc, err);
-    int nd = (int)scene.domains.size();
-    std::vector<HostMesh> meshes(nd);
-    std::vector<int> codes(nd, SPRAY_OK);
-    std::vector<std::string> errs(nd);
-    int nthreads = std::max(1, c->cfg.nthreads);
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-    for (int d = 0; d < nd; ++d) {
-      HostDomain& hd = scene.domains[d];
-      codes[d] = readPly(hd.filename, &meshes[d], &errs[d]);
-      if (codes[d] != SPRAY_OK) continue;
-      // trimesh_buffer.cc:139-156: transform vertices when not identity
-      if (!transformIsIdentity(hd.xf)) {
-        size_t nv = meshes[d].v.size() / 3;
-        for (size_t i = 0; i < nv; ++i) {
-          float p[3] = {meshes[d].v[i * 3], meshes[d].v[i * 3 + 1], meshes[d].v[i * 3 + 2]}, o[3];
-          transformPoint(hd.xf, p, o);
-          std::memcpy(&meshes[d].v[i * 3], o, 12);
-        }
-      }
-    }
-    for (int d = 0; d < nd; ++d)
-      if (codes[d] != SPRAY_OK) throw StatusError(codes[d], errs[d]);
-    if (c->cfg.ao_mode == 0 && scene.lights.empty())
-      throw StatusError(SPRAY_ERR_INVALID, "path tracing needs at least one light");
-    c->scene = std::move(scene);
-    tracerRelease(c);
-    buildPool(c, meshes);
+    loadScenePly(c, scene);
     c->loaded = true;
     defaultCamera(c);
+  });
+}
+
+int spray_scenegen_load(spray_ctx* ctx, const spray_scenegen_params* p, const char* name,
+                        const char* extra_lines) {
+  if (!ctx || !p || !name) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    SPRAY_HIP(hipSetDevice(c->device));
+    c->loaded = false;
+    c->generated = false;
+    generateScene(c, *p, name, extra_lines ? extra_lines : "", false);
+    c->loaded = true;
+    defaultCamera(c);
+  });
+}
+
+int spray_remesh(spray_ctx* ctx, double phase) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->loaded || !c->generated)
+      throw StatusError(SPRAY_ERR_STATE, "remeshing needs a scene made by spray_scenegen_load");
+    SPRAY_HIP(hipSetDevice(c->device));
+    spray_scenegen_params p = c->gen;
+    p.phase = phase;
+    c->loaded = false;
+    generateScene(c, p, c->gen_name, c->gen_extra, true);
+    c->loaded = true;
+  });
+}
+
+int spray_domain_mesh(spray_ctx* ctx, int id, float* v, unsigned int* f, unsigned int* rgb,
+                      long long* nverts, long long* nfaces) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    needScene(&ctx->c);
+    SPRAY_HIP(hipSetDevice(ctx->c.device));
+    if (id < 0 || id >= ctx->c.pool.ndomains) throw StatusError(SPRAY_ERR_INVALID, "bad domain id");
+    domainMesh(&ctx->c, id, v, f, rgb, nverts, nfaces);
+  });
+}
+
+int spray_write_scene(spray_ctx* ctx, const char* dir, const char* name) {
+  if (!ctx || !dir || !name) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    needScene(&ctx->c);
+    SPRAY_HIP(hipSetDevice(ctx->c.device));
+    writeScene(&ctx->c, dir, name);
+  });
+}
+
+int spray_ply_info(spray_ctx* ctx, const char* path, long long* nverts, long long* nfaces,
+                   float lo[3], float hi[3]) {
+  if (!ctx || !path || !nverts || !nfaces || !lo || !hi) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    SPRAY_HIP(hipSetDevice(ctx->c.device));
+    plyInfo(&ctx->c, path, nverts, nfaces, lo, hi);
   });
 }
 
@@ -254,8 +286,7 @@ int spray_domain_normals(spray_ctx* ctx, int id, float* out) {
       std::copy(P.h_nrm[id].begin(), P.h_nrm[id].end(), out);
       return;
     }
-    long long nv = (id + 1 < P.ndomains ? P.desc[id + 1].vtx_off : P.total_verts) - P.desc[id].vtx_off;
-    SPRAY_HIP(hipMemcpy(out, P.nrm.as<float>() + P.desc[id].vtx_off * 3, (size_t)nv * 12,
+    SPRAY_HIP(hipMemcpy(out, P.nrm.as<float>() + P.desc[id].vtx_off * 3, (size_t)P.nverts[id] * 12,
                         hipMemcpyDeviceToHost));
   });
 }

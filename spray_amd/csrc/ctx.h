@@ -63,6 +63,7 @@ struct DomainPool {
   int ndomains = 0;
   long long total_tris = 0, total_verts = 0, total_nodes = 0;
   std::vector<DomainDesc> desc;  // host copy
+  std::vector<long long> nverts; // vertices per domain as loaded
   DevBuf d_desc;                 // DomainDesc[ndomains]
   DevBuf vtx, face, rgb, nrm;    // float3 / uint3 / uint32 / float3
   DevBuf nodes;                  // BNode4
@@ -104,6 +105,28 @@ struct DomainPool {
   }
 };
 
+// Where buildPool takes each domain's raw mesh from: host meshes, PLY
+// bodies decoded on the GPU, or the GPU marching-cubes generator.
+struct Ctx;
+struct MeshSource {
+  virtual ~MeshSource() = default;
+  virtual long long numVerts(int d) const = 0;
+  virtual long long numFaces(int d) const = 0;
+  // Issues, on stream s, the writes of domain d's vertices (3 floats each),
+  // faces (3 indices each, every index < numVerts(d)) and packed colours.
+  virtual void fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) = 0;
+  // After the fills issued so far: throws if one of them found bad input.
+  virtual void check(Ctx* c) { (void)c; }
+};
+
+struct HostMeshSource : MeshSource {
+  std::vector<HostMesh>& m;
+  explicit HostMeshSource(std::vector<HostMesh>& meshes) : m(meshes) {}
+  long long numVerts(int d) const override { return (long long)m[d].v.size() / 3; }
+  long long numFaces(int d) const override { return (long long)m[d].f.size() / 3; }
+  void fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) override;
+};
+
 struct TracerState;  // tracer.hip
 struct Predictor;    // mlp.hip
 
@@ -133,10 +156,25 @@ struct Ctx {
   std::string err;
   // scratch for spray_trace_rays / spray_domain_list
   DevBuf q_in, q_out;
+  // a scene made by the GPU generator (spray_scenegen_load) can be remeshed
+  bool generated = false;
+  spray_scenegen_params gen{};
+  std::string gen_name, gen_extra;
 };
 
-// bvh.hip
-void buildPool(Ctx* c, std::vector<HostMesh>& meshes);
+// bvh.hip: builds the pool from `src` (the domains of c->scene); with
+// keep_buffers the pool's device buffers are reused where large enough
+void buildPool(Ctx* c, MeshSource& src, bool keep_buffers = false);
+// mesh_gpu.hip: scene loading from PLY files (binary bodies decoded on the
+// GPU), the GPU marching-cubes generator, mesh readback and the scene writer
+void loadScenePly(Ctx* c, HostScene& scene);
+void generateScene(Ctx* c, const spray_scenegen_params& p, const std::string& name,
+                   const std::string& extra, bool remesh);
+void domainMesh(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, long long* nv_out,
+                long long* nf_out);
+void writeScene(Ctx* c, const std::string& dir, const std::string& name);
+void plyInfo(Ctx* c, const std::string& path, long long* nv, long long* nf, float lo[3],
+             float hi[3]);
 // paged pool (pool_cache.cc): make domain d resident and have stream s wait
 // for it; start loading d ahead (never evicting `keep`); record that the
 // kernels on s that use d's block have been issued; cache state after a build

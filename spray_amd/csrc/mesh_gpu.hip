@@ -1,0 +1,801 @@
+// Scene input and output on the GPU: binary PLY bodies decoded in HBM, the
+// marching-cubes scene generator as kernels (a generated scene can be
+// remeshed in place every frame), mesh readback, the scene writer and the
+// PLY info query.
+//
+// Replaces the reference's PlyLoader::load (io/ply_loader.cc:179-324) as
+// called by TriMeshBuffer::load (render/trimesh_buffer.cc:117-169), and the
+// ply_header_reader app (apps/ply_header_reader.cc:36-108: a PLY file's
+// vertex and face counts and the bounds of its vertices).
+//
+// Binary little-endian bodies are read from the file straight into pinned
+// memory, copied to HBM as they are and decoded there: each block stages
+// 256 records with aligned dword loads in LDS and every lane takes one
+// record out of LDS (vertex: float x y z [uchar r g b]; face: list count +
+// 3 indices of 1, 2 or 4 bytes, unsigned little-endian as the host parser
+// reads them).  ASCII files stay on the host parser.  Errors are the host
+// parser's: a face whose count is not 3 or whose index is out of range
+// (the first such face of the file decides the message).
+//
+// The generator reproduces scenegen.cc's CPU meshes in the same vertex and
+// face order: field values in double at the grid points, one vertex per
+// sign-changing grid edge numbered by an exclusive scan over the edges in
+// (axis, k, j, i) order, then the case table's triangles per cell numbered
+// by a scan over the cells in (k, j, i) order.  Every double operation is
+// the CPU's, in the CPU's order (no contraction), so meshes agree
+// bit-for-bit wherever the device and host sin agree in the last place.
+#include <hip/hip_runtime.h>
+
+#include <fcntl.h>
+#include <omp.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "ctx.h"
+#include "kernels_common.h"
+
+namespace spray {
+
+namespace {
+
+int gridOf(long long n, int block, long long cap = 65535) {
+  long long g = (n + block - 1) / block;
+  return (int)std::max<long long>(1, std::min<long long>(g, cap));
+}
+
+// ---------------------------------------------------------------------------
+// Bounds: per-block min/max of float points, then one block over the partials
+constexpr int kBlk = 256;
+constexpr int kBoundBlocks = 1024;
+
+__device__ void blockBounds(float lo[3], float hi[3], float* part) {
+  __shared__ float red[6][kBlk / 64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int c = 0; c < 3; ++c) {
+    float a = waveMin(lo[c]), b = waveMax(hi[c]);
+    if (l == 0) {
+      red[c][w] = a;
+      red[3 + c][w] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    float v = red[threadIdx.x][0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+      v = threadIdx.x < 3 ? fminf(v, red[threadIdx.x][i]) : fmaxf(v, red[threadIdx.x][i]);
+    part[blockIdx.x * 6 + threadIdx.x] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlk) kBoundsFinal(const float* part, int nparts, float* out) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fminf(lo[c], part[i * 6 + c]);
+      hi[c] = fmaxf(hi[c], part[i * 6 + 3 + c]);
+    }
+  blockBounds(lo, hi, out);  // a single block: writes out[0..5]
+}
+
+__global__ void __launch_bounds__(kBlk) kPointBounds(const float* v, long long nv, float* part) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nv;
+       i += (long long)gridDim.x * blockDim.x)
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fminf(lo[c], v[i * 3 + c]);
+      hi[c] = fmaxf(hi[c], v[i * 3 + c]);
+    }
+  blockBounds(lo, hi, part);
+}
+
+// ---------------------------------------------------------------------------
+// PLY body decode
+constexpr int kMaxStride = 16;  // face: 4-byte count + 3 four-byte indices
+
+struct Xform {
+  float m[16];
+};
+
+// Stages records [r0, r0 + blockDim.x) of `stride` bytes in LDS; the body
+// buffer is dword aligned and padded past its end.  Returns the first
+// record's address in LDS.
+__device__ __forceinline__ const uint8_t* stageRecords(const uint8_t* body, long long r0,
+                                                       long long n, int stride, uint32_t* lds) {
+  const long long b0 = r0 * stride, b1 = min(n, r0 + (long long)blockDim.x) * stride;
+  const long long w0 = b0 >> 2, w1 = (b1 + 3) >> 2;
+  const uint32_t* words = (const uint32_t*)body;
+  for (long long w = w0 + threadIdx.x; w < w1; w += blockDim.x) lds[w - w0] = words[w];
+  __syncthreads();
+  return (const uint8_t*)lds + (b0 - (w0 << 2));
+}
+
+__device__ __forceinline__ uint32_t leBytes(const uint8_t* p, int n) {
+  uint32_t v = 0;
+  for (int i = 0; i < n; ++i) v |= (uint32_t)p[i] << (8 * i);
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlk) kPlyVerts(const uint8_t* __restrict__ body, long long nv,
+                                                  int stride, int has_rgb, Xform xf, int apply,
+                                                  float* __restrict__ v,
+                                                  uint32_t* __restrict__ rgb) {
+  __shared__ uint32_t lds[(kBlk * kMaxStride + 8) / 4];
+  for (long long r0 = blockIdx.x * (long long)kBlk; r0 < nv; r0 += (long long)gridDim.x * kBlk) {
+    const uint8_t* rec = stageRecords(body, r0, nv, stride, lds);
+    const long long i = r0 + threadIdx.x;
+    if (i < nv) {
+      const uint8_t* p = rec + threadIdx.x * stride;
+      float q[3];
+      for (int k = 0; k < 3; ++k) q[k] = __uint_as_float(leBytes(p + 4 * k, 4));
+      if (apply) {  // trimesh_buffer.cc:139-156, glm mat4 * vec4(p, 1)
+        float o[3];
+        for (int r = 0; r < 3; ++r) {
+          float a = xf.m[r] * q[0] + xf.m[4 + r] * q[1];
+          float b = xf.m[8 + r] * q[2] + xf.m[12 + r] * 1.0f;
+          o[r] = a + b;
+        }
+        for (int k = 0; k < 3; ++k) q[k] = o[k];
+      }
+      for (int k = 0; k < 3; ++k) v[i * 3 + k] = q[k];
+      rgb[i] = has_rgb ? ((uint32_t)p[12] << 16) | ((uint32_t)p[13] << 8) | p[14] : 0u;
+    }
+    __syncthreads();  // the next stage overwrites lds
+  }
+}
+
+// err: min over bad faces of (face << 1 | kind), kind 0 = not a triangle,
+// 1 = index out of range.  Bad faces get index 0 so later kernels stay in
+// bounds.
+__global__ void __launch_bounds__(kBlk) kPlyFaces(const uint8_t* __restrict__ body, long long nf,
+                                                  int lb, int ib, long long nv,
+                                                  uint32_t* __restrict__ f,
+                                                  unsigned long long* err) {
+  __shared__ uint32_t lds[(kBlk * kMaxStride + 8) / 4];
+  const int stride = lb + 3 * ib;
+  for (long long r0 = blockIdx.x * (long long)kBlk; r0 < nf; r0 += (long long)gridDim.x * kBlk) {
+    const uint8_t* rec = stageRecords(body, r0, nf, stride, lds);
+    const long long i = r0 + threadIdx.x;
+    if (i < nf) {
+      const uint8_t* p = rec + threadIdx.x * stride;
+      const uint32_t cnt = leBytes(p, lb);
+      uint32_t idx[3];
+      for (int k = 0; k < 3; ++k) idx[k] = leBytes(p + lb + k * ib, ib);
+      int bad = -1;
+      if (cnt != 3)
+        bad = 0;
+      else if (idx[0] >= nv || idx[1] >= nv || idx[2] >= nv)
+        bad = 1;
+      for (int k = 0; k < 3; ++k) f[i * 3 + k] = bad < 0 ? idx[k] : 0u;
+      if (bad >= 0) atomicMin(err, ((unsigned long long)i << 1) | (unsigned long long)bad);
+    }
+    __syncthreads();
+  }
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Reads n bytes at `off` of `path` into dst with parallel preads.
+bool readRange(const std::string& path, long long off, size_t n, char* dst, int nthreads) {
+  int fd = open(path.c_str(), O_RDONLY);
+  if (fd < 0) return false;
+  const size_t chunk = (size_t)8 << 20;
+  const long long nchunks = (long long)((n + chunk - 1) / chunk);
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (long long k = 0; k < nchunks; ++k) {
+    const size_t o = (size_t)k * chunk, len = std::min(chunk, n - o);
+    size_t got = 0;
+    while (got < len) {
+      ssize_t r = pread(fd, dst + o + got, len - got, (off_t)(off + (long long)(o + got)));
+      if (r <= 0) {
+#pragma omp atomic write
+        failed = 1;
+        break;
+      }
+      got += (size_t)r;
+    }
+  }
+  close(fd);
+  return !failed;
+}
+
+// Host checks of a binary body against the file size (the host parser's
+// "truncated" errors, first element first).
+int checkBody(const std::string& file, const PlyLayout& L, std::string* err) {
+  const long long vb = L.nv * L.vertexStride(), fb = L.nf * L.faceStride();
+  const long long first = L.faces_first ? fb : vb;
+  if (L.file_bytes < L.body + first) {
+    *err = file + (L.faces_first ? ": truncated faces" : ": truncated vertices");
+    return SPRAY_ERR_FORMAT;
+  }
+  if (L.file_bytes < L.body + vb + fb) {
+    *err = file + (L.faces_first ? ": truncated vertices" : ": truncated faces");
+    return SPRAY_ERR_FORMAT;
+  }
+  return SPRAY_OK;
+}
+
+// Pinned double buffer + device staging for PLY bodies.
+struct BodyStager {
+  void* pinned[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  size_t pin_bytes = 0;
+  DevBuf dev;
+  int slot = 0;
+  hipStream_t last = nullptr;
+  explicit BodyStager(size_t max_body) : pin_bytes(max_body) {
+    for (int k = 0; k < 2; ++k) {
+      SPRAY_HIP(hipHostMalloc(&pinned[k], std::max<size_t>(pin_bytes, 64), hipHostMallocDefault));
+      SPRAY_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+    }
+    dev.reserve(max_body + 512 + 64);
+  }
+  ~BodyStager() {
+    if (last) (void)hipStreamSynchronize(last);
+    for (int k = 0; k < 2; ++k) {
+      if (pinned[k]) (void)hipHostFree(pinned[k]);
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+    dev.release();
+  }
+  // Reads the body of `file` and copies its vertex and face element to the
+  // device (each at a 256-byte aligned address); decodes them on s.
+  void decode(const std::string& file, const PlyLayout& L, const float* xf, bool apply,
+              float* v, uint32_t* f, uint32_t* rgb, unsigned long long* err, hipStream_t s,
+              int nthreads) {
+    last = s;
+    const size_t vb = (size_t)L.nv * L.vertexStride(), fb = (size_t)L.nf * L.faceStride();
+    char* host = (char*)pinned[slot];
+    SPRAY_HIP(hipEventSynchronize(ev[slot]));  // the last copy out of this slot is done
+    if (!readRange(file, L.body, vb + fb, host, nthreads))
+      throw StatusError(SPRAY_ERR_IO, "unable to read " + file);
+    char* dv = dev.as<char>();
+    char* df = dv + align256(vb);
+    const char* hv = host + (L.faces_first ? fb : 0);
+    const char* hf = host + (L.faces_first ? 0 : vb);
+    SPRAY_HIP(hipMemcpyAsync(dv, hv, vb, hipMemcpyHostToDevice, s));
+    SPRAY_HIP(hipMemcpyAsync(df, hf, fb, hipMemcpyHostToDevice, s));
+    SPRAY_HIP(hipEventRecord(ev[slot], s));
+    slot ^= 1;
+    Xform X;
+    std::memcpy(X.m, xf, sizeof(X.m));
+    kPlyVerts<<<gridOf(L.nv, kBlk), kBlk, 0, s>>>((const uint8_t*)dv, L.nv, L.vertexStride(),
+                                                   L.rgb ? 1 : 0, X, apply ? 1 : 0, v, rgb);
+    kPlyFaces<<<gridOf(L.nf, kBlk), kBlk, 0, s>>>((const uint8_t*)df, L.nf, L.list_bytes,
+                                                   L.index_bytes, L.nv, f, err);
+    SPRAY_HIP(hipGetLastError());
+  }
+};
+
+std::string faceError(const std::string& file, unsigned long long key) {
+  return file + ((key & 1ull) ? ": face index out of range" : ": non-triangle face");
+}
+
+// Domains from PLY files: binary bodies decoded on the GPU, ASCII files
+// parsed on the host (with the domain transform applied there).
+struct PlySource : MeshSource {
+  const HostScene& scene;
+  std::vector<PlyLayout> L;
+  std::vector<HostMesh> host;
+  std::vector<int> codes;
+  std::vector<std::string> errs;
+  std::vector<char> filled;
+  BodyStager* stager = nullptr;
+  DevBuf errbuf;
+  int nthreads;
+  explicit PlySource(Ctx* c) : scene(c->scene) {
+    const int nd = (int)scene.domains.size();
+    L.resize(nd);
+    host.resize(nd);
+    codes.assign(nd, SPRAY_OK);
+    errs.resize(nd);
+    filled.assign(nd, 0);
+    nthreads = std::max(1, c->cfg.nthreads);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int d = 0; d < nd; ++d) {
+      const HostDomain& hd = scene.domains[d];
+      codes[d] = readPlyHeader(hd.filename, &L[d], &errs[d]);
+      if (codes[d] != SPRAY_OK) continue;
+      if (L[d].binary) {
+        codes[d] = checkBody(hd.filename, L[d], &errs[d]);
+        continue;
+      }
+      codes[d] = readPly(hd.filename, &host[d], &errs[d]);
+      if (codes[d] != SPRAY_OK || transformIsIdentity(hd.xf)) continue;
+      // trimesh_buffer.cc:139-156: transform vertices when not identity
+      size_t nv = host[d].v.size() / 3;
+      for (size_t i = 0; i < nv; ++i) {
+        float p[3] = {host[d].v[i * 3], host[d].v[i * 3 + 1], host[d].v[i * 3 + 2]}, o[3];
+        transformPoint(hd.xf, p, o);
+        std::memcpy(&host[d].v[i * 3], o, 12);
+      }
+    }
+    for (int d = 0; d < nd; ++d)
+      if (codes[d] != SPRAY_OK) throw StatusError(codes[d], errs[d]);
+    size_t max_body = 0;
+    for (int d = 0; d < nd; ++d)
+      if (L[d].binary)
+        max_body = std::max(max_body, (size_t)(L[d].nv * L[d].vertexStride() +
+                                               L[d].nf * L[d].faceStride()));
+    errbuf.reserve((size_t)nd * 8 + 64);
+    SPRAY_HIP(hipMemsetAsync(errbuf.p, 0xff, (size_t)nd * 8, c->stream));
+    if (max_body) stager = new BodyStager(max_body);
+  }
+  ~PlySource() override {
+    delete stager;
+    errbuf.release();
+  }
+  long long numVerts(int d) const override {
+    return L[d].binary ? L[d].nv : (long long)host[d].v.size() / 3;
+  }
+  long long numFaces(int d) const override {
+    return L[d].binary ? L[d].nf : (long long)host[d].f.size() / 3;
+  }
+  void fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) override {
+    (void)c;
+    filled[d] = 1;
+    if (!L[d].binary) {
+      HostMeshSource hs(host);
+      hs.fill(c, d, v, f, rgb, s);
+      return;
+    }
+    const HostDomain& hd = scene.domains[d];
+    stager->decode(hd.filename, L[d], hd.xf, !transformIsIdentity(hd.xf), v, f, rgb,
+                   errbuf.as<unsigned long long>() + d, s, nthreads);
+  }
+  void check(Ctx* c) override {
+    SPRAY_HIP(hipStreamSynchronize(c->stream));
+    const int nd = (int)L.size();
+    std::vector<unsigned long long> e(nd);
+    SPRAY_HIP(hipMemcpy(e.data(), errbuf.p, (size_t)nd * 8, hipMemcpyDeviceToHost));
+    for (int d = 0; d < nd; ++d)
+      if (filled[d] && e[d] != ~0ull)
+        throw StatusError(SPRAY_ERR_FORMAT, faceError(scene.domains[d].filename, e[d]));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Marching cubes (scenegen.cc's mcGenerate on the GPU)
+struct McGrid {
+  int n, N;              // cells and grid points per axis
+  long long E;           // grid edges per axis: n * N * N
+  double ox, oy, oz, h;  // domain origin and cell size
+  double a, b, phase;
+};
+
+struct McCorners {  // per local edge: corner offset and axis
+  int dx[12], dy[12], dz[12], axis[12];
+};
+
+__device__ __forceinline__ double mcF(const double* f, const McGrid& g, int i, int j, int k) {
+  return f[((long long)k * g.N + j) * g.N + i];
+}
+
+// position of edge (axis, i, j, k) in the axis' edge numbering
+__device__ __forceinline__ long long mcEdgeIndex(const McGrid& g, int axis, int i, int j, int k) {
+  if (axis == 0) return ((long long)k * g.N + j) * g.n + i;
+  if (axis == 1) return ((long long)k * g.n + j) * g.N + i;
+  return ((long long)k * g.N + j) * g.N + i;
+}
+
+__device__ __forceinline__ void mcEdgeDecode(const McGrid& g, long long e, int* axis, int* i,
+                                             int* j, int* k) {
+  const int ax = (int)(e / g.E);
+  long long r = e - ax * g.E;
+  const int ni = ax == 0 ? g.n : g.N, nj = ax == 1 ? g.n : g.N;
+  *axis = ax;
+  *i = (int)(r % ni);
+  r /= ni;
+  *j = (int)(r % nj);
+  *k = (int)(r / nj);
+}
+
+__global__ void __launch_bounds__(kBlk) kMcField(McGrid g, double* __restrict__ f) {
+  const long long np = (long long)g.N * g.N * g.N;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < np;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.N), j = (int)((t / g.N) % g.N), k = (int)(t / ((long long)g.N * g.N));
+    const double x = g.ox + (double)i * g.h, y = g.oy + (double)j * g.h,
+                 z = g.oz + (double)k * g.h;
+    f[t] = sin(g.a * x + g.phase) * sin(g.a * y) * sin(g.a * z) +
+           0.3 * sin(g.b * (x + y + z) + g.phase);
+  }
+}
+
+__device__ __forceinline__ void mcVertex(const McGrid& g, int axis, int i, int j, int k,
+                                         double f0, double f1, float out[3]) {
+  const double t = (0.0 - f0) / (f1 - f0);
+  double pos[3] = {g.ox + (double)i * g.h, g.oy + (double)j * g.h, g.oz + (double)k * g.h};
+  pos[axis] += t * g.h;
+  for (int c = 0; c < 3; ++c) out[c] = (float)pos[c];
+}
+
+// flag[e] = 1 where the field changes sign along edge e (flag[3E] = 0); with
+// `part`, also the per-block bounds of the edge vertices.
+__global__ void __launch_bounds__(kBlk) kMcEdges(McGrid g, const double* __restrict__ f,
+                                                 uint32_t* __restrict__ flag, float* part) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  const long long ne = 3 * g.E;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    int axis, i, j, k;
+    mcEdgeDecode(g, e, &axis, &i, &j, &k);
+    const double f0 = mcF(f, g, i, j, k);
+    const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
+    const bool cut = (f0 < 0.0) != (f1 < 0.0);
+    flag[e] = cut ? 1u : 0u;
+    if (part && cut) {
+      float p[3];
+      mcVertex(g, axis, i, j, k, f0, f1, p);
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = fminf(lo[c], p[c]);
+        hi[c] = fmaxf(hi[c], p[c]);
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) flag[ne] = 0u;
+  if (part) blockBounds(lo, hi, part);
+}
+
+__global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g, const double* __restrict__ f,
+                                                 const uint32_t* __restrict__ off, uint32_t color,
+                                                 float* __restrict__ v,
+                                                 uint32_t* __restrict__ rgb) {
+  const long long ne = 3 * g.E;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const uint32_t id = off[e];
+    if (off[e + 1] == id) continue;
+    int axis, i, j, k;
+    mcEdgeDecode(g, e, &axis, &i, &j, &k);
+    const double f0 = mcF(f, g, i, j, k);
+    const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
+    float p[3];
+    mcVertex(g, axis, i, j, k, f0, f1, p);
+    for (int c = 0; c < 3; ++c) v[(size_t)id * 3 + c] = p[c];
+    rgb[id] = color;
+  }
+}
+
+__device__ __forceinline__ int mcCase(const McGrid& g, const double* f, int i, int j, int k) {
+  int cs = 0;
+  for (int c = 0; c < 8; ++c)
+    if (mcF(f, g, i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) < 0.0) cs |= 1 << c;
+  return cs;
+}
+
+__global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const double* __restrict__ f,
+                                                     const uint8_t* __restrict__ ntri,
+                                                     uint32_t* __restrict__ cnt) {
+  const long long nc = (long long)g.n * g.n * g.n;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nc;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.n), j = (int)((t / g.n) % g.n), k = (int)(t / ((long long)g.n * g.n));
+    cnt[t] = ntri[mcCase(g, f, i, j, k)];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nc] = 0u;
+}
+
+__global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const double* __restrict__ f,
+                                                 const int8_t* __restrict__ table, McCorners ec,
+                                                 const uint32_t* __restrict__ eoff,
+                                                 const uint32_t* __restrict__ coff,
+                                                 uint32_t* __restrict__ out) {
+  const long long nc = (long long)g.n * g.n * g.n;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nc;
+       t += (long long)gridDim.x * blockDim.x) {
+    const uint32_t base = coff[t];
+    if (coff[t + 1] == base) continue;
+    const int i = (int)(t % g.n), j = (int)((t / g.n) % g.n), k = (int)(t / ((long long)g.n * g.n));
+    const int8_t* tr = table + mcCase(g, f, i, j, k) * 25;
+    for (int q = 0; tr[q] >= 0; q += 3)
+      for (int r = 0; r < 3; ++r) {
+        const int e = tr[q + r];
+        const int ax = ec.axis[e];
+        const long long idx =
+            ax * g.E + mcEdgeIndex(g, ax, i + ec.dx[e], j + ec.dy[e], k + ec.dz[e]);
+        out[((size_t)base + q / 3) * 3 + r] = eoff[idx];
+      }
+  }
+}
+
+__global__ void kMcTotals(const uint32_t* vtotal, const uint32_t* ftotal, uint32_t* out) {
+  out[0] = *vtotal;
+  out[1] = *ftotal;
+}
+
+struct McSource : MeshSource {
+  spray_scenegen_params p;
+  int nd;
+  std::vector<McGrid> grid;
+  std::vector<long long> nv, nf;
+  std::vector<float> bounds;  // 6 per domain: lo xyz, hi xyz
+  DevBuf field, eflag, ccnt, table, ntri, part, dev_out;
+  McCorners ec;
+  McSource(Ctx* c, const spray_scenegen_params& prm) : p(prm) {
+    nd = p.nx * p.ny * p.nz;
+    grid.resize(nd);
+    long long max_pts = 0, max_e = 0, max_c = 0;
+    for (int d = 0; d < nd; ++d) {
+      double a, b;
+      int n;
+      mcDomainParams(p, d, &a, &b, &n);
+      McGrid& g = grid[d];
+      g.n = n;
+      g.N = n + 1;
+      g.E = (long long)n * g.N * g.N;
+      g.ox = (double)(d % p.nx);
+      g.oy = (double)((d / p.nx) % p.ny);
+      g.oz = (double)(d / (p.nx * p.ny));
+      g.h = 1.0 / n;
+      g.a = a;
+      g.b = b;
+      g.phase = p.phase;
+      max_pts = std::max(max_pts, (long long)g.N * g.N * g.N);
+      max_e = std::max(max_e, 3 * g.E + 1);
+      max_c = std::max(max_c, (long long)n * n * n + 1);
+    }
+    if (max_e >= (1ll << 32) || max_c >= (1ll << 32))
+      throw StatusError(SPRAY_ERR_CAPACITY, "scenegen: grid too fine for 32-bit numbering");
+    field.reserve((size_t)max_pts * 8 + 64);
+    eflag.reserve((size_t)max_e * 4 + 64);
+    ccnt.reserve((size_t)max_c * 4 + 64);
+    const int8_t* T = mcTriTable();
+    uint8_t counts[256];
+    for (int cs = 0; cs < 256; ++cs) {
+      int q = 0;
+      while (T[cs * 25 + q] >= 0) q += 3;
+      counts[cs] = (uint8_t)(q / 3);
+    }
+    for (int e = 0; e < 12; ++e) mcEdgeCorner(e, &ec.dx[e], &ec.dy[e], &ec.dz[e], &ec.axis[e]);
+    table.reserve(256 * 25);
+    ntri.reserve(256);
+    part.reserve((size_t)kBoundBlocks * 6 * 4 + 64);
+    dev_out.reserve((size_t)nd * 8 * 4 + 64);
+    hipStream_t s = c->stream;
+    SPRAY_HIP(hipMemcpyAsync(table.p, T, 256 * 25, hipMemcpyHostToDevice, s));
+    SPRAY_HIP(hipMemcpyAsync(ntri.p, counts, 256, hipMemcpyHostToDevice, s));
+    // count pass: vertex and face totals and vertex bounds of every domain
+    uint32_t* tot = dev_out.as<uint32_t>();
+    float* bnd = (float*)(tot + 2 * (size_t)nd);
+    for (int d = 0; d < nd; ++d) {
+      const McGrid& g = grid[d];
+      const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
+      kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>());
+      kMcEdges<<<gridOf(ne, kBlk, kBoundBlocks), kBlk, 0, s>>>(g, field.as<double>(),
+                                                              eflag.as<uint32_t>(), part.as<float>());
+      kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), gridOf(ne, kBlk, kBoundBlocks), bnd + 6 * d);
+      exclusiveScan(eflag.as<uint32_t>(), eflag.as<uint32_t>(), ne + 1, s);
+      kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), ntri.as<uint8_t>(),
+                                                    ccnt.as<uint32_t>());
+      exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), nc + 1, s);
+      kMcTotals<<<1, 1, 0, s>>>(eflag.as<uint32_t>() + ne, ccnt.as<uint32_t>() + nc, tot + 2 * d);
+    }
+    SPRAY_HIP(hipGetLastError());
+    std::vector<uint32_t> h((size_t)nd * 2);
+    bounds.resize((size_t)nd * 6);
+    SPRAY_HIP(hipMemcpyAsync(h.data(), tot, (size_t)nd * 8, hipMemcpyDeviceToHost, s));
+    SPRAY_HIP(hipMemcpyAsync(bounds.data(), bnd, (size_t)nd * 24, hipMemcpyDeviceToHost, s));
+    SPRAY_HIP(hipStreamSynchronize(s));
+    nv.resize(nd);
+    nf.resize(nd);
+    for (int d = 0; d < nd; ++d) {
+      nv[d] = h[2 * d];
+      nf[d] = h[2 * d + 1];
+    }
+  }
+  ~McSource() override {
+    for (DevBuf* b : {&field, &eflag, &ccnt, &table, &ntri, &part, &dev_out}) b->release();
+  }
+  long long numVerts(int d) const override { return nv[d]; }
+  long long numFaces(int d) const override { return nf[d]; }
+  void fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) override {
+    (void)c;
+    const McGrid& g = grid[d];
+    const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
+    kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>());
+    kMcEdges<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), eflag.as<uint32_t>(), nullptr);
+    exclusiveScan(eflag.as<uint32_t>(), eflag.as<uint32_t>(), ne + 1, s);
+    kMcVerts<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), eflag.as<uint32_t>(), p.rgb,
+                                              v, rgb);
+    kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), ntri.as<uint8_t>(),
+                                                  ccnt.as<uint32_t>());
+    exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), nc + 1, s);
+    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), table.as<int8_t>(), ec,
+                                              eflag.as<uint32_t>(), ccnt.as<uint32_t>(), f);
+    SPRAY_HIP(hipGetLastError());
+  }
+};
+
+void requireLight(const Ctx* c, const HostScene& scene) {
+  if (c->cfg.ao_mode == 0 && scene.lights.empty())
+    throw StatusError(SPRAY_ERR_INVALID, "path tracing needs at least one light");
+}
+
+long long domainVerts(const DomainPool& P, int d) { return P.nverts[d]; }
+
+}  // namespace
+
+void loadScenePly(Ctx* c, HostScene& scene) {
+  c->scene = std::move(scene);
+  c->generated = false;
+  PlySource src(c);
+  requireLight(c, c->scene);
+  tracerRelease(c);
+  buildPool(c, src);
+}
+
+void generateScene(Ctx* c, const spray_scenegen_params& p, const std::string& name,
+                   const std::string& extra, bool remesh) {
+  const int nd = p.nx * p.ny * p.nz;
+  if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0 || (p.res <= 0 && !p.per_domain_random))
+    throw StatusError(SPRAY_ERR_INVALID, "scenegen: empty domain grid or resolution");
+  if (nd > kMaxDomains) throw StatusError(SPRAY_ERR_INVALID, "scenegen: too many domains");
+  McSource src(c, p);
+  // the .spray description spray_scenegen_write would write for this scene
+  std::string text = extra + "\n";
+  const float white[3] = {1.f, 1.f, 1.f};
+  for (int d = 0; d < nd; ++d) {
+    if (src.nf[d] == 0)
+      throw StatusError(SPRAY_ERR_INVALID, "scenegen: a domain has no surface (empty mesh)");
+    char fname[64];
+    std::snprintf(fname, sizeof(fname), "%s_d%04d.ply", name.c_str(), d);
+    text += sceneDomainBlock(d, fname, kDiffuse, white, 1.f, 1.f, &src.bounds[6 * d],
+                             &src.bounds[6 * d + 3], src.nv[d], src.nf[d]);
+  }
+  HostScene scene;
+  std::string err;
+  int rc = parseSceneText(text, name + ".spray", "", &scene, &err);
+  if (rc != SPRAY_OK) throw StatusError(rc, err);
+  requireLight(c, scene);
+  c->scene = std::move(scene);
+  if (remesh) {
+    c->param_version++;  // pool addresses change: re-capture the pass graphs
+  } else {
+    tracerRelease(c);
+  }
+  buildPool(c, src, remesh);
+  c->generated = true;
+  c->gen = p;
+  c->gen_name = name;
+  c->gen_extra = extra;
+}
+
+void domainMesh(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, long long* nv_out,
+                long long* nf_out) {
+  const DomainPool& P = c->pool;
+  const DomainDesc& D = P.desc[d];
+  const long long nv = domainVerts(P, d);
+  if (nv_out) *nv_out = nv;
+  if (nf_out) *nf_out = D.num_tris;
+  if (!v && !f && !rgb) return;
+  if (P.paged) throw StatusError(SPRAY_ERR_STATE, "a paged pool keeps no raw meshes");
+  SPRAY_HIP(hipStreamSynchronize(c->stream));
+  if (v)
+    SPRAY_HIP(hipMemcpy(v, P.vtx.as<float>() + D.vtx_off * 3, (size_t)nv * 12, hipMemcpyDeviceToHost));
+  if (f)
+    SPRAY_HIP(hipMemcpy(f, P.face.as<uint32_t>() + D.face_off * 3, (size_t)D.num_tris * 12,
+                        hipMemcpyDeviceToHost));
+  if (rgb)
+    SPRAY_HIP(hipMemcpy(rgb, P.rgb.as<uint32_t>() + D.vtx_off, (size_t)nv * 4, hipMemcpyDeviceToHost));
+}
+
+void writeScene(Ctx* c, const std::string& dir, const std::string& name) {
+  const DomainPool& P = c->pool;
+  if (P.paged) throw StatusError(SPRAY_ERR_STATE, "a paged pool keeps no raw meshes");
+  const int nd = P.ndomains;
+  std::vector<std::string> blocks(nd);
+  std::vector<int> failed(nd, 0);
+  long long total = 0;
+  for (int d = 0; d < nd; ++d) total += P.desc[d].num_tris;
+  SPRAY_HIP(hipStreamSynchronize(c->stream));
+  const int nthreads = std::max(1, std::min(c->cfg.nthreads, 16));
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int d = 0; d < nd; ++d) {
+    if (hipSetDevice(c->device) != hipSuccess) {
+      failed[d] = 2;
+      continue;
+    }
+    const HostDomain& hd = c->scene.domains[d];
+    const long long nv = domainVerts(P, d), nf = P.desc[d].num_tris;
+    HostMesh m;
+    m.v.resize((size_t)nv * 3);
+    m.f.resize((size_t)nf * 3);
+    m.rgb.resize((size_t)nv);
+    const DomainDesc& D = P.desc[d];
+    if (hipMemcpy(m.v.data(), P.vtx.as<float>() + D.vtx_off * 3, (size_t)nv * 12,
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(m.f.data(), P.face.as<uint32_t>() + D.face_off * 3, (size_t)nf * 12,
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(m.rgb.data(), P.rgb.as<uint32_t>() + D.vtx_off, (size_t)nv * 4,
+                  hipMemcpyDeviceToHost) != hipSuccess) {
+      failed[d] = 2;
+      continue;
+    }
+    char fname[64];
+    std::snprintf(fname, sizeof(fname), "%s_d%04d.ply", name.c_str(), d);
+    if (!writePlyBinary(dir + "/" + fname, m.v.data(), (size_t)nv, m.f.data(), (size_t)nf,
+                        m.rgb.data())) {
+      failed[d] = 1;
+      continue;
+    }
+    blocks[d] = sceneDomainBlock(d, fname, hd.mtl, hd.mtl_rgb, hd.eta_a, hd.eta_b, hd.lo, hd.hi, nv,
+                                 nf);
+  }
+  for (int d = 0; d < nd; ++d) {
+    if (failed[d] == 2) throw DeviceError("mesh readback failed");
+    if (failed[d]) throw StatusError(SPRAY_ERR_IO, "cannot write PLY into " + dir);
+  }
+  const std::string spath = dir + "/" + name + ".spray";
+  FILE* fp = std::fopen(spath.c_str(), "w");
+  if (!fp) throw StatusError(SPRAY_ERR_IO, "cannot write " + spath);
+  std::fprintf(fp, "# spray_amd scene: %d domains, %lld triangles\n", nd, total);
+  for (const HostLight& L : c->scene.lights) std::fprintf(fp, "%s\n", sceneLightLine(L).c_str());
+  std::fprintf(fp, "\n");
+  for (auto& b : blocks) std::fputs(b.c_str(), fp);
+  if (std::fclose(fp) != 0) throw StatusError(SPRAY_ERR_IO, "cannot write " + spath);
+}
+
+void plyInfo(Ctx* c, const std::string& path, long long* nv, long long* nf, float lo[3],
+             float hi[3]) {
+  PlyLayout L;
+  std::string err;
+  int rc = readPlyHeader(path, &L, &err);
+  if (rc != SPRAY_OK) throw StatusError(rc, err);
+  HostMesh host;
+  if (L.binary) {
+    rc = checkBody(path, L, &err);
+  } else {
+    rc = readPly(path, &host, &err);  // ASCII bodies are parsed on the host
+  }
+  if (rc != SPRAY_OK) throw StatusError(rc, err);
+  hipStream_t s = c->stream;
+  struct Bufs {
+    DevBuf v, f, rgb, aux;
+    ~Bufs() {
+      for (DevBuf* b : {&v, &f, &rgb, &aux}) b->release();
+    }
+  } B;
+  // aux: block partials, the final bounds, the face error key
+  B.aux.reserve((size_t)kBoundBlocks * 24 + 64);
+  float* part = B.aux.as<float>();
+  float* out = part + (size_t)kBoundBlocks * 6;
+  unsigned long long* ekey = (unsigned long long*)(out + 8);
+  B.v.reserve((size_t)L.nv * 12 + 64);
+  if (L.binary) {
+    B.f.reserve((size_t)L.nf * 12 + 64);
+    B.rgb.reserve((size_t)L.nv * 4 + 64);
+    BodyStager st((size_t)(L.nv * L.vertexStride() + L.nf * L.faceStride()));
+    float id[16];
+    for (int i = 0; i < 16; ++i) id[i] = (i % 5 == 0) ? 1.f : 0.f;
+    SPRAY_HIP(hipMemsetAsync(ekey, 0xff, 8, s));
+    st.decode(path, L, id, false, B.v.as<float>(), B.f.as<uint32_t>(), B.rgb.as<uint32_t>(), ekey,
+              s, std::max(1, c->cfg.nthreads));
+    unsigned long long e = ~0ull;
+    SPRAY_HIP(hipMemcpyAsync(&e, ekey, 8, hipMemcpyDeviceToHost, s));
+    SPRAY_HIP(hipStreamSynchronize(s));
+    if (e != ~0ull) throw StatusError(SPRAY_ERR_FORMAT, faceError(path, e));
+  } else {
+    SPRAY_HIP(hipMemcpyAsync(B.v.p, host.v.data(), host.v.size() * 4, hipMemcpyHostToDevice, s));
+  }
+  // Aabb::merge over every vertex (apps/ply_header_reader.cc:84-88)
+  const int nb = gridOf(L.nv, kBlk, kBoundBlocks);
+  kPointBounds<<<nb, kBlk, 0, s>>>(B.v.as<float>(), L.nv, part);
+  kBoundsFinal<<<1, kBlk, 0, s>>>(part, nb, out);
+  SPRAY_HIP(hipGetLastError());
+  float b[6];
+  SPRAY_HIP(hipMemcpyAsync(b, out, 24, hipMemcpyDeviceToHost, s));
+  SPRAY_HIP(hipStreamSynchronize(s));
+  *nv = L.nv;
+  *nf = L.nf;
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = b[k];
+    hi[k] = b[3 + k];
+  }
+}
+
+}  // namespace spray

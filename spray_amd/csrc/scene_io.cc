@@ -78,13 +78,10 @@ void transformPoint(const float xf[16], const float p[3], float out[3]) {
   }
 }
 
-int parseSceneFile(const std::string& spray_file, const std::string& ply_path, HostScene* s,
-                   std::string* err) {
-  std::ifstream in(spray_file);
-  if (!in.is_open()) {
-    *err = "unable to open input file " + spray_file;
-    return SPRAY_ERR_IO;
-  }
+namespace {
+
+int parseSceneStream(std::istream& in, const std::string& spray_file, const std::string& ply_path,
+                     HostScene* s, std::string* err) {
   s->domains.clear();
   s->lights.clear();
   auto bad = [&](const std::string& m) {
@@ -221,12 +218,28 @@ int parseSceneFile(const std::string& spray_file, const std::string& ply_path, H
   return SPRAY_OK;
 }
 
-int readPly(const std::string& filename, HostMesh* m, std::string* err) {
-  std::ifstream in(filename, std::ios::in | std::ios::binary);
+}  // namespace
+
+int parseSceneFile(const std::string& spray_file, const std::string& ply_path, HostScene* s,
+                   std::string* err) {
+  std::ifstream in(spray_file);
   if (!in.is_open()) {
-    *err = "unable to open " + filename;
+    *err = "unable to open input file " + spray_file;
     return SPRAY_ERR_IO;
   }
+  return parseSceneStream(in, spray_file, ply_path, s, err);
+}
+
+int parseSceneText(const std::string& text, const std::string& name, const std::string& ply_path,
+                   HostScene* s, std::string* err) {
+  std::istringstream in(text);
+  return parseSceneStream(in, name, ply_path, s, err);
+}
+
+namespace {
+
+// Parses the header; `in` is left at the first element's data.
+int parsePlyHeader(std::istream& in, const std::string& filename, PlyLayout* L, std::string* err) {
   auto bad = [&](const std::string& msg) {
     *err = filename + ": " + msg;
     return SPRAY_ERR_FORMAT;
@@ -310,6 +323,51 @@ int readPly(const std::string& filename, HostMesh* m, std::string* err) {
   if (!ended || fmt == kNone || nv <= 0 || nf <= 0) return bad("incomplete header");
   if (xyz != 3 || (rgb != 0 && rgb != 3)) return bad("vertex needs x y z [red green blue]");
   if (!list_bytes) return bad("face element has no index list");
+  L->binary = fmt == kLittle;
+  L->nv = nv;
+  L->nf = nf;
+  L->rgb = rgb != 0;
+  L->list_bytes = list_bytes;
+  L->index_bytes = index_bytes;
+  L->faces_first = order[0] == 1;
+  L->body = (long long)in.tellg();
+  return SPRAY_OK;
+}
+
+}  // namespace
+
+int readPlyHeader(const std::string& filename, PlyLayout* L, std::string* err) {
+  std::ifstream in(filename, std::ios::in | std::ios::binary);
+  if (!in.is_open()) {
+    *err = "unable to open " + filename;
+    return SPRAY_ERR_IO;
+  }
+  int rc = parsePlyHeader(in, filename, L, err);
+  if (rc != SPRAY_OK) return rc;
+  in.seekg(0, std::ios::end);
+  L->file_bytes = (long long)in.tellg();
+  return SPRAY_OK;
+}
+
+int readPly(const std::string& filename, HostMesh* m, std::string* err) {
+  std::ifstream in(filename, std::ios::in | std::ios::binary);
+  if (!in.is_open()) {
+    *err = "unable to open " + filename;
+    return SPRAY_ERR_IO;
+  }
+  auto bad = [&](const std::string& msg) {
+    *err = filename + ": " + msg;
+    return SPRAY_ERR_FORMAT;
+  };
+  PlyLayout L;
+  int rc = parsePlyHeader(in, filename, &L, err);
+  if (rc != SPRAY_OK) return rc;
+  const long long nv = L.nv, nf = L.nf;
+  const bool rgb = L.rgb;
+  const int list_bytes = L.list_bytes, index_bytes = L.index_bytes;
+  enum { kAscii, kLittle } fmt = L.binary ? kLittle : kAscii;
+  const int order[2] = {L.faces_first ? 1 : 0, L.faces_first ? 0 : 1};
+  std::string line;
   m->v.resize((size_t)nv * 3);
   m->f.resize((size_t)nf * 3);
   m->rgb.assign((size_t)nv, 0u);
@@ -369,6 +427,68 @@ int readPly(const std::string& filename, HostMesh* m, std::string* err) {
     }
   }
   return SPRAY_OK;
+}
+
+bool writePlyBinary(const std::string& path, const float* v, size_t nv, const uint32_t* f,
+                    size_t nf, const uint32_t* rgb) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) return false;
+  std::fprintf(fp,
+               "ply\nformat binary_little_endian 1.0\ncomment spray_amd marching cubes\n"
+               "element vertex %zu\nproperty float x\nproperty float y\nproperty float z\n"
+               "property uchar red\nproperty uchar green\nproperty uchar blue\n"
+               "element face %zu\nproperty list uchar int vertex_indices\nend_header\n",
+               nv, nf);
+  std::vector<uint8_t> buf(nv * 15);
+  for (size_t i = 0; i < nv; ++i) {
+    std::memcpy(&buf[i * 15], &v[i * 3], 12);
+    uint32_t c = rgb[i];
+    buf[i * 15 + 12] = (c >> 16) & 0xff;
+    buf[i * 15 + 13] = (c >> 8) & 0xff;
+    buf[i * 15 + 14] = c & 0xff;
+  }
+  bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+  std::vector<uint8_t> fb(nf * 13);
+  for (size_t i = 0; i < nf; ++i) {
+    fb[i * 13] = 3;
+    std::memcpy(&fb[i * 13 + 1], &f[i * 3], 12);
+  }
+  ok = std::fwrite(fb.data(), 1, fb.size(), fp) == fb.size() && ok;
+  return std::fclose(fp) == 0 && ok;
+}
+
+// %.9g round-trips every float, so a written scene reads back bit-exactly.
+std::string sceneLightLine(const HostLight& L) {
+  char b[512];
+  if (L.type == kPointLight)
+    std::snprintf(b, sizeof(b), "light point %.9g %.9g %.9g %.9g %.9g %.9g", L.pos[0], L.pos[1],
+                  L.pos[2], L.radiance[0], L.radiance[1], L.radiance[2]);
+  else if (L.type == kHemisphereLight)
+    std::snprintf(b, sizeof(b), "light diffuse %.9g %.9g %.9g", L.radiance[0], L.radiance[1],
+                  L.radiance[2]);
+  else
+    std::snprintf(b, sizeof(b),
+                  "light rect %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g",
+                  L.pos[0], L.pos[1], L.pos[2], L.e1[0], L.e1[1], L.e1[2], L.e2[0], L.e2[1],
+                  L.e2[2], L.radiance[0], L.radiance[1], L.radiance[2]);
+  return b;
+}
+
+std::string sceneDomainBlock(int id, const std::string& file, int mtl, const float mtl_rgb[3],
+                             float eta_a, float eta_b, const float lo[3], const float hi[3],
+                             long long nv, long long nf) {
+  char m[160], b[768];
+  if (mtl == kDiffuse || mtl == kMirror)
+    std::snprintf(m, sizeof(m), "mtl %s %.9g %.9g %.9g", mtl == kDiffuse ? "diffuse" : "mirror",
+                  mtl_rgb[0], mtl_rgb[1], mtl_rgb[2]);
+  else
+    std::snprintf(m, sizeof(m), "mtl %s %.9g %.9g", mtl == kGlass ? "glass" : "transmission",
+                  eta_a, eta_b);
+  std::snprintf(b, sizeof(b),
+                "# domain %d\ndomain\nfile %s\n%s\nbound %.9g %.9g %.9g %.9g %.9g %.9g\n"
+                "vertex %lld\nface %lld\n\n",
+                id, file.c_str(), m, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], nv, nf);
+  return b;
 }
 
 }  // namespace spray

@@ -98,8 +98,12 @@ const McTable& table() {
   return t;
 }
 
+}  // namespace
+
+const int8_t* mcTriTable() { return &table().tri[0][0]; }
+
 // local edge id -> (corner with 0 on axis, axis)
-void edgeInfo(int e, int* dx, int* dy, int* dz, int* axis) {
+void mcEdgeCorner(int e, int* dx, int* dy, int* dz, int* axis) {
   *axis = e / 4;
   int k = e % 4;
   int c = 0, m = 0;
@@ -115,6 +119,8 @@ void edgeInfo(int e, int* dx, int* dy, int* dz, int* axis) {
   *dy = (c >> 1) & 1;
   *dz = (c >> 2) & 1;
 }
+
+namespace {
 
 uint64_t splitmix64(uint64_t& x) {
   uint64_t z = (x += 0x9e3779b97f4a7c15ull);
@@ -197,7 +203,7 @@ void mcGenerate(const spray_scenegen_params& p, int d, HostMesh* m) {
   }
   m->f.clear();
   int ex[12], ey[12], ez[12], eax[12];
-  for (int e = 0; e < 12; ++e) edgeInfo(e, &ex[e], &ey[e], &ez[e], &eax[e]);
+  for (int e = 0; e < 12; ++e) mcEdgeCorner(e, &ex[e], &ey[e], &ez[e], &eax[e]);
   for (int k = 0; k < n; ++k)
     for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) {
@@ -216,37 +222,6 @@ void mcGenerate(const spray_scenegen_params& p, int d, HostMesh* m) {
   m->rgb.assign(m->v.size() / 3, p.rgb);
 }
 
-namespace {
-
-bool writePly(const std::string& path, const HostMesh& m) {
-  FILE* fp = std::fopen(path.c_str(), "wb");
-  if (!fp) return false;
-  size_t nv = m.v.size() / 3, nf = m.f.size() / 3;
-  std::fprintf(fp,
-               "ply\nformat binary_little_endian 1.0\ncomment spray_amd marching cubes\n"
-               "element vertex %zu\nproperty float x\nproperty float y\nproperty float z\n"
-               "property uchar red\nproperty uchar green\nproperty uchar blue\n"
-               "element face %zu\nproperty list uchar int vertex_indices\nend_header\n",
-               nv, nf);
-  std::vector<uint8_t> buf(nv * 15);
-  for (size_t i = 0; i < nv; ++i) {
-    std::memcpy(&buf[i * 15], &m.v[i * 3], 12);
-    uint32_t c = m.rgb[i];
-    buf[i * 15 + 12] = (c >> 16) & 0xff;
-    buf[i * 15 + 13] = (c >> 8) & 0xff;
-    buf[i * 15 + 14] = c & 0xff;
-  }
-  std::fwrite(buf.data(), 1, buf.size(), fp);
-  std::vector<uint8_t> fb(nf * 13);
-  for (size_t i = 0; i < nf; ++i) {
-    fb[i * 13] = 3;
-    std::memcpy(&fb[i * 13 + 1], &m.f[i * 3], 12);
-  }
-  std::fwrite(fb.data(), 1, fb.size(), fp);
-  return std::fclose(fp) == 0;
-}
-
-}  // namespace
 
 int scenegenWrite(const spray_scenegen_params& p, const char* dir, const char* name,
                   const char* extra_lines, int64_t* tris_out, std::string* err) {
@@ -267,7 +242,7 @@ int scenegenWrite(const spray_scenegen_params& p, const char* dir, const char* n
     char fname[64];
     std::snprintf(fname, sizeof(fname), "%s_d%04d.ply", name, d);
     std::string path = std::string(dir) + "/" + fname;
-    if (nf == 0 || !writePly(path, m)) {
+    if (nf == 0 || !writePlyBinary(path, m.v.data(), nv, m.f.data(), nf, m.rgb.data())) {
       failed[d] = nf == 0 ? 2 : 1;
       continue;
     }
@@ -277,12 +252,9 @@ int scenegenWrite(const spray_scenegen_params& p, const char* dir, const char* n
         lo[c] = std::fmin(lo[c], m.v[i * 3 + c]);
         hi[c] = std::fmax(hi[c], m.v[i * 3 + c]);
       }
-    char buf[512];
-    std::snprintf(buf, sizeof(buf),
-                  "# domain %d\ndomain\nfile %s\nmtl diffuse 1 1 1\n"
-                  "bound %.9g %.9g %.9g %.9g %.9g %.9g\nvertex %zu\nface %zu\n\n",
-                  d, fname, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], nv, nf);
-    blocks[d] = buf;
+    const float white[3] = {1.f, 1.f, 1.f};
+    blocks[d] = sceneDomainBlock(d, fname, kDiffuse, white, 1.f, 1.f, lo, hi, (long long)nv,
+                                 (long long)nf);
   }
   int64_t total = 0;
   for (int d = 0; d < nd; ++d) {

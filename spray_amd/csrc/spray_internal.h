@@ -52,16 +52,46 @@ struct HostScene {
   float lo[3], hi[3];
 };
 
+// Layout of a PLY file as its header describes it (io/ply_loader.cc:114-178).
+struct PlyLayout {
+  bool binary = false;      // binary_little_endian; else ascii
+  long long nv = 0, nf = 0;
+  bool rgb = false;         // vertex has red green blue (uchar) after x y z (float)
+  int list_bytes = 0;       // face list count type size
+  int index_bytes = 0;      // face list index type size
+  bool faces_first = false; // the face element precedes the vertex element
+  long long body = 0;       // byte offset of the first element's data
+  long long file_bytes = 0;
+  int vertexStride() const { return 12 + (rgb ? 3 : 0); }
+  int faceStride() const { return list_bytes + 3 * index_bytes; }
+};
+
 // scene_io.cc
 int parseSceneFile(const std::string& spray_file, const std::string& ply_path, HostScene* s,
                    std::string* err);
+// the same grammar over text held in memory (`name` labels errors)
+int parseSceneText(const std::string& text, const std::string& name, const std::string& ply_path,
+                   HostScene* s, std::string* err);
+int readPlyHeader(const std::string& filename, PlyLayout* L, std::string* err);
 int readPly(const std::string& filename, HostMesh* m, std::string* err);
 bool transformIsIdentity(const float xf[16]);
 void transformPoint(const float xf[16], const float p[3], float out[3]);
+// binary little-endian PLY: float x y z, uchar red green blue, list uchar int
+bool writePlyBinary(const std::string& path, const float* v, size_t nv, const uint32_t* f,
+                    size_t nf, const uint32_t* rgb);
+// .spray lines of a light / of a domain with identity transform
+std::string sceneLightLine(const HostLight& L);
+std::string sceneDomainBlock(int id, const std::string& file, int mtl, const float mtl_rgb[3],
+                             float eta_a, float eta_b, const float lo[3], const float hi[3],
+                             long long nv, long long nf);
 
 // scenegen.cc
 void mcDomainParams(const spray_scenegen_params& p, int d, double* a, double* b, int* res);
 void mcGenerate(const spray_scenegen_params& p, int d, HostMesh* m);
+// marching-cubes case table: up to 8 triangles (3 local edge ids each) per
+// case, -1 terminated; local edge e = (corner with 0 on axis e/4, axis)
+const int8_t* mcTriTable();  // [256][25]
+void mcEdgeCorner(int e, int* dx, int* dy, int* dz, int* axis);
 int scenegenWrite(const spray_scenegen_params& p, const char* dir, const char* name,
                   const char* extra_lines, int64_t* tris_out, std::string* err);
 

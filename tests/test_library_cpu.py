@@ -188,3 +188,18 @@ def test_write_ppm_matches_reference_format(tmp_path):
     assert px[3] == (0, 511, 1023) and px[4] == (0, 0, 0) and px[5] == (1023, 0, 1022)
     with pytest.raises(sa.SprayError):
         sa.write_ppm("/nonexistent_dir_for_test/x.ppm", img)
+
+
+def test_ply_info_file_format_matches_header_reader(tmp_path):
+    """write_ply_info prints what apps/ply_header_reader.cc:90-104 printed
+    into examples/wavelet/wavelet_example.spray: counts, and float bounds
+    with 16 significant digits (values from the oracle's PLY reader)."""
+    import spray_amd as sa
+    from test_oracle_geometry import GOLD, ply_read, spray_declared
+    for d in spray_declared(os.path.join(GOLD, "wavelet_example.spray")):
+        nv, nf, _, lo, hi = ply_read(os.path.join(GOLD, d["file"][0]))
+        out = tmp_path / "info.txt"
+        sa.write_ply_info(d["file"][0], out, (nv, nf, lo, hi))
+        assert out.read_text().splitlines() == [
+            f"file {d['file'][0]}", f"vertex {d['vertex'][0]}", f"face {d['face'][0]}",
+            "bound " + " ".join(d["bound"])]
