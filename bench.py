@@ -11,7 +11,11 @@ samples).  Synthetic geometry generated in-process.
 Other workloads (--workload): "c1" AO on the same geometry (512x512, 1 spp,
 8 AO rays, max distance 0.5) and "c3" path tracing on 256 domains in an
 8x8x4 grid with log-uniform per-domain triangle budgets (10k..2M, rescaled to
-~120M in total), 1920x1080, 16 spp, depth 6.
+~120M in total), 1920x1080, 16 spp, depth 6; "c5" the in-situ stress case:
+1024 domains in a 16x8x8 grid, ~500k triangles each (160^3 cells), generated
+on the GPU and remeshed on the GPU before every timed frame with the field
+phase advancing 0.05 per frame, 3840x2160, 16 spp, depth 8; the learned
+schedule fine-tunes the predictor online on every frame's hit log.
 
 Prints one JSON line.  `value` is Mrays/s of useful ray queries (eye +
 non-stale bounce + non-stale shadow rays, identical for every schedule) over
@@ -55,6 +59,13 @@ WORKLOADS = {
                cam=None, cpu_tiles=1,
                title="C3 path tracing: 256 domains (8x8x4 MC isosurfaces, log-uniform 10k-2M tris each, "
                      "{tris} tris), {w}x{h}, {spp} spp, depth 6, RR>=3, rect light"),
+    "c5": dict(grid=(16, 8, 8), gen=dict(res=160, rgb=0xB3B3B3,
+                                         extra_lines="light rect 2 8.5 1 12 0 0 0 0 6 4 4 4"),
+               render=dict(image_w=3840, image_h=2160, pixel_samples=16, ao_mode=0, ao_samples=1, bounces=8,
+                           rr_start=3),
+               cam=None, cpu_tiles=0, remesh=0.05,
+               title="C5 in-situ: 1024 domains (16x8x8 MC isosurfaces remeshed on the GPU every frame, "
+                     "phase +0.05 per frame, {tris} tris), {w}x{h}, {spp} spp, depth 8, RR>=3, rect light"),
 }
 
 
@@ -116,11 +127,15 @@ def main():
     import spray_amd as sa
 
     wl = WORKLOADS[args.workload]
+    remesh_step = wl.get("remesh")
     tmp = tempfile.mkdtemp(prefix=f"spray_{args.workload}_")
     t0 = time.time()
     nx, ny, nz = wl["grid"]
     g = dict(wl["gen"])
-    tris = sa.scenegen_write(tmp, args.workload, nx, ny, nz, g.pop("res"), **g)
+    res = g.pop("res")
+    tris = 0
+    if not remesh_step:  # in-situ scenes are generated on the GPU below, with no files
+        tris = sa.scenegen_write(tmp, args.workload, nx, ny, nz, res, **g)
     gen_s = time.time() - t0
     kw = dict(wl["render"])
     if args.size:
@@ -137,7 +152,11 @@ def main():
     ctx.set_cache_size(args.cache_domains)
     ctx.set_tiles_in_flight(args.tiles_in_flight)
     t0 = time.time()
-    ctx.load_scene(os.path.join(tmp, args.workload + ".spray"), tmp)
+    if remesh_step:
+        ctx.scenegen_load(args.workload, nx, ny, nz, res, **g)
+        tris = sum(ctx.mesh_size(i)[1] for i in range(ctx.num_domains))
+    else:
+        ctx.load_scene(os.path.join(tmp, args.workload + ".spray"), tmp)
     load_s = time.time() - t0
     print(f"# {args.workload}: {tris} triangles, generated in {gen_s:.1f} s, loaded in {load_s:.1f} s",
           file=sys.stderr, flush=True)
@@ -153,9 +172,12 @@ def main():
         sched = schedules[name]
         try:
             ctx.set_schedule(sched)
+            if remesh_step:
+                ctx.remesh(0.0)
             if name == "learned":
                 # train the predictor on hit logs of plain-schedule frames
-                ctx.set_schedule(0)
+                # (in-situ: of a batched frame, the plain one taking minutes)
+                ctx.set_schedule(1 if remesh_step else 0)
                 ctx.predictor_log(True, 16_000_000)
                 ctx.trace_frame(want_image=False)
                 ctx.predictor_log(False)
@@ -170,17 +192,31 @@ def main():
         for k in range(args.warmup):
             ctx.trace_frame(want_image=False)
             print(f"# {name}: warmup frame {k + 1}/{args.warmup}", file=sys.stderr, flush=True)
-        frame_ms, walls = [], []
+        frame_ms, walls, remesh_s, train_s = [], [], [], []
         st = None
         t_start = time.perf_counter()
-        for _ in range(args.steps):
+        for step in range(args.steps):
+            if remesh_step:  # the in-situ frame: remesh at the next phase, then trace
+                tw = time.perf_counter()
+                ctx.remesh(remesh_step * (step + 1))
+                remesh_s.append(time.perf_counter() - tw)
+                print(f"# {name}: remesh {step + 1}/{args.steps} {1000.0 * remesh_s[-1]:.1f} ms",
+                      file=sys.stderr, flush=True)
+                if name == "learned":
+                    ctx.predictor_log(True, 4_000_000)
             tw = time.perf_counter()
             img, st = ctx.trace_frame(want_image=False)
             walls.append(time.perf_counter() - tw)
             frame_ms.append(st["trace_ms"])
             print(f"# {name}: frame {len(walls)}/{args.steps} {1000.0 * walls[-1]:.1f} ms", file=sys.stderr,
                   flush=True)
+            if remesh_step and name == "learned":  # online fine-tuning on this frame's hit log
+                tw = time.perf_counter()
+                ctx.predictor_log(False)
+                ctx.predictor_train(1, 4096, 1e-3)
+                train_s.append(time.perf_counter() - tw)
         total_s = time.perf_counter() - t_start
+        frames_s = sum(walls)
         img, st_img = ctx.trace_frame(want_image=True)
         images[name] = img
         # kernel timing of the dominant (trace) kernel, on the tracer's stream
@@ -189,11 +225,13 @@ def main():
         _, st_prof = ctx.trace_frame(want_image=False)
         launches, kern_ms = ctx.kernel_time(reset=True)
         ctx.profile_kernels(args.no_graph)
-        print(f"# {name}: {st['useful_rays'] * args.steps / total_s / 1e6:.1f} Mrays/s, "
-              f"{1000.0 * total_s / args.steps:.1f} ms/frame", file=sys.stderr, flush=True)
+        # in-situ: the frame rate counts the frames; remesh and training are reported beside it
+        timed_s = frames_s if remesh_step else total_s
+        print(f"# {name}: {st['useful_rays'] * args.steps / timed_s / 1e6:.1f} Mrays/s, "
+              f"{1000.0 * timed_s / args.steps:.1f} ms/frame", file=sys.stderr, flush=True)
         results[name] = dict(
-            mrays_s=st["useful_rays"] * args.steps / total_s / 1e6,
-            ms_per_frame=1000.0 * total_s / args.steps,
+            mrays_s=st["useful_rays"] * args.steps / timed_s / 1e6,
+            ms_per_frame=1000.0 * timed_s / args.steps,
             device_ms_median=statistics.median(frame_ms),
             useful_rays=st["useful_rays"], radiance_visits=st["radiance_visits"],
             shadow_visits=st["shadow_visits"], shades=st["shades"], final_hits=st["final_hits"],
@@ -206,6 +244,12 @@ def main():
             cache_bytes=st["cache_bytes"],
             cache_h2d_gbs=st["cache_bytes"] / (st["trace_ms"] * 1e6) if st["trace_ms"] > 0 else 0.0,
         )
+        if remesh_step:
+            results[name].update(
+                remesh_ms=round(1000.0 * statistics.mean(remesh_s), 1),
+                train_ms=round(1000.0 * statistics.mean(train_s), 1) if train_s else 0.0,
+                mrays_s_with_remesh=st["useful_rays"] * args.steps / total_s / 1e6,
+                final_phase=remesh_step * args.steps)
     ref_name = "plain" if "plain" in images else next(iter(images), None)
     for name, img in images.items():
         if ref_name and name != ref_name:
@@ -233,7 +277,7 @@ def main():
                 "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not remesh_step:
         cpu = cpu_baseline(tmp, args.workload, kw, cam, args.cpu_tiles or wl["cpu_tiles"])
 
     line = {

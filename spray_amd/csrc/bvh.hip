@@ -1,6 +1,6 @@
 // HBM domain pool: vertex normals, BVH build (Morton codes, LDS radix sort,
-// PLOC clustering or a Karras hierarchy with atomic bottom-up refit,
-// collapse to a 4-wide BVH)
+// PLOC clustering or a Karras hierarchy whose inner boxes are leaf-range
+// unions, collapse to a 4-wide BVH)
 // and the batch ray queries behind spray_trace_rays / spray_domain_list.
 //
 // Replaces the reference's per-domain Embree scene (rtcCommit in
@@ -98,8 +98,14 @@ __device__ __forceinline__ float fromOrdered(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-__global__ void kCentroidBounds(const float* __restrict__ vtx, const uint32_t* __restrict__ face,
-                                int nf, uint32_t* bmin, uint32_t* bmax) {
+// Launched on a small grid (kBoundsGrid blocks): per-block reduction in LDS,
+// then one atomic per block and bound, so the six bound words see few
+// same-address atomics.
+constexpr int kBoundsGrid = 256;
+__global__ void __launch_bounds__(256) kCentroidBounds(const float* __restrict__ vtx,
+                                                       const uint32_t* __restrict__ face, int nf,
+                                                       uint32_t* bmin, uint32_t* bmax) {
+  __shared__ float red[6][4];
   float lo[3] = {kInfF, kInfF, kInfF}, hi[3] = {-kInfF, -kInfF, -kInfF};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
     for (int k = 0; k < 3; ++k) {
@@ -110,12 +116,23 @@ __global__ void kCentroidBounds(const float* __restrict__ vtx, const uint32_t* _
       hi[k] = fmaxf(hi[k], cen);
     }
   }
+  const int w = threadIdx.x >> 6;
   for (int k = 0; k < 3; ++k) {
     float l = waveMin(lo[k]), h = waveMax(hi[k]);
     if ((threadIdx.x & 63) == 0) {
-      atomicMin(&bmin[k], orderedBits(l));
-      atomicMax(&bmax[k], orderedBits(h));
+      red[k][w] = l;
+      red[3 + k][w] = h;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    float v = red[k][0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = k < 3 ? fminf(v, red[k][i]) : fmaxf(v, red[k][i]);
+    if (k < 3)
+      atomicMin(&bmin[k], orderedBits(v));
+    else
+      atomicMax(&bmax[k - 3], orderedBits(v));
   }
 }
 
@@ -292,40 +309,121 @@ __global__ void kLeaves(const float* __restrict__ vtx, const uint32_t* __restric
   }
 }
 
-__device__ __forceinline__ float ldCoherent(const float* p) {
-  return __uint_as_float(
-      __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ __forceinline__ void stCoherent(float* p, float v) {
-  __hip_atomic_store((uint32_t*)p, __float_as_uint(v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+// Inner-node boxes without a bottom-up climb: in a Karras hierarchy inner
+// node p covers the sorted leaves [first[p], last[p]], so its box is the
+// union of those leaf boxes, a range min/max.  Leaves are grouped in blocks
+// of 64; kLeafScan writes, per leaf, the union from its block's start
+// (prefix) and to its block's end (suffix), and per block the block's box;
+// a sparse table over block boxes (level j: unions of 2^j consecutive
+// blocks) answers the whole blocks of a range with two lookups.  min/max
+// are exact and order-free, so the boxes equal a bottom-up refit's, with
+// no device-scope fences or atomics.
+constexpr int kRangeBlock = 64;
+
+struct RangeLevels {
+  long long off[32];  // float offset of table level j (level 0: block boxes)
+};
+
+__device__ __forceinline__ void boxMerge(float lo[3], float hi[3], const float* b) {
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = fminf(lo[k], b[k]);
+    hi[k] = fmaxf(hi[k], b[3 + k]);
+  }
 }
 
-// Bottom-up refit: the second child to arrive at a node unions both boxes.
-__global__ void kRefit(int n, const int* __restrict__ left, const int* __restrict__ right,
-                       const int* __restrict__ parent_inner, const int* __restrict__ parent_leaf,
-                       const float* __restrict__ leafbox, float* innerbox, uint32_t* flags) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    int p = parent_leaf[i];
-    while (p >= 0) {
-      __threadfence();
-      uint32_t old = atomicAdd(&flags[p], 1u);
-      if (old == 0) break;
-      __threadfence();
-      float box[6];
-      for (int k = 0; k < 6; ++k) box[k] = (k < 3) ? kInfF : -kInfF;
-      int cs[2] = {left[p], right[p]};
-      for (int c = 0; c < 2; ++c) {
-        const float* src = cs[c] < 0 ? leafbox + (size_t)(~cs[c]) * 6 : innerbox + (size_t)cs[c] * 6;
-        for (int k = 0; k < 3; ++k) {
-          box[k] = fminf(box[k], ldCoherent(src + k));
-          box[3 + k] = fmaxf(box[3 + k], ldCoherent(src + 3 + k));
+__global__ void __launch_bounds__(256) kLeafScan(const float* __restrict__ leafbox, int n,
+                                                 float* __restrict__ pre, float* __restrict__ suf,
+                                                 float* __restrict__ blockbox) {
+  const int lane = threadIdx.x & 63;
+  const long long nblk = (n + kRangeBlock - 1) / kRangeBlock;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long b = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6; b < nblk; b += nw) {
+    const long long i = b * kRangeBlock + lane;
+    const bool in = i < n;
+    float plo[3], phi[3], slo[3], shi[3];
+    for (int k = 0; k < 3; ++k) {
+      plo[k] = slo[k] = in ? leafbox[i * 6 + k] : kInfF;
+      phi[k] = shi[k] = in ? leafbox[i * 6 + 3 + k] : -kInfF;
+    }
+    for (int o = 1; o < 64; o <<= 1)
+      for (int k = 0; k < 3; ++k) {
+        const float a = __shfl_up(plo[k], o, 64), b2 = __shfl_up(phi[k], o, 64);
+        const float c = __shfl_down(slo[k], o, 64), d = __shfl_down(shi[k], o, 64);
+        if (lane >= o) {
+          plo[k] = fminf(plo[k], a);
+          phi[k] = fmaxf(phi[k], b2);
+        }
+        if (lane + o < 64) {
+          slo[k] = fminf(slo[k], c);
+          shi[k] = fmaxf(shi[k], d);
         }
       }
-      for (int k = 0; k < 6; ++k) stCoherent(innerbox + (size_t)p * 6 + k, box[k]);
-      p = parent_inner[p];
+    if (in)
+      for (int k = 0; k < 3; ++k) {
+        pre[i * 6 + k] = plo[k];
+        pre[i * 6 + 3 + k] = phi[k];
+        suf[i * 6 + k] = slo[k];
+        suf[i * 6 + 3 + k] = shi[k];
+      }
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) {
+        blockbox[b * 6 + k] = slo[k];
+        blockbox[b * 6 + 3 + k] = shi[k];
+      }
+  }
+}
+
+// table level j from level j - 1: m entries, each the union of entries i
+// and i + half of the level below
+__global__ void kRangeLevel(const float* __restrict__ src, float* __restrict__ dst, long long m,
+                            long long half) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < m;
+       i += (long long)gridDim.x * blockDim.x)
+    for (int k = 0; k < 3; ++k) {
+      dst[i * 6 + k] = fminf(src[i * 6 + k], src[(i + half) * 6 + k]);
+      dst[i * 6 + 3 + k] = fmaxf(src[i * 6 + 3 + k], src[(i + half) * 6 + 3 + k]);
+    }
+}
+
+__global__ void kRangeBoxes(int ninner, const int* __restrict__ first, const int* __restrict__ last,
+                            const float* __restrict__ leafbox, const float* __restrict__ pre,
+                            const float* __restrict__ suf, const float* __restrict__ table,
+                            RangeLevels L, float* __restrict__ innerbox) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ninner; p += gridDim.x * blockDim.x) {
+    const int a = first[p], b = last[p];
+    const int ba = a / kRangeBlock, bb = b / kRangeBlock;
+    float lo[3] = {kInfF, kInfF, kInfF}, hi[3] = {-kInfF, -kInfF, -kInfF};
+    if (ba == bb) {
+      for (int i = a; i <= b; ++i) boxMerge(lo, hi, leafbox + (size_t)i * 6);
+    } else {
+      boxMerge(lo, hi, suf + (size_t)a * 6);
+      boxMerge(lo, hi, pre + (size_t)b * 6);
+      if (bb - ba >= 2) {
+        const int x = ba + 1, y = bb - 1;
+        const int j = 31 - __clz(y - x + 1);
+        const float* lv = table + L.off[j];
+        boxMerge(lo, hi, lv + (size_t)x * 6);
+        boxMerge(lo, hi, lv + (size_t)(y - (1 << j) + 1) * 6);
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      innerbox[(size_t)p * 6 + k] = lo[k];
+      innerbox[(size_t)p * 6 + 3 + k] = hi[k];
     }
   }
+}
+
+// Table levels for n leaves: offsets and total floats.
+static long long rangeLevels(int n, RangeLevels* L, int* nlev) {
+  const long long nblk = (n + kRangeBlock - 1) / kRangeBlock;
+  long long off = 0;
+  int j = 0;
+  for (long long w = 1; w <= nblk && j < 32; w <<= 1, ++j) {
+    L->off[j] = off;
+    off += (nblk - w + 1) * 6;
+  }
+  *nlev = j;
+  return off;
 }
 
 // Collapse the binary tree into 4-wide nodes, one frontier level per launch.
@@ -599,6 +697,43 @@ static int gridFor(long long n, int block) {
   return (int)g;
 }
 
+// Scratch of one domain build, sized for the largest domain.
+struct BuildScratch {
+  DevBuf keys, vals, keys2, vals2, hist, lb, ib, lft, rgt, fst, lst, pin, plf, fa, fb, cnts,
+      scal, adj, rpre, rsuf, rtab;
+  DevBuf pc, pc2, pb, pb2, pnn, pcnt, pmf, pkf, pmp, pkp, pperm, pord;  // PLOC
+  void reserve(int maxf, long long maxv, bool ploc) {
+    const size_t f4 = (size_t)maxf * 4 + 64;
+    for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &lft, &rgt, &fst, &lst, &pin, &plf})
+      b->reserve(f4);
+    hist.reserve((size_t)256 * ((maxf + kRsTile - 1) / kRsTile) * 4 + 64);
+    lb.reserve((size_t)maxf * 24 + 64);
+    ib.reserve((size_t)maxf * 24 + 64);
+    fa.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
+    fb.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
+    cnts.reserve((size_t)(maxv + 1) * 12 + 64);
+    scal.reserve(64);
+    adj.reserve((size_t)maxf * 12 + 64);
+    rpre.reserve((size_t)maxf * 24 + 64);
+    rsuf.reserve((size_t)maxf * 24 + 64);
+    RangeLevels L;
+    int nlev;
+    rtab.reserve((size_t)rangeLevels(std::max(maxf, 1), &L, &nlev) * 4 + 64);
+    if (ploc) {
+      for (DevBuf* b : {&pc, &pc2, &pnn, &pcnt, &pmf, &pkf, &pmp, &pkp, &pperm, &pord}) b->reserve(f4);
+      pb.reserve((size_t)maxf * 24 + 64);
+      pb2.reserve((size_t)maxf * 24 + 64);
+    }
+  }
+  ~BuildScratch() {
+    for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
+                      &plf, &fa, &fb, &cnts, &scal, &adj, &rpre, &rsuf, &rtab, &pc, &pc2,
+                      &pb, &pb2, &pnn, &pcnt,
+                      &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
+      b->release();
+  }
+};
+
 void HostMeshSource::fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) {
   (void)c;
   SPRAY_HIP(hipMemcpyAsync(v, m[d].v.data(), m[d].v.size() * 4, hipMemcpyHostToDevice, s));
@@ -693,39 +828,8 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     }
     src.check(c);
   }
-  // build scratch sized for the largest domain
-  int nblk = (maxf + kRsTile - 1) / kRsTile;
-  DevBuf keys, vals, keys2, vals2, hist, lb, ib, lft, rgt, fst, lst, pin, plf, flags, fa, fb, cnts,
-      scal;
-  keys.reserve((size_t)maxf * 4 + 64);
-  vals.reserve((size_t)maxf * 4 + 64);
-  keys2.reserve((size_t)maxf * 4 + 64);
-  vals2.reserve((size_t)maxf * 4 + 64);
-  hist.reserve((size_t)256 * nblk * 4 + 64);
-  lb.reserve((size_t)maxf * 24 + 64);
-  ib.reserve((size_t)maxf * 24 + 64);
-  lft.reserve((size_t)maxf * 4 + 64);
-  rgt.reserve((size_t)maxf * 4 + 64);
-  fst.reserve((size_t)maxf * 4 + 64);
-  lst.reserve((size_t)maxf * 4 + 64);
-  pin.reserve((size_t)maxf * 4 + 64);
-  plf.reserve((size_t)maxf * 4 + 64);
-  flags.reserve((size_t)maxf * 4 + 64);
-  fa.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
-  fb.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
-  cnts.reserve((size_t)(maxv + 1) * 12 + 64);
-  scal.reserve(64);
-  DevBuf adj;
-  adj.reserve((size_t)maxf * 12 + 64);
-  DevBuf pc, pc2, pb, pb2, pnn, pcnt, pmf, pkf, pmp, pkp, pperm, pord;
-  if (c->bvh_builder == 1) {
-    for (DevBuf* b : {&pc, &pc2, &pnn, &pcnt, &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
-      b->reserve((size_t)maxf * 4 + 64);
-    pb.reserve((size_t)maxf * 24 + 64);
-    pb2.reserve((size_t)maxf * 24 + 64);
-  }
-  P.bvh_depth.assign(nd, 0);
-  for (int d = 0; d < nd; ++d) {
+  // one domain's normals, shading records and BVH, issued on stream s
+  auto buildOne = [&](BuildScratch& S, int d, hipStream_t s) {
     DomainDesc& D = P.desc[d];
     const int nf = D.num_tris;
     const long long nv = src.numVerts(d);
@@ -739,7 +843,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     const float* vtx = P.vtx.as<float>() + D.vtx_off * 3;
     const uint32_t* face = P.face.as<uint32_t>() + D.face_off * 3;
     // normals
-    uint32_t* cnt = cnts.as<uint32_t>();
+    uint32_t* cnt = S.cnts.as<uint32_t>();
     uint32_t* off = cnt + (nv + 1);
     uint32_t* cur = off + (nv + 1);
     SPRAY_HIP(hipMemsetAsync(cnt, 0, (size_t)(nv + 1) * 4, s));
@@ -747,33 +851,34 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     kVertCount<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, cnt);
     exclusiveScan(cnt, off, nv + 1, s);
     kVertFill<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, off, cur,
-                                                              adj.as<uint32_t>());
-    kVertNormal<<<gridFor(nv, 256), 256, 0, s>>>(vtx, face, nv, off, cnt, adj.as<uint32_t>(),
+                                                              S.adj.as<uint32_t>());
+    kVertNormal<<<gridFor(nv, 256), 256, 0, s>>>(vtx, face, nv, off, cnt, S.adj.as<uint32_t>(),
                                                   P.nrm.as<float>() + D.vtx_off * 3);
     kFaceShade<<<gridFor(nf, 256), 256, 0, s>>>(face, nf, P.nrm.as<float>() + D.vtx_off * 3,
                                                  P.rgb.as<uint32_t>() + D.vtx_off,
                                                  P.fshade.as<float4>() + D.shade_off * 3);
     // Morton codes
-    uint32_t* bmm = scal.as<uint32_t>();
+    uint32_t* bmm = S.scal.as<uint32_t>();
     uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
     SPRAY_HIP(hipMemcpyAsync(bmm, init, 24, hipMemcpyHostToDevice, s));
-    kCentroidBounds<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, bmm, bmm + 3);
-    kMorton<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, bmm, bmm + 3, keys.as<uint32_t>(),
-                                             vals.as<uint32_t>());
+    kCentroidBounds<<<std::min(gridFor(nf, 256), kBoundsGrid), 256, 0, s>>>(vtx, face, nf, bmm,
+                                                                            bmm + 3);
+    kMorton<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, bmm, bmm + 3, S.keys.as<uint32_t>(),
+                                             S.vals.as<uint32_t>());
     // radix sort, 4 passes of 8 bits
     int nb = (nf + kRsTile - 1) / kRsTile;
-    uint32_t *ka = keys.as<uint32_t>(), *va = vals.as<uint32_t>(), *kb = keys2.as<uint32_t>(),
-             *vb = vals2.as<uint32_t>();
+    uint32_t *ka = S.keys.as<uint32_t>(), *va = S.vals.as<uint32_t>(), *kb = S.keys2.as<uint32_t>(),
+             *vb = S.vals2.as<uint32_t>();
     for (int pass = 0; pass < 4; ++pass) {
-      kRsHist<<<nb, kRsBlock, 0, s>>>(ka, nf, pass * 8, hist.as<uint32_t>());
-      exclusiveScan(hist.as<uint32_t>(), hist.as<uint32_t>(), (long long)256 * nb, s);
-      kRsScatter<<<nb, kRsBlock, 0, s>>>(ka, va, kb, vb, nf, pass * 8, hist.as<uint32_t>());
+      kRsHist<<<nb, kRsBlock, 0, s>>>(ka, nf, pass * 8, S.hist.as<uint32_t>());
+      exclusiveScan(S.hist.as<uint32_t>(), S.hist.as<uint32_t>(), (long long)256 * nb, s);
+      kRsScatter<<<nb, kRsBlock, 0, s>>>(ka, va, kb, vb, nf, pass * 8, S.hist.as<uint32_t>());
       std::swap(ka, kb);
       std::swap(va, vb);
     }
     LTri* tris = P.tris.as<LTri>() + D.tri_off;
     BNode4* nodes = P.nodes.as<BNode4>() + D.node_off;
-    float* leafbox = lb.as<float>();
+    float* leafbox = S.lb.as<float>();
     const int node_cap = nf / 2 + 16;
     // paged pool: the finished domain's image goes to pinned host memory
     // (and its normals to the host, for the normals query)
@@ -795,22 +900,22 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       D.num_nodes = 1;
       P.bvh_depth[d] = 1;
       pageOut();
-      continue;
+      return;
     }
-    int* L = lft.as<int>();
-    int* R = rgt.as<int>();
-    int* F = fst.as<int>();
-    int* Lst = lst.as<int>();
-    float* inner = ib.as<float>();
+    int* L = S.lft.as<int>();
+    int* R = S.rgt.as<int>();
+    int* F = S.fst.as<int>();
+    int* Lst = S.lst.as<int>();
+    float* inner = S.ib.as<float>();
     // Collapse the binary tree under `root` into 4-wide nodes; returns the
     // number of 4-wide levels.
     auto collapse = [&](int root) {
-      int* ctr = (int*)(scal.as<uint32_t>() + 8);  // [node_counter, nout, overflow]
+      int* ctr = (int*)(S.scal.as<uint32_t>() + 8);  // [node_counter, nout, overflow]
       int hinit[3] = {1, 0, 0};
       SPRAY_HIP(hipMemcpyAsync(ctr, hinit, 12, hipMemcpyHostToDevice, s));
       FrontierItem item{root, 0};
-      SPRAY_HIP(hipMemcpyAsync(fa.p, &item, sizeof(item), hipMemcpyHostToDevice, s));
-      FrontierItem *fin = fa.as<FrontierItem>(), *fout = fb.as<FrontierItem>();
+      SPRAY_HIP(hipMemcpyAsync(S.fa.p, &item, sizeof(item), hipMemcpyHostToDevice, s));
+      FrontierItem *fin = S.fa.as<FrontierItem>(), *fout = S.fb.as<FrontierItem>();
       int nin = 1, depth = 0;
       while (nin > 0) {
         SPRAY_HIP(hipMemsetAsync(ctr + 1, 0, 4, s));
@@ -832,16 +937,16 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     int depth = -1;
     if (c->bvh_builder == 1) {
       // PLOC passes until one cluster is left
-      int* cid = pc.as<int>();
-      int* cid2 = pc2.as<int>();
-      float* cbox = pb.as<float>();
-      float* cbox2 = pb2.as<float>();
-      int* nn = pnn.as<int>();
-      int* icount = pcnt.as<int>();
-      uint32_t* mflag = pmf.as<uint32_t>();
-      uint32_t* kflag = pkf.as<uint32_t>();
-      uint32_t* mpos = pmp.as<uint32_t>();
-      uint32_t* kpos = pkp.as<uint32_t>();
+      int* cid = S.pc.as<int>();
+      int* cid2 = S.pc2.as<int>();
+      float* cbox = S.pb.as<float>();
+      float* cbox2 = S.pb2.as<float>();
+      int* nn = S.pnn.as<int>();
+      int* icount = S.pcnt.as<int>();
+      uint32_t* mflag = S.pmf.as<uint32_t>();
+      uint32_t* kflag = S.pkf.as<uint32_t>();
+      uint32_t* mpos = S.pmp.as<uint32_t>();
+      uint32_t* kpos = S.pkp.as<uint32_t>();
       kPlocInit<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, cid, cbox);
       std::vector<int> pass_base{0};
       int n = nf, made = 0;
@@ -870,13 +975,13 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       if (ok) {
         // depth-first leaf order: roots of later passes first
         const int root = made - 1;
-        uint32_t* perm = pperm.as<uint32_t>();
+        uint32_t* perm = S.pperm.as<uint32_t>();
         SPRAY_HIP(hipMemsetAsync(F + root, 0, 4, s));
         for (int p = (int)pass_base.size() - 1; p > 0; --p) {
           int lo = pass_base[p - 1], hi = pass_base[p];
           kPlocRanges<<<gridFor(hi - lo, 256), 256, 0, s>>>(lo, hi, L, R, icount, F, Lst, perm);
         }
-        uint32_t* order = pord.as<uint32_t>();
+        uint32_t* order = S.pord.as<uint32_t>();
         kComposeOrder<<<gridFor(nf, 256), 256, 0, s>>>(perm, va, nf, order);
         kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, order, leafbox, tris);
         depth = collapse(root);
@@ -885,20 +990,38 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     }
     if (depth < 0) {
       // Karras LBVH: leaves in Morton order, bottom-up refit
-      int* PI = pin.as<int>();
-      int* PL = plf.as<int>();
+      int* PI = S.pin.as<int>();
+      int* PL = S.plf.as<int>();
       kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris);
       SPRAY_HIP(hipMemsetAsync(PI, 0xff, (size_t)nf * 4, s));  // root parent = -1
       kKarras<<<gridFor(nf, 256), 256, 0, s>>>(ka, nf, L, R, F, Lst, PI, PL);
-      SPRAY_HIP(hipMemsetAsync(flags.p, 0, (size_t)nf * 4, s));
-      kRefit<<<gridFor(nf, 256), 256, 0, s>>>(nf, L, R, PI, PL, leafbox, inner,
-                                              flags.as<uint32_t>());
+      // inner boxes as unions over leaf ranges (replaces a bottom-up refit)
+      RangeLevels RL;
+      int nlev;
+      rangeLevels(nf, &RL, &nlev);
+      const long long nblk = (nf + kRangeBlock - 1) / kRangeBlock;
+      float* tab = S.rtab.as<float>();
+      kLeafScan<<<gridFor(nblk * 64, 256), 256, 0, s>>>(leafbox, nf, S.rpre.as<float>(),
+                                                        S.rsuf.as<float>(), tab);
+      for (int j = 1; j < nlev; ++j) {
+        const long long m = nblk - (1ll << j) + 1;
+        kRangeLevel<<<gridFor(m, 256), 256, 0, s>>>(tab + RL.off[j - 1], tab + RL.off[j], m,
+                                                    1ll << (j - 1));
+      }
+      kRangeBoxes<<<gridFor(nf - 1, 256), 256, 0, s>>>(nf - 1, F, Lst, leafbox, S.rpre.as<float>(),
+                                                       S.rsuf.as<float>(), tab, RL, inner);
       depth = collapse(0);
       if (!fitsStack(depth))
         throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
     }
     P.bvh_depth[d] = depth;
     pageOut();
+  };
+  P.bvh_depth.assign(nd, 0);
+  {
+    BuildScratch S;
+    S.reserve(maxf, maxv, c->bvh_builder == 1);
+    for (int d = 0; d < nd; ++d) buildOne(S, d, s);
   }
   SPRAY_HIP(hipStreamSynchronize(s));
   long long used = 0;
@@ -917,10 +1040,6 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     P.d_desc.reserve(sizeof(DomainDesc) * nd);
     SPRAY_HIP(hipMemcpy(P.d_desc.p, P.desc.data(), sizeof(DomainDesc) * nd, hipMemcpyHostToDevice));
   }
-  for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
-                    &plf, &flags, &fa, &fb, &cnts, &scal, &adj, &pc, &pc2, &pb, &pb2, &pnn, &pcnt,
-                    &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
-    b->release();
   P.build_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 }

@@ -84,12 +84,18 @@ __global__ void kScanAdd(uint32_t* out, long long n, const uint32_t* sums) {
 void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s) {
   if (n <= 0) return;
   long long nb = (n + kScanTile - 1) / kScanTile;
-  static thread_local DevBuf sums;
-  sums.reserve(std::max<size_t>((size_t)nb * 4 + 64, (size_t)4 << 20));
-  kScanTiles<<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, sums.as<uint32_t>());
+  // per host thread (build workers scan concurrently); freed when the thread ends
+  struct Sums {
+    DevBuf b;
+    ~Sums() { b.release(); }
+  };
+  static thread_local Sums sums;
+  sums.b.reserve(std::max<size_t>((size_t)nb * 4 + 64, (size_t)4 << 20));
+  uint32_t* tile_sums = sums.b.as<uint32_t>();
+  kScanTiles<<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, tile_sums);
   if (nb > 1) {
-    kScanSums<<<1, 1024, 0, s>>>(sums.as<uint32_t>(), nb);
-    kScanAdd<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, sums.as<uint32_t>());
+    kScanSums<<<1, 1024, 0, s>>>(tile_sums, nb);
+    kScanAdd<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, tile_sums);
   }
   SPRAY_HIP(hipGetLastError());
 }
