@@ -437,49 +437,73 @@ __device__ __forceinline__ float boxArea(const float* b) {
   return dx * dy + dy * dz + dz * dx;
 }
 
-__global__ void kCollapse(const FrontierItem* __restrict__ fin, int nin, FrontierItem* fout,
-                          int* nout, int* node_counter, int node_cap, const int* __restrict__ left,
-                          const int* __restrict__ right, const int* __restrict__ first,
-                          const int* __restrict__ last, const float* __restrict__ leafbox,
-                          const float* __restrict__ innerbox, BNode4* nodes, int* overflow) {
-  for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < nin; it += gridDim.x * blockDim.x) {
-    FrontierItem f = fin[it];
-    int ch[4];
-    int nc = 2;
-    ch[0] = left[f.bnode];
-    ch[1] = right[f.bnode];
+// Takes cnt consecutive entries of *ctr per lane, in lane order, with one
+// atomic per wave; every lane of the wave calls it (cnt may be 0).
+__device__ __forceinline__ int waveAlloc(int* ctr, int cnt) {
+  const int lane = threadIdx.x & 63;
+  int incl = cnt;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  const int total = __shfl(incl, 63, 64);
+  int base = 0;
+  if (lane == 63 && total > 0) base = atomicAdd(ctr, total);
+  base = __shfl(base, 63, 64);
+  return base + incl - cnt;
+}
+
+// One frontier level: *nin items in fin (0: nothing to do), next level's
+// items counted in *nout.  Launched level after level without host round
+// trips; the loop bound is uniform per block so whole waves allocate.
+__global__ void kCollapse(const FrontierItem* __restrict__ fin, const int* __restrict__ nin_ptr,
+                          FrontierItem* fout, int* nout, int* node_counter, int node_cap,
+                          const int* __restrict__ left, const int* __restrict__ right,
+                          const int* __restrict__ first, const int* __restrict__ last,
+                          const float* __restrict__ leafbox, const float* __restrict__ innerbox,
+                          BNode4* nodes, int* overflow) {
+  const int nin = *nin_ptr;
+  for (int base = blockIdx.x * blockDim.x; base < nin; base += gridDim.x * blockDim.x) {
+    const int it = base + threadIdx.x;
+    const bool act = it < nin;
+    FrontierItem f{0, 0};
+    int ch[4] = {0, 0, 0, 0};
+    int nc = 0;
     auto count = [&](int c) { return c < 0 ? 1 : last[c] - first[c] + 1; };
-    while (nc < 4) {
-      int best = -1;
-      float barea = -1.f;
-      for (int i = 0; i < nc; ++i) {
-        if (ch[i] >= 0 && count(ch[i]) > kLeafMax) {
-          float a = boxArea(innerbox + (size_t)ch[i] * 6);
-          if (a > barea) {
-            barea = a;
-            best = i;
+    int ninner = 0;
+    if (act) {
+      f = fin[it];
+      nc = 2;
+      ch[0] = left[f.bnode];
+      ch[1] = right[f.bnode];
+      while (nc < 4) {
+        int best = -1;
+        float barea = -1.f;
+        for (int i = 0; i < nc; ++i) {
+          if (ch[i] >= 0 && count(ch[i]) > kLeafMax) {
+            float a = boxArea(innerbox + (size_t)ch[i] * 6);
+            if (a > barea) {
+              barea = a;
+              best = i;
+            }
           }
         }
+        if (best < 0) break;
+        int c = ch[best];
+        ch[best] = left[c];
+        ch[nc++] = right[c];
       }
-      if (best < 0) break;
-      int c = ch[best];
-      ch[best] = left[c];
-      ch[nc++] = right[c];
+      // the inner children of a node take consecutive slots (siblings share
+      // cache lines when a ray enters several of them)
+      for (int i = 0; i < nc; ++i) ninner += (ch[i] >= 0 && count(ch[i]) > kLeafMax) ? 1 : 0;
     }
-    // the inner children of a node take consecutive slots (siblings share
-    // cache lines when a ray enters several of them)
-    int ninner = 0;
-    for (int i = 0; i < nc; ++i) ninner += (ch[i] >= 0 && count(ch[i]) > kLeafMax) ? 1 : 0;
-    int slot = 0, q = 0;
-    if (ninner > 0) {
-      slot = atomicAdd(node_counter, ninner);
-      if (slot + ninner > node_cap) {
-        atomicExch(overflow, 1);
-        ninner = 0;
-      } else {
-        q = atomicAdd(nout, ninner);
-      }
+    int slot = waveAlloc(node_counter, ninner);
+    if (ninner > 0 && slot + ninner > node_cap) {
+      atomicExch(overflow, 1);
+      ninner = 0;
     }
+    int q = waveAlloc(nout, ninner);
+    if (!act) continue;
     BNode4 nd;
     for (int i = 0; i < 4; ++i) {
       float b[6];
@@ -698,9 +722,15 @@ static int gridFor(long long n, int block) {
 }
 
 // Scratch of one domain build, sized for the largest domain.
+// 4-wide collapse levels launched per host wait, and at most (deeper trees
+// do not fit the traversal stack anyway)
+constexpr int kLevelBatch = 16;
+constexpr int kMaxLevels = 48;
+constexpr int kLevelWords = 2 + kMaxLevels + 1;
+
 struct BuildScratch {
   DevBuf keys, vals, keys2, vals2, hist, lb, ib, lft, rgt, fst, lst, pin, plf, fa, fb, cnts,
-      scal, adj, rpre, rsuf, rtab;
+      scal, adj, rpre, rsuf, rtab, lvl;
   DevBuf pc, pc2, pb, pb2, pnn, pcnt, pmf, pkf, pmp, pkp, pperm, pord;  // PLOC
   void reserve(int maxf, long long maxv, bool ploc) {
     const size_t f4 = (size_t)maxf * 4 + 64;
@@ -713,6 +743,7 @@ struct BuildScratch {
     fb.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
     cnts.reserve((size_t)(maxv + 1) * 12 + 64);
     scal.reserve(64);
+    lvl.reserve(kLevelWords * 4 + 64);
     adj.reserve((size_t)maxf * 12 + 64);
     rpre.reserve((size_t)maxf * 24 + 64);
     rsuf.reserve((size_t)maxf * 24 + 64);
@@ -727,7 +758,7 @@ struct BuildScratch {
   }
   ~BuildScratch() {
     for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
-                      &plf, &fa, &fb, &cnts, &scal, &adj, &rpre, &rsuf, &rtab, &pc, &pc2,
+                      &plf, &fa, &fb, &cnts, &scal, &adj, &rpre, &rsuf, &rtab, &lvl, &pc, &pc2,
                       &pb, &pb2, &pnn, &pcnt,
                       &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
       b->release();
@@ -909,28 +940,40 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     float* inner = S.ib.as<float>();
     // Collapse the binary tree under `root` into 4-wide nodes; returns the
     // number of 4-wide levels.
+    // Levels are launched kLevelBatch at a time; each reads its item count
+    // from the device, so the host waits once per batch (a level past the
+    // last finds 0 items).  Returns the number of levels with items
+    // (kMaxLevels + 1 when deeper than kMaxLevels).
     auto collapse = [&](int root) {
-      int* ctr = (int*)(S.scal.as<uint32_t>() + 8);  // [node_counter, nout, overflow]
-      int hinit[3] = {1, 0, 0};
-      SPRAY_HIP(hipMemcpyAsync(ctr, hinit, 12, hipMemcpyHostToDevice, s));
+      int* cc = S.lvl.as<int>();  // [0] node counter, [1] overflow, [2 + l] items of level l
+      int init[kLevelWords] = {};
+      init[0] = 1;
+      init[2] = 1;
+      SPRAY_HIP(hipMemcpyAsync(cc, init, sizeof(init), hipMemcpyHostToDevice, s));
       FrontierItem item{root, 0};
       SPRAY_HIP(hipMemcpyAsync(S.fa.p, &item, sizeof(item), hipMemcpyHostToDevice, s));
-      FrontierItem *fin = S.fa.as<FrontierItem>(), *fout = S.fb.as<FrontierItem>();
-      int nin = 1, depth = 0;
-      while (nin > 0) {
-        SPRAY_HIP(hipMemsetAsync(ctr + 1, 0, 4, s));
-        kCollapse<<<gridFor(nin, 256), 256, 0, s>>>(fin, nin, fout, ctr + 1, ctr, node_cap, L, R,
-                                                     F, Lst, leafbox, inner, nodes, ctr + 2);
-        int h[3];
-        SPRAY_HIP(hipMemcpyAsync(h, ctr, 12, hipMemcpyDeviceToHost, s));
+      const int grid = std::min(gridFor(std::max(1, nf / 8), 256), 2048);
+      int done = 0;
+      for (;;) {
+        const int upto = std::min(done + kLevelBatch, kMaxLevels);
+        for (int l = done; l < upto; ++l) {
+          FrontierItem* fin = (l & 1) ? S.fb.as<FrontierItem>() : S.fa.as<FrontierItem>();
+          FrontierItem* fout = (l & 1) ? S.fa.as<FrontierItem>() : S.fb.as<FrontierItem>();
+          kCollapse<<<grid, 256, 0, s>>>(fin, cc + 2 + l, fout, cc + 3 + l, cc, node_cap, L, R, F,
+                                         Lst, leafbox, inner, nodes, cc + 1);
+        }
+        done = upto;
+        int h[kLevelWords];
+        SPRAY_HIP(hipMemcpyAsync(h, cc, sizeof(h), hipMemcpyDeviceToHost, s));
         SPRAY_HIP(hipStreamSynchronize(s));
-        if (h[2]) throw StatusError(SPRAY_ERR_CAPACITY, "BVH node capacity exceeded");
-        nin = h[1];
+        if (h[1]) throw StatusError(SPRAY_ERR_CAPACITY, "BVH node capacity exceeded");
         D.num_nodes = h[0];
-        std::swap(fin, fout);
-        ++depth;
+        if (h[2 + done] == 0 || done == kMaxLevels) {
+          int depth = 0;
+          while (depth < done && h[2 + depth] > 0) ++depth;
+          return h[2 + done] > 0 ? kMaxLevels + 1 : depth;
+        }
       }
-      return depth;
     };
     // worst-case stack need: three pushes per 4-wide level
     auto fitsStack = [](int depth) { return 3 * depth + 1 <= kRing + kOverflow; };
