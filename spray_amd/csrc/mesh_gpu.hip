@@ -19,9 +19,10 @@
 //
 // The generator reproduces scenegen.cc's CPU meshes in the same vertex and
 // face order: field values in double at the grid points, one vertex per
-// sign-changing grid edge numbered by an exclusive scan over the edges in
-// (axis, k, j, i) order, then the case table's triangles per cell numbered
-// by a scan over the cells in (k, j, i) order.  Every double operation is
+// sign-changing grid edge numbered in (axis, k, j, i) edge order (a cut
+// bitmask, per-block counts of 256 edges scanned, a popcount rank inside the
+// block), then the case table's triangles per cell numbered in (k, j, i)
+// cell order (per-block counts scanned, an LDS scan inside the block).  Every double operation is
 // the CPU's, in the CPU's order (no contraction), so meshes agree
 // bit-for-bit wherever the device and host sin agree in the last place.
 #include <hip/hip_runtime.h>
@@ -414,42 +415,73 @@ __device__ __forceinline__ void mcVertex(const McGrid& g, int axis, int i, int j
   for (int c = 0; c < 3; ++c) out[c] = (float)pos[c];
 }
 
-// flag[e] = 1 where the field changes sign along edge e (flag[3E] = 0); with
-// `part`, also the per-block bounds of the edge vertices.
-__global__ void __launch_bounds__(kBlk) kMcEdges(McGrid g, const double* __restrict__ f,
-                                                 uint32_t* __restrict__ flag, float* part) {
+// Edge cuts as bits: bit e of the mask is set where the field changes sign
+// along edge e; edges go in blocks of 256 (four 64-bit words), bcnt[blk] =
+// cuts in the block (bcnt[nblk] = 0, for the exclusive scan of the block
+// counts); with `part`, also the per-block bounds of the edge vertices.
+__global__ void __launch_bounds__(kBlk) kMcEdgeBits(McGrid g, const double* __restrict__ f,
+                                                    unsigned long long* __restrict__ bits,
+                                                    uint32_t* __restrict__ bcnt, float* part) {
+  __shared__ uint32_t wsum[kBlk / 64];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  const long long ne = 3 * g.E;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ne;
-       e += (long long)gridDim.x * blockDim.x) {
-    int axis, i, j, k;
-    mcEdgeDecode(g, e, &axis, &i, &j, &k);
-    const double f0 = mcF(f, g, i, j, k);
-    const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
-    const bool cut = (f0 < 0.0) != (f1 < 0.0);
-    flag[e] = cut ? 1u : 0u;
-    if (part && cut) {
-      float p[3];
-      mcVertex(g, axis, i, j, k, f0, f1, p);
-      for (int c = 0; c < 3; ++c) {
-        lo[c] = fminf(lo[c], p[c]);
-        hi[c] = fmaxf(hi[c], p[c]);
+  const long long ne = 3 * g.E, nblk = (ne + kBlk - 1) / kBlk;
+  const int w = threadIdx.x >> 6;
+  for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long long e = blk * kBlk + threadIdx.x;
+    bool cut = false;
+    if (e < ne) {
+      int axis, i, j, k;
+      mcEdgeDecode(g, e, &axis, &i, &j, &k);
+      const double f0 = mcF(f, g, i, j, k);
+      const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
+      cut = (f0 < 0.0) != (f1 < 0.0);
+      if (part && cut) {
+        float p[3];
+        mcVertex(g, axis, i, j, k, f0, f1, p);
+        for (int c = 0; c < 3; ++c) {
+          lo[c] = fminf(lo[c], p[c]);
+          hi[c] = fmaxf(hi[c], p[c]);
+        }
       }
     }
+    const unsigned long long m = __ballot(cut);
+    if ((threadIdx.x & 63) == 0) {
+      bits[blk * (kBlk / 64) + w] = m;
+      wsum[w] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int q = 0; q < kBlk / 64; ++q) t += wsum[q];
+      bcnt[blk] = t;
+    }
+    __syncthreads();
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) flag[ne] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) bcnt[nblk] = 0u;
   if (part) blockBounds(lo, hi, part);
 }
 
+// Vertex id of a cut edge: its block's offset plus the cuts before it in
+// the block.
+__device__ __forceinline__ uint32_t mcEdgeId(const unsigned long long* __restrict__ bits,
+                                             const uint32_t* __restrict__ eoff, long long e) {
+  const long long blk = e / kBlk;
+  const int w = (int)((e >> 6) & (kBlk / 64 - 1)), b = (int)(e & 63);
+  uint32_t id = eoff[blk];
+  for (int q = 0; q < w; ++q) id += (uint32_t)__popcll(bits[blk * (kBlk / 64) + q]);
+  return id + (uint32_t)__popcll(bits[blk * (kBlk / 64) + w] & ((1ull << b) - 1ull));
+}
+
 __global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g, const double* __restrict__ f,
-                                                 const uint32_t* __restrict__ off, uint32_t color,
+                                                 const unsigned long long* __restrict__ bits,
+                                                 const uint32_t* __restrict__ eoff, uint32_t color,
                                                  float* __restrict__ v,
                                                  uint32_t* __restrict__ rgb) {
   const long long ne = 3 * g.E;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ne;
        e += (long long)gridDim.x * blockDim.x) {
-    const uint32_t id = off[e];
-    if (off[e + 1] == id) continue;
+    if (!((bits[e >> 6] >> (e & 63)) & 1ull)) continue;
+    const uint32_t id = mcEdgeId(bits, eoff, e);
     int axis, i, j, k;
     mcEdgeDecode(g, e, &axis, &i, &j, &k);
     const double f0 = mcF(f, g, i, j, k);
@@ -468,38 +500,79 @@ __device__ __forceinline__ int mcCase(const McGrid& g, const double* f, int i, i
   return cs;
 }
 
-__global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const double* __restrict__ f,
-                                                     const uint8_t* __restrict__ ntri,
-                                                     uint32_t* __restrict__ cnt) {
-  const long long nc = (long long)g.n * g.n * g.n;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nc;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % g.n), j = (int)((t / g.n) % g.n), k = (int)(t / ((long long)g.n * g.n));
-    cnt[t] = ntri[mcCase(g, f, i, j, k)];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nc] = 0u;
+__device__ __forceinline__ void mcCellDecode(const McGrid& g, long long t, int* i, int* j, int* k) {
+  *i = (int)(t % g.n);
+  *j = (int)((t / g.n) % g.n);
+  *k = (int)(t / ((long long)g.n * g.n));
 }
 
+// Triangles per block of 256 cells (bcnt[nblk] = 0).
+__global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const double* __restrict__ f,
+                                                     const uint8_t* __restrict__ ntri,
+                                                     uint32_t* __restrict__ bcnt) {
+  __shared__ uint32_t wsum[kBlk / 64];
+  const long long nc = (long long)g.n * g.n * g.n, nblk = (nc + kBlk - 1) / kBlk;
+  for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long long t = blk * kBlk + threadIdx.x;
+    uint32_t n = 0;
+    if (t < nc) {
+      int i, j, k;
+      mcCellDecode(g, t, &i, &j, &k);
+      n = ntri[mcCase(g, f, i, j, k)];
+    }
+    n = waveSumU(n);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t s = 0;
+      for (int q = 0; q < kBlk / 64; ++q) s += wsum[q];
+      bcnt[blk] = s;
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) bcnt[nblk] = 0u;
+}
+
+// Triangles of every cell, in cell order: the block's offset plus an
+// exclusive scan of the per-cell counts inside the block.
 __global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const double* __restrict__ f,
                                                  const int8_t* __restrict__ table, McCorners ec,
+                                                 const unsigned long long* __restrict__ bits,
                                                  const uint32_t* __restrict__ eoff,
                                                  const uint32_t* __restrict__ coff,
                                                  uint32_t* __restrict__ out) {
-  const long long nc = (long long)g.n * g.n * g.n;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nc;
-       t += (long long)gridDim.x * blockDim.x) {
-    const uint32_t base = coff[t];
-    if (coff[t + 1] == base) continue;
-    const int i = (int)(t % g.n), j = (int)((t / g.n) % g.n), k = (int)(t / ((long long)g.n * g.n));
-    const int8_t* tr = table + mcCase(g, f, i, j, k) * 25;
-    for (int q = 0; tr[q] >= 0; q += 3)
+  __shared__ uint32_t wsum[kBlk / 64];
+  const long long nc = (long long)g.n * g.n * g.n, nblk = (nc + kBlk - 1) / kBlk;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long long t = blk * kBlk + threadIdx.x;
+    int i = 0, j = 0, k = 0, cs = 0;
+    uint32_t n = 0;
+    if (t < nc) {
+      mcCellDecode(g, t, &i, &j, &k);
+      cs = mcCase(g, f, i, j, k);
+      const int8_t* tr = table + cs * 25;
+      while (tr[3 * n] >= 0) ++n;
+    }
+    uint32_t incl = n;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t x = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += x;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = coff[blk] + incl - n;
+    for (int q = 0; q < w; ++q) base += wsum[q];
+    const int8_t* tr = table + cs * 25;
+    for (uint32_t q = 0; q < n; ++q)
       for (int r = 0; r < 3; ++r) {
-        const int e = tr[q + r];
+        const int e = tr[3 * q + r];
         const int ax = ec.axis[e];
         const long long idx =
             ax * g.E + mcEdgeIndex(g, ax, i + ec.dx[e], j + ec.dy[e], k + ec.dz[e]);
-        out[((size_t)base + q / 3) * 3 + r] = eoff[idx];
+        out[((size_t)base + q) * 3 + r] = mcEdgeId(bits, eoff, idx);
       }
+    __syncthreads();  // wsum is rewritten by the next block of cells
   }
 }
 
@@ -514,7 +587,7 @@ struct McSource : MeshSource {
   std::vector<McGrid> grid;
   std::vector<long long> nv, nf;
   std::vector<float> bounds;  // 6 per domain: lo xyz, hi xyz
-  DevBuf field, eflag, ccnt, table, ntri, part, dev_out;
+  DevBuf field, ebits, ecnt, ccnt, table, ntri, part, dev_out;
   McCorners ec;
   McSource(Ctx* c, const spray_scenegen_params& prm) : p(prm) {
     nd = p.nx * p.ny * p.nz;
@@ -542,8 +615,10 @@ struct McSource : MeshSource {
     if (max_e >= (1ll << 32) || max_c >= (1ll << 32))
       throw StatusError(SPRAY_ERR_CAPACITY, "scenegen: grid too fine for 32-bit numbering");
     field.reserve((size_t)max_pts * 8 + 64);
-    eflag.reserve((size_t)max_e * 4 + 64);
-    ccnt.reserve((size_t)max_c * 4 + 64);
+    const long long eblk = (max_e + kBlk - 1) / kBlk, cblk = (max_c + kBlk - 1) / kBlk;
+    ebits.reserve((size_t)eblk * (kBlk / 64) * 8 + 64);
+    ecnt.reserve((size_t)(eblk + 1) * 4 + 64);
+    ccnt.reserve((size_t)(cblk + 1) * 4 + 64);
     const int8_t* T = mcTriTable();
     uint8_t counts[256];
     for (int cs = 0; cs < 256; ++cs) {
@@ -566,14 +641,16 @@ struct McSource : MeshSource {
       const McGrid& g = grid[d];
       const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
       kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>());
-      kMcEdges<<<gridOf(ne, kBlk, kBoundBlocks), kBlk, 0, s>>>(g, field.as<double>(),
-                                                              eflag.as<uint32_t>(), part.as<float>());
-      kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), gridOf(ne, kBlk, kBoundBlocks), bnd + 6 * d);
-      exclusiveScan(eflag.as<uint32_t>(), eflag.as<uint32_t>(), ne + 1, s);
+      const long long eb = (ne + kBlk - 1) / kBlk, cb = (nc + kBlk - 1) / kBlk;
+      const int egrid = gridOf(ne, kBlk, kBoundBlocks);
+      kMcEdgeBits<<<egrid, kBlk, 0, s>>>(g, field.as<double>(), ebits.as<unsigned long long>(),
+                                         ecnt.as<uint32_t>(), part.as<float>());
+      kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), egrid, bnd + 6 * d);
+      exclusiveScan(ecnt.as<uint32_t>(), ecnt.as<uint32_t>(), eb + 1, s);
       kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), ntri.as<uint8_t>(),
                                                     ccnt.as<uint32_t>());
-      exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), nc + 1, s);
-      kMcTotals<<<1, 1, 0, s>>>(eflag.as<uint32_t>() + ne, ccnt.as<uint32_t>() + nc, tot + 2 * d);
+      exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), cb + 1, s);
+      kMcTotals<<<1, 1, 0, s>>>(ecnt.as<uint32_t>() + eb, ccnt.as<uint32_t>() + cb, tot + 2 * d);
     }
     SPRAY_HIP(hipGetLastError());
     std::vector<uint32_t> h((size_t)nd * 2);
@@ -589,7 +666,7 @@ struct McSource : MeshSource {
     }
   }
   ~McSource() override {
-    for (DevBuf* b : {&field, &eflag, &ccnt, &table, &ntri, &part, &dev_out}) b->release();
+    for (DevBuf* b : {&field, &ebits, &ecnt, &ccnt, &table, &ntri, &part, &dev_out}) b->release();
   }
   long long numVerts(int d) const override { return nv[d]; }
   long long numFaces(int d) const override { return nf[d]; }
@@ -598,15 +675,18 @@ struct McSource : MeshSource {
     const McGrid& g = grid[d];
     const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
     kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>());
-    kMcEdges<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), eflag.as<uint32_t>(), nullptr);
-    exclusiveScan(eflag.as<uint32_t>(), eflag.as<uint32_t>(), ne + 1, s);
-    kMcVerts<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), eflag.as<uint32_t>(), p.rgb,
-                                              v, rgb);
+    const long long eb = (ne + kBlk - 1) / kBlk, cb = (nc + kBlk - 1) / kBlk;
+    unsigned long long* bits = ebits.as<unsigned long long>();
+    kMcEdgeBits<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), bits, ecnt.as<uint32_t>(),
+                                                  nullptr);
+    exclusiveScan(ecnt.as<uint32_t>(), ecnt.as<uint32_t>(), eb + 1, s);
+    kMcVerts<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), bits, ecnt.as<uint32_t>(),
+                                              p.rgb, v, rgb);
     kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), ntri.as<uint8_t>(),
                                                   ccnt.as<uint32_t>());
-    exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), nc + 1, s);
-    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), table.as<int8_t>(), ec,
-                                              eflag.as<uint32_t>(), ccnt.as<uint32_t>(), f);
+    exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), cb + 1, s);
+    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), table.as<int8_t>(), ec, bits,
+                                              ecnt.as<uint32_t>(), ccnt.as<uint32_t>(), f);
     SPRAY_HIP(hipGetLastError());
   }
 };
