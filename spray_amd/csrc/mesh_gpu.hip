@@ -395,16 +395,25 @@ __device__ __forceinline__ void mcEdgeDecode(const McGrid& g, long long e, int* 
   *k = (int)(r / nj);
 }
 
-__global__ void __launch_bounds__(kBlk) kMcField(McGrid g, double* __restrict__ f) {
+// Field values, and their signs as bytes (1: inside, f < 0) for the cut
+// and case tests, which then read 1 byte per grid point instead of 8.
+__global__ void __launch_bounds__(kBlk) kMcField(McGrid g, double* __restrict__ f,
+                                                 uint8_t* __restrict__ sgn) {
   const long long np = (long long)g.N * g.N * g.N;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < np;
        t += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(t % g.N), j = (int)((t / g.N) % g.N), k = (int)(t / ((long long)g.N * g.N));
     const double x = g.ox + (double)i * g.h, y = g.oy + (double)j * g.h,
                  z = g.oz + (double)k * g.h;
-    f[t] = sin(g.a * x + g.phase) * sin(g.a * y) * sin(g.a * z) +
-           0.3 * sin(g.b * (x + y + z) + g.phase);
+    const double v = sin(g.a * x + g.phase) * sin(g.a * y) * sin(g.a * z) +
+                     0.3 * sin(g.b * (x + y + z) + g.phase);
+    f[t] = v;
+    sgn[t] = v < 0.0 ? 1 : 0;
   }
+}
+
+__device__ __forceinline__ int mcS(const uint8_t* sgn, const McGrid& g, int i, int j, int k) {
+  return sgn[((long long)k * g.N + j) * g.N + i];
 }
 
 __device__ __forceinline__ void mcVertex(const McGrid& g, int axis, int i, int j, int k,
@@ -420,6 +429,7 @@ __device__ __forceinline__ void mcVertex(const McGrid& g, int axis, int i, int j
 // cuts in the block (bcnt[nblk] = 0, for the exclusive scan of the block
 // counts); with `part`, also the per-block bounds of the edge vertices.
 __global__ void __launch_bounds__(kBlk) kMcEdgeBits(McGrid g, const double* __restrict__ f,
+                                                    const uint8_t* __restrict__ sgn,
                                                     unsigned long long* __restrict__ bits,
                                                     uint32_t* __restrict__ bcnt, float* part) {
   __shared__ uint32_t wsum[kBlk / 64];
@@ -432,10 +442,10 @@ __global__ void __launch_bounds__(kBlk) kMcEdgeBits(McGrid g, const double* __re
     if (e < ne) {
       int axis, i, j, k;
       mcEdgeDecode(g, e, &axis, &i, &j, &k);
-      const double f0 = mcF(f, g, i, j, k);
-      const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
-      cut = (f0 < 0.0) != (f1 < 0.0);
+      cut = mcS(sgn, g, i, j, k) != mcS(sgn, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
       if (part && cut) {
+        const double f0 = mcF(f, g, i, j, k);
+        const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
         float p[3];
         mcVertex(g, axis, i, j, k, f0, f1, p);
         for (int c = 0; c < 3; ++c) {
@@ -493,10 +503,9 @@ __global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g, const double* __restr
   }
 }
 
-__device__ __forceinline__ int mcCase(const McGrid& g, const double* f, int i, int j, int k) {
+__device__ __forceinline__ int mcCase(const McGrid& g, const uint8_t* sgn, int i, int j, int k) {
   int cs = 0;
-  for (int c = 0; c < 8; ++c)
-    if (mcF(f, g, i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) < 0.0) cs |= 1 << c;
+  for (int c = 0; c < 8; ++c) cs |= mcS(sgn, g, i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) << c;
   return cs;
 }
 
@@ -507,7 +516,7 @@ __device__ __forceinline__ void mcCellDecode(const McGrid& g, long long t, int* 
 }
 
 // Triangles per block of 256 cells (bcnt[nblk] = 0).
-__global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const double* __restrict__ f,
+__global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const uint8_t* __restrict__ sgn,
                                                      const uint8_t* __restrict__ ntri,
                                                      uint32_t* __restrict__ bcnt) {
   __shared__ uint32_t wsum[kBlk / 64];
@@ -518,7 +527,7 @@ __global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const double* __r
     if (t < nc) {
       int i, j, k;
       mcCellDecode(g, t, &i, &j, &k);
-      n = ntri[mcCase(g, f, i, j, k)];
+      n = ntri[mcCase(g, sgn, i, j, k)];
     }
     n = waveSumU(n);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
@@ -535,7 +544,7 @@ __global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const double* __r
 
 // Triangles of every cell, in cell order: the block's offset plus an
 // exclusive scan of the per-cell counts inside the block.
-__global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const double* __restrict__ f,
+__global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const uint8_t* __restrict__ sgn,
                                                  const int8_t* __restrict__ table, McCorners ec,
                                                  const unsigned long long* __restrict__ bits,
                                                  const uint32_t* __restrict__ eoff,
@@ -550,7 +559,7 @@ __global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const double* __restr
     uint32_t n = 0;
     if (t < nc) {
       mcCellDecode(g, t, &i, &j, &k);
-      cs = mcCase(g, f, i, j, k);
+      cs = mcCase(g, sgn, i, j, k);
       const int8_t* tr = table + cs * 25;
       while (tr[3 * n] >= 0) ++n;
     }
@@ -587,7 +596,7 @@ struct McSource : MeshSource {
   std::vector<McGrid> grid;
   std::vector<long long> nv, nf;
   std::vector<float> bounds;  // 6 per domain: lo xyz, hi xyz
-  DevBuf field, ebits, ecnt, ccnt, table, ntri, part, dev_out;
+  DevBuf field, sign, ebits, ecnt, ccnt, table, ntri, part, dev_out;
   McCorners ec;
   McSource(Ctx* c, const spray_scenegen_params& prm) : p(prm) {
     nd = p.nx * p.ny * p.nz;
@@ -615,6 +624,7 @@ struct McSource : MeshSource {
     if (max_e >= (1ll << 32) || max_c >= (1ll << 32))
       throw StatusError(SPRAY_ERR_CAPACITY, "scenegen: grid too fine for 32-bit numbering");
     field.reserve((size_t)max_pts * 8 + 64);
+    sign.reserve((size_t)max_pts + 64);
     const long long eblk = (max_e + kBlk - 1) / kBlk, cblk = (max_c + kBlk - 1) / kBlk;
     ebits.reserve((size_t)eblk * (kBlk / 64) * 8 + 64);
     ecnt.reserve((size_t)(eblk + 1) * 4 + 64);
@@ -640,14 +650,15 @@ struct McSource : MeshSource {
     for (int d = 0; d < nd; ++d) {
       const McGrid& g = grid[d];
       const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
-      kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>());
+      kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>());
       const long long eb = (ne + kBlk - 1) / kBlk, cb = (nc + kBlk - 1) / kBlk;
       const int egrid = gridOf(ne, kBlk, kBoundBlocks);
-      kMcEdgeBits<<<egrid, kBlk, 0, s>>>(g, field.as<double>(), ebits.as<unsigned long long>(),
+      kMcEdgeBits<<<egrid, kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>(),
+                                         ebits.as<unsigned long long>(),
                                          ecnt.as<uint32_t>(), part.as<float>());
       kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), egrid, bnd + 6 * d);
       exclusiveScan(ecnt.as<uint32_t>(), ecnt.as<uint32_t>(), eb + 1, s);
-      kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), ntri.as<uint8_t>(),
+      kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sign.as<uint8_t>(), ntri.as<uint8_t>(),
                                                     ccnt.as<uint32_t>());
       exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), cb + 1, s);
       kMcTotals<<<1, 1, 0, s>>>(ecnt.as<uint32_t>() + eb, ccnt.as<uint32_t>() + cb, tot + 2 * d);
@@ -666,7 +677,8 @@ struct McSource : MeshSource {
     }
   }
   ~McSource() override {
-    for (DevBuf* b : {&field, &ebits, &ecnt, &ccnt, &table, &ntri, &part, &dev_out}) b->release();
+    for (DevBuf* b : {&field, &sign, &ebits, &ecnt, &ccnt, &table, &ntri, &part, &dev_out})
+      b->release();
   }
   long long numVerts(int d) const override { return nv[d]; }
   long long numFaces(int d) const override { return nf[d]; }
@@ -674,18 +686,18 @@ struct McSource : MeshSource {
     (void)c;
     const McGrid& g = grid[d];
     const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
-    kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>());
+    kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>());
     const long long eb = (ne + kBlk - 1) / kBlk, cb = (nc + kBlk - 1) / kBlk;
     unsigned long long* bits = ebits.as<unsigned long long>();
-    kMcEdgeBits<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), bits, ecnt.as<uint32_t>(),
-                                                  nullptr);
+    kMcEdgeBits<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>(), bits,
+                                                  ecnt.as<uint32_t>(), nullptr);
     exclusiveScan(ecnt.as<uint32_t>(), ecnt.as<uint32_t>(), eb + 1, s);
     kMcVerts<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), bits, ecnt.as<uint32_t>(),
                                               p.rgb, v, rgb);
-    kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), ntri.as<uint8_t>(),
+    kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sign.as<uint8_t>(), ntri.as<uint8_t>(),
                                                   ccnt.as<uint32_t>());
     exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), cb + 1, s);
-    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, field.as<double>(), table.as<int8_t>(), ec, bits,
+    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sign.as<uint8_t>(), table.as<int8_t>(), ec, bits,
                                               ecnt.as<uint32_t>(), ccnt.as<uint32_t>(), f);
     SPRAY_HIP(hipGetLastError());
   }
