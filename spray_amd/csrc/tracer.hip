@@ -35,17 +35,24 @@ constexpr int kBlock = 256;
 constexpr int kQueueGrid = 1024;
 constexpr int kShadeRec = 5;  // float4s per shade record
 constexpr int kDomGroup = 8;  // domains per culling group of the domain-list pass
+constexpr int kGroupsPerSuper = 8;  // culling groups per supergroup (64 domains)
+__host__ __device__ inline int numGroups(int nd) { return (nd + kDomGroup - 1) / kDomGroup; }
+__host__ __device__ inline int numSupers(int nd) {
+  return (numGroups(nd) + kGroupsPerSuper - 1) / kGroupsPerSuper;
+}
 
-// kCount's dynamic LDS (byte offsets): domain boxes, group boxes, counts,
-// scores, the block's hit-mask words ([word][thread]), domain permutation.
+// kCount's dynamic LDS (byte offsets): domain boxes, group boxes,
+// supergroup boxes, counts, scores, the block's hit-mask words
+// ([word][thread]), domain permutation.
 struct CountLds {
-  size_t gbox, lc, ls, mask, perm, total;
+  size_t gbox, sbox, lc, ls, mask, perm, total;
 };
 __host__ __device__ inline CountLds countLds(int nd, int block) {
-  const int ng = (nd + kDomGroup - 1) / kDomGroup, nw = (nd + 63) / 64;
+  const int ng = numGroups(nd), ns = numSupers(nd), nw = (nd + 63) / 64;
   CountLds L;
   L.gbox = (size_t)nd * 32;
-  L.lc = L.gbox + (size_t)ng * 32;
+  L.sbox = L.gbox + (size_t)ng * 32;
+  L.lc = L.sbox + (size_t)ns * 32;
   L.ls = L.lc + (size_t)nd * 8;
   L.mask = (L.ls + (size_t)nd * 4 + 7) & ~(size_t)7;
   L.perm = L.mask + (size_t)nw * block * 8;
@@ -227,51 +234,66 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
 // entries sorted (registers), the domains hit as a bit mask (one word per 64
 // domains, this lane's words at mask[w * stride]) so that the scatter
 // re-tests only those, and the number of domains hit.  Domains are visited
-// in groups of kDomGroup (Morton order of their centres); a group's members
-// are tested only when some lane of the wave hits the group's box.  The
-// member tests are the exact ones, so the list does not depend on grouping.
+// in groups of kDomGroup (Morton order of their centres), groups in
+// supergroups of kGroupsPerSuper: a supergroup's groups are tested when some
+// lane of the wave hits its box, and each lane then tests the members of
+// only the groups it hits itself.  The member tests are the exact ones and
+// the list is kept sorted by (t, id), so the result does not depend on the
+// grouping or on the order the groups are visited in.
 __device__ __forceinline__ int rayDomains(const float4* box, const float4* gbox,
-                                          const uint16_t* perm, int nd, const float o[3],
-                                          const float inv[3], bool safe, int spawn, uint32_t* lc_kind,
-                                          float tt[kTopDepths], int ti[kTopDepths], int* kept_o,
+                                          const float4* sbox, const uint16_t* perm, int nd,
+                                          const float o[3], const float inv[3], bool safe,
+                                          int spawn, uint32_t* lc_kind, float tt[kTopDepths],
+                                          int ti[kTopDepths], int* kept_o,
                                           unsigned long long* mask, int stride) {
   int kept = 0, hits = 0;
-  const int ng = (nd + kDomGroup - 1) / kDomGroup;
-  for (int g = 0; g < ng; ++g) {
-    if (__ballot(slabBoxClosed(gbox[2 * g], gbox[2 * g + 1], o, inv, safe)) == 0ull) continue;
-    const int k1 = min(nd, (g + 1) * kDomGroup);
-    for (int k = g * kDomGroup; k < k1; ++k) {
-      const int dd = perm[k];  // wave-uniform
-      float tmin;
-      // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
-      // spawning domain was tested right after shading
-      if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
-      ++hits;
-      atomicAdd(&lc_kind[dd], 1u);
-      mask[(dd >> 6) * stride] |= 1ull << (dd & 63);
-      // insertion into the (t, id)-sorted top list (unrolled: stays in
-      // registers); steps past every active lane's list end are skipped
-      if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
-        float ct = tmin;
-        int ci = dd;
+  const int ng = numGroups(nd), ns = numSupers(nd);
+  for (int sg = 0; sg < ns; ++sg) {
+    const bool hs = slabBoxClosed(sbox[2 * sg], sbox[2 * sg + 1], o, inv, safe);
+    if (__ballot(hs) == 0ull) continue;
+    const int g0 = sg * kGroupsPerSuper, g1 = min(ng, g0 + kGroupsPerSuper);
+    uint32_t gm = 0;
+    if (hs)
+      for (int g = g0; g < g1; ++g)
+        if (slabBoxClosed(gbox[2 * g], gbox[2 * g + 1], o, inv, safe)) gm |= 1u << (g - g0);
+    while (__ballot(gm != 0u) != 0ull) {
+      if (gm == 0u) continue;
+      const int g = g0 + __ffs(gm) - 1;
+      gm &= gm - 1u;
+      const int k1 = min(nd, (g + 1) * kDomGroup);
+      for (int k = g * kDomGroup; k < k1; ++k) {
+        const int dd = perm[k];
+        float tmin;
+        // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
+        // spawning domain was tested right after shading
+        if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
+        ++hits;
+        atomicAdd(&lc_kind[dd], 1u);
+        mask[(dd >> 6) * stride] |= 1ull << (dd & 63);
+        // insertion into the (t, id)-sorted top list (unrolled: stays in
+        // registers); steps past every active lane's list end are skipped
+        if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
+          float ct = tmin;
+          int ci = dd;
 #pragma unroll
-        for (int j = 0; j < kTopDepths; ++j) {
-          if (__ballot(j <= kept) == 0ull) break;
-          if (j < kept) {
-            if (ct < tt[j] || (ct == tt[j] && ci < ti[j])) {
-              float xt = tt[j];
-              int xi = ti[j];
+          for (int j = 0; j < kTopDepths; ++j) {
+            if (__ballot(j <= kept) == 0ull) break;
+            if (j < kept) {
+              if (ct < tt[j] || (ct == tt[j] && ci < ti[j])) {
+                float xt = tt[j];
+                int xi = ti[j];
+                tt[j] = ct;
+                ti[j] = ci;
+                ct = xt;
+                ci = xi;
+              }
+            } else if (j == kept) {
               tt[j] = ct;
               ti[j] = ci;
-              ct = xt;
-              ci = xi;
             }
-          } else if (j == kept) {
-            tt[j] = ct;
-            ti[j] = ci;
           }
+          if (kept < kTopDepths) ++kept;
         }
-        if (kept < kTopDepths) ++kept;
       }
     }
   }
@@ -285,18 +307,19 @@ __device__ __forceinline__ int rayDomains(const float4* box, const float4* gbox,
 __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
-  const int nd = P.ndomains, ng = (nd + kDomGroup - 1) / kDomGroup, nw = (nd + 63) >> 6;
+  const int nd = P.ndomains, ng = numGroups(nd), ns = numSupers(nd), nw = (nd + 63) >> 6;
   const CountLds L = countLds(nd, blockDim.x);
   unsigned char* smc = (unsigned char*)sm;
   float4* box = (float4*)smc;                                 // 2 * nd: (lo, 0), (hi, 0)
   float4* gbox = (float4*)(smc + L.gbox);                     // 2 * ng
+  float4* sbox = (float4*)(smc + L.sbox);                     // 2 * ns
   uint32_t* lc = (uint32_t*)(smc + L.lc);                     // 2 * nd counts
   uint32_t* ls = (uint32_t*)(smc + L.ls);                     // nd score (32-bit per block)
   unsigned long long* lmask = (unsigned long long*)(smc + L.mask) + threadIdx.x;
   uint16_t* perm = (uint16_t*)(smc + L.perm);                 // nd
   for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
-  for (int i = threadIdx.x; i < 2 * ng; i += blockDim.x) box[2 * nd + i] = P.dbox[2 * nd + i];
-  const uint16_t* dperm = (const uint16_t*)(P.dbox + 2 * nd + 2 * ng);
+  for (int i = threadIdx.x; i < 2 * (ng + ns); i += blockDim.x) box[2 * nd + i] = P.dbox[2 * nd + i];
+  const uint16_t* dperm = (const uint16_t*)(P.dbox + 2 * nd + 2 * ng + 2 * ns);
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
     lc[i] = 0;
     lc[nd + i] = 0;
@@ -336,8 +359,8 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
     if (valid) {
       for (int w = 0; w < nw; ++w) lmask[w * blockDim.x] = 0ull;
-      hits = rayDomains(box, gbox, perm, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti, &kept,
-                        lmask, blockDim.x);
+      hits = rayDomains(box, gbox, sbox, perm, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti,
+                        &kept, lmask, blockDim.x);
       for (int w = 0; w < nw; ++w) P.dmask[(size_t)i * nw + w] = lmask[w * blockDim.x];
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
@@ -1357,7 +1380,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->vbuf, (size_t)S * kHistory * 8);
   need(T, T->rcand, (size_t)T->r_cap + 64);
   need(T, T->dmask, ((size_t)T->r_cap + T->s_cap) * ((nd + 63) / 64) * 8 + 64);
-  need(T, T->dbox, (size_t)nd * 32 + (size_t)(nd + kDomGroup - 1) / kDomGroup * 32 + nd * 2 + 64);
+  need(T, T->dbox, (size_t)nd * 32 + (size_t)(numGroups(nd) + numSupers(nd)) * 32 + nd * 2 + 64);
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
   need(T, T->cnt, (size_t)2 * nd * 4 + 64);
@@ -1499,7 +1522,7 @@ static size_t countShmem(int nd) { return countLds(nd, kBlock).total; }
 // id), kDomGroup consecutive ones per group, each group with its union box.
 static void domainBoxesUpload(Ctx* c, hipStream_t s) {
   TracerState* T = c->tracer;
-  const int nd = c->pool.ndomains, ng = (nd + kDomGroup - 1) / kDomGroup;
+  const int nd = c->pool.ndomains, ng = numGroups(nd), ns = numSupers(nd);
   const auto& D = c->pool.desc;
   float smin[3] = {kInfF, kInfF, kInfF}, smax[3] = {-kInfF, -kInfF, -kInfF};
   for (int d = 0; d < nd; ++d)
@@ -1519,11 +1542,12 @@ static void domainBoxesUpload(Ctx* c, hipStream_t s) {
     key[d] = {code, d};
   }
   std::sort(key.begin(), key.end());
-  const size_t bytes = (size_t)nd * 32 + (size_t)ng * 32 + (size_t)nd * 2;
+  const size_t bytes = (size_t)nd * 32 + (size_t)(ng + ns) * 32 + (size_t)nd * 2;
   T->h_box.assign((bytes + 3) / 4, 0.f);
   float* box = T->h_box.data();
   float* gbox = box + (size_t)nd * 8;
-  uint16_t* perm = (uint16_t*)(gbox + (size_t)ng * 8);
+  float* sbox = gbox + (size_t)ng * 8;
+  uint16_t* perm = (uint16_t*)(sbox + (size_t)ns * 8);
   for (int d = 0; d < nd; ++d)
     for (int k = 0; k < 3; ++k) {
       box[(size_t)d * 8 + k] = D[d].lo[k];
@@ -1543,6 +1567,18 @@ static void domainBoxesUpload(Ctx* c, hipStream_t s) {
         gb[4 + k] = std::max(gb[4 + k], D[d].hi[k]);
       }
     }
+  }
+  for (int q = 0; q < ns; ++q) {  // supergroups: unions of consecutive groups
+    float* sb = sbox + (size_t)q * 8;
+    for (int k = 0; k < 3; ++k) {
+      sb[k] = kInfF;
+      sb[4 + k] = -kInfF;
+    }
+    for (int g = q * kGroupsPerSuper; g < std::min(ng, (q + 1) * kGroupsPerSuper); ++g)
+      for (int k = 0; k < 3; ++k) {
+        sb[k] = std::min(sb[k], gbox[(size_t)g * 8 + k]);
+        sb[4 + k] = std::max(sb[4 + k], gbox[(size_t)g * 8 + 4 + k]);
+      }
   }
   SPRAY_HIP(hipMemcpyAsync(T->dbox.p, T->h_box.data(), bytes, hipMemcpyHostToDevice, s));
 }

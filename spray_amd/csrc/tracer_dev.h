@@ -59,7 +59,8 @@ struct TraceParams {
   // pool
   const DomainDesc* desc;
   // per domain (lo, 0), (hi, 0); then per culling group of kDomGroup
-  // domains the union box; then the domain ids in group order (uint16)
+  // domains the union box; then per supergroup of kGroupsPerSuper groups
+  // the union box; then the domain ids in group order (uint16)
   const float4* dbox;
   const BNode4* nodes;
   const LTri* tris;
