@@ -219,6 +219,7 @@ def main():
         frames_s = sum(walls)
         img, st_img = ctx.trace_frame(want_image=True)
         images[name] = img
+        lit = float((img[..., :3].max(axis=2) > 0).mean())
         # kernel timing of the dominant (trace) kernel, on the tracer's stream
         ctx.profile_kernels(True)
         ctx.kernel_time(reset=True)
@@ -243,6 +244,7 @@ def main():
             cache_hits=st["cache_hits"], cache_misses=st["cache_misses"],
             cache_bytes=st["cache_bytes"],
             cache_h2d_gbs=st["cache_bytes"] / (st["trace_ms"] * 1e6) if st["trace_ms"] > 0 else 0.0,
+            image_mean=float(img[..., :3].mean()), image_lit_fraction=lit,
         )
         if remesh_step:
             results[name].update(
