@@ -60,12 +60,13 @@ WORKLOADS = {
                title="C3 path tracing: 256 domains (8x8x4 MC isosurfaces, log-uniform 10k-2M tris each, "
                      "{tris} tris), {w}x{h}, {spp} spp, depth 6, RR>=3, rect light"),
     "c5": dict(grid=(16, 8, 8), gen=dict(res=160, rgb=0xB3B3B3,
-                                         extra_lines="light rect 2 8.5 1 12 0 0 0 0 6 4 4 4"),
+                                         extra_lines="light rect 2 1 8.5 0 6 0 12 0 0 4 4 4"),
                render=dict(image_w=3840, image_h=2160, pixel_samples=16, ao_mode=0, ao_samples=1, bounces=8,
                            rr_start=3),
                cam=None, cpu_tiles=0, remesh=0.05,
                title="C5 in-situ: 1024 domains (16x8x8 MC isosurfaces remeshed on the GPU every frame, "
-                     "phase +0.05 per frame, {tris} tris), {w}x{h}, {spp} spp, depth 8, RR>=3, rect light"),
+                     "phase +0.05 per frame, {tris} tris), {w}x{h}, {spp} spp, depth 8, RR>=3, rect light facing the "
+                     "scene's front (+z) face"),
 }
 
 
