@@ -40,15 +40,38 @@ __host__ __device__ inline int numGroups(int nd) { return (nd + kDomGroup - 1) /
 __host__ __device__ inline int numSupers(int nd) {
   return (numGroups(nd) + kGroupsPerSuper - 1) / kGroupsPerSuper;
 }
+// Scenes with at least kGridMinDomains domains use a domain grid instead of
+// the groups: a uniform grid over the domain boxes (at most kGridCells
+// cells, kGridMaxRes per axis), each cell listing the domains whose box,
+// grown by kGridGrow of a cell, overlaps it; at most kGridIdsPerDomain list
+// entries per domain on average (the grid is coarsened until that holds).
+constexpr int kGridMinDomains = 512;
+constexpr int kGridCells = 32768;
+constexpr int kGridMaxRes = 64;
+constexpr int kGridIdsPerDomain = 32;
+constexpr float kGridGrow = 1e-3f;
+// dbox bytes: domain boxes, group boxes, supergroup boxes, domain ids in
+// group order; then the grid header (4 float4), cell offsets, cell ids
+__host__ __device__ inline size_t groupBytes(int nd) {
+  return ((size_t)nd * 32 + (size_t)(numGroups(nd) + numSupers(nd)) * 32 + (size_t)nd * 2 + 15) &
+         ~(size_t)15;
+}
+__host__ __device__ inline size_t gridOffBytes(int nd) { return groupBytes(nd) + 64; }
+__host__ __device__ inline size_t gridIdsBytes(int nd) {
+  return gridOffBytes(nd) + (size_t)(kGridCells + 1) * 4;
+}
+__host__ __device__ inline size_t dboxBytes(int nd) {
+  return gridIdsBytes(nd) + (size_t)kGridIdsPerDomain * nd * 2 + 64;
+}
 
-// kCount's dynamic LDS (byte offsets): domain boxes, group boxes,
-// supergroup boxes, counts, scores, the block's hit-mask words
-// ([word][thread]), domain permutation.
+// kCount's dynamic LDS (byte offsets): domain boxes, group boxes and
+// supergroup boxes (group form only), counts, scores, the block's hit-mask
+// words ([word][thread]), domain permutation (group form only).
 struct CountLds {
   size_t gbox, sbox, lc, ls, mask, perm, total;
 };
-__host__ __device__ inline CountLds countLds(int nd, int block) {
-  const int ng = numGroups(nd), ns = numSupers(nd), nw = (nd + 63) / 64;
+__host__ __device__ inline CountLds countLds(int nd, int block, bool grid) {
+  const int ng = grid ? 0 : numGroups(nd), ns = grid ? 0 : numSupers(nd), nw = (nd + 63) / 64;
   CountLds L;
   L.gbox = (size_t)nd * 32;
   L.sbox = L.gbox + (size_t)ng * 32;
@@ -56,9 +79,17 @@ __host__ __device__ inline CountLds countLds(int nd, int block) {
   L.ls = L.lc + (size_t)nd * 8;
   L.mask = (L.ls + (size_t)nd * 4 + 7) & ~(size_t)7;
   L.perm = L.mask + (size_t)nw * block * 8;
-  L.total = L.perm + (size_t)nd * 2;
+  L.total = L.perm + (grid ? 0 : (size_t)nd * 2);
   return L;
 }
+__host__ __device__ inline bool useDomainGrid(int nd) { return nd >= kGridMinDomains; }
+
+// Grid header as uploaded: (lo, res x bits), (hi, res y bits), (cell size,
+// res z bits), (1 / cell size, 0); resolution 0: no grid (groups).
+struct DomGrid {
+  float lo[3], hi[3], h[3], inv[3];
+  int r[3];
+};
 constexpr int kShadeBlock = 512;  // threads per shading block (one pool atomic per block)
 // The trace kernel's waves reserve queue entries kRange at a time (one
 // atomic per range, not per refill); the ranges still follow queue order,
@@ -233,19 +264,20 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
 // One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
 // entries sorted (registers), the domains hit as a bit mask (one word per 64
 // domains, this lane's words at mask[w * stride]) so that the scatter
-// re-tests only those, and the number of domains hit.  Domains are visited
-// in groups of kDomGroup (Morton order of their centres), groups in
+// re-tests only those, and the number of domains hit.  Below kGridMinDomains
+// domains are visited in groups of kDomGroup (Morton order of their
+// centres), groups in
 // supergroups of kGroupsPerSuper: a supergroup's groups are tested when some
 // lane of the wave hits its box, and each lane then tests the members of
 // only the groups it hits itself.  The member tests are the exact ones and
 // the list is kept sorted by (t, id), so the result does not depend on the
 // grouping or on the order the groups are visited in.
-__device__ __forceinline__ int rayDomains(const float4* box, const float4* gbox,
-                                          const float4* sbox, const uint16_t* perm, int nd,
-                                          const float o[3], const float inv[3], bool safe,
-                                          int spawn, uint32_t* lc_kind, float tt[kTopDepths],
-                                          int ti[kTopDepths], int* kept_o,
-                                          unsigned long long* mask, int stride) {
+__device__ __forceinline__ int rayDomainsGroups(const float4* box, const float4* gbox,
+                                                const float4* sbox, const uint16_t* perm, int nd,
+                                                const float o[3], const float inv[3], bool safe,
+                                                int spawn, uint32_t* lc_kind, float tt[kTopDepths],
+                                                int ti[kTopDepths], int* kept_o,
+                                                unsigned long long* mask, int stride) {
   int kept = 0, hits = 0;
   const int ng = numGroups(nd), ns = numSupers(nd);
   for (int sg = 0; sg < ns; ++sg) {
@@ -301,14 +333,118 @@ __device__ __forceinline__ int rayDomains(const float4* box, const float4* gbox,
   return hits;
 }
 
+// One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
+// entries sorted (registers), the domains hit as a bit mask (one word per 64
+// domains, this lane's words at mask[w * stride]) so that the scatter
+// re-tests only those, and the number of domains hit.  From kGridMinDomains
+// domains on, candidates come from a 3D-DDA walk of the domain grid over the
+// ray's t range inside the grid (from max(0, entry)); every domain a ray
+// hits overlaps a cell the walk
+// visits (cells list the domains of boxes grown by a thousandth of a cell,
+// which absorbs the walk's rounding at cell faces and corners).  Each
+// candidate not already marked gets the exact slab test, and the list is
+// kept sorted by (t, id), so the result does not depend on the grid or on
+// the visiting order.
+__device__ __forceinline__ int rayDomainsGrid(const float4* box, const DomGrid& G,
+                                              const uint32_t* __restrict__ coff,
+                                              const uint16_t* __restrict__ cids, const float o[3],
+                                              const float d[3], const float inv[3], bool safe,
+                                              int spawn, uint32_t* lc_kind, float tt[kTopDepths],
+                                              int ti[kTopDepths], int* kept_o,
+                                              unsigned long long* mask, int stride) {
+  int kept = 0, hits = 0;
+  // the ray's interval inside the grid box; an axis with an infinite 1/dir
+  // does not step and only needs the origin inside the slab
+  float t0 = 0.f, t1 = kInfF;
+  bool in = true;
+  bool moves[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    moves[k] = fabsf(inv[k]) < kInfF;
+    if (moves[k]) {
+      const float a = (G.lo[k] - o[k]) * inv[k], b = (G.hi[k] - o[k]) * inv[k];
+      t0 = fmaxf(t0, fminf(a, b));
+      t1 = fminf(t1, fmaxf(a, b));
+    } else if (o[k] < G.lo[k] || o[k] > G.hi[k]) {
+      in = false;
+    }
+  }
+  if (in && t0 <= t1) {
+    int c[3], st[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float p = o[k] + t0 * d[k];
+      const int ck = (int)floorf((p - G.lo[k]) * G.inv[k]);
+      c[k] = min(max(ck, 0), G.r[k] - 1);
+      st[k] = !moves[k] ? 0 : (d[k] > 0.f ? 1 : -1);
+    }
+    const int max_steps = G.r[0] + G.r[1] + G.r[2] + 3;
+    for (int step = 0; step < max_steps; ++step) {
+      const int cell = (c[2] * G.r[1] + c[1]) * G.r[0] + c[0];
+      const uint32_t q1 = coff[cell + 1];
+      for (uint32_t q = coff[cell]; q < q1; ++q) {
+        const int dd = cids[q];
+        unsigned long long* mw = &mask[(dd >> 6) * stride];
+        if ((*mw >> (dd & 63)) & 1ull) continue;  // already a hit
+        float tmin;
+        // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
+        // spawning domain was tested right after shading
+        if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
+        ++hits;
+        atomicAdd(&lc_kind[dd], 1u);
+        *mw |= 1ull << (dd & 63);
+        // insertion into the (t, id)-sorted top list (unrolled: stays in
+        // registers); steps past every active lane's list end are skipped
+        if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
+          float ct = tmin;
+          int ci = dd;
+#pragma unroll
+          for (int j = 0; j < kTopDepths; ++j) {
+            if (__ballot(j <= kept) == 0ull) break;
+            if (j < kept) {
+              if (ct < tt[j] || (ct == tt[j] && ci < ti[j])) {
+                float xt = tt[j];
+                int xi = ti[j];
+                tt[j] = ct;
+                ti[j] = ci;
+                ct = xt;
+                ci = xi;
+              }
+            } else if (j == kept) {
+              tt[j] = ct;
+              ti[j] = ci;
+            }
+          }
+          if (kept < kTopDepths) ++kept;
+        }
+      }
+      // next cell: the axis whose cell face the ray crosses first
+      float tn[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        tn[k] = st[k] == 0 ? kInfF
+                           : (G.lo[k] + (float)(c[k] + (st[k] > 0 ? 1 : 0)) * G.h[k] - o[k]) * inv[k];
+      const int a = tn[0] <= tn[1] ? (tn[0] <= tn[2] ? 0 : 2) : (tn[1] <= tn[2] ? 1 : 2);
+      if (!(tn[a] <= t1)) break;
+      c[a] += st[a];
+      if (c[a] < 0 || c[a] >= G.r[a]) break;
+    }
+  }
+  *kept_o = kept;
+  return hits;
+}
+
 // Isector::isectAll / isectWithoutCurrentDomain (ooc_isector.h:126-174) as
 // counts: entries per domain and the DomainStats score
 // (ooc_domain_stats.cc:60-88) weight max(0, 16 - position in the ray's list).
+// kGrid: the domain-grid form (scenes of kGridMinDomains domains and more),
+// else the group form; one instantiation each keeps their registers apart.
+template <bool kGrid>
 __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
   const int nd = P.ndomains, ng = numGroups(nd), ns = numSupers(nd), nw = (nd + 63) >> 6;
-  const CountLds L = countLds(nd, blockDim.x);
+  const CountLds L = countLds(nd, blockDim.x, kGrid);
   unsigned char* smc = (unsigned char*)sm;
   float4* box = (float4*)smc;                                 // 2 * nd: (lo, 0), (hi, 0)
   float4* gbox = (float4*)(smc + L.gbox);                     // 2 * ng
@@ -317,14 +453,25 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   uint32_t* ls = (uint32_t*)(smc + L.ls);                     // nd score (32-bit per block)
   unsigned long long* lmask = (unsigned long long*)(smc + L.mask) + threadIdx.x;
   uint16_t* perm = (uint16_t*)(smc + L.perm);                 // nd
-  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
-  for (int i = threadIdx.x; i < 2 * (ng + ns); i += blockDim.x) box[2 * nd + i] = P.dbox[2 * nd + i];
-  const uint16_t* dperm = (const uint16_t*)(P.dbox + 2 * nd + 2 * ng + 2 * ns);
+  for (int i = threadIdx.x; i < 2 * (kGrid ? nd : nd + ng + ns); i += blockDim.x) box[i] = P.dbox[i];
+  const uint16_t* dperm = (const uint16_t*)(P.dbox + 2 * (nd + ng + ns));
+  DomGrid G;
+  if (kGrid) {
+    const float4* gh = (const float4*)((const char*)P.dbox + groupBytes(nd));
+    const float4 a = gh[0], b = gh[1], h = gh[2], v = gh[3];
+    G.lo[0] = a.x, G.lo[1] = a.y, G.lo[2] = a.z;
+    G.hi[0] = b.x, G.hi[1] = b.y, G.hi[2] = b.z;
+    G.h[0] = h.x, G.h[1] = h.y, G.h[2] = h.z;
+    G.inv[0] = v.x, G.inv[1] = v.y, G.inv[2] = v.z;
+    G.r[0] = __float_as_int(a.w), G.r[1] = __float_as_int(b.w), G.r[2] = __float_as_int(h.w);
+  }
+  const uint32_t* coff = (const uint32_t*)((const char*)P.dbox + gridOffBytes(nd));
+  const uint16_t* cids = (const uint16_t*)((const char*)P.dbox + gridIdsBytes(nd));
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
     lc[i] = 0;
     lc[nd + i] = 0;
     ls[i] = 0;
-    perm[i] = dperm[i];
+    if (!kGrid) perm[i] = dperm[i];
   }
   __syncthreads();
   const DevCounters* c = P.ctr;
@@ -342,7 +489,7 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     float tt[kTopDepths];
     int ti[kTopDepths];
     int kept = 0, hits = 0;
-    float o[3] = {0.f, 0.f, 0.f}, inv[3] = {1.f, 1.f, 1.f};
+    float o[3] = {0.f, 0.f, 0.f}, inv[3] = {1.f, 1.f, 1.f}, dir[3] = {1.f, 1.f, 1.f};
     int spawn = -1;
     if (valid) {
       const RayPool& pool = kind ? P.S : P.R;
@@ -350,6 +497,9 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
       o[0] = op.x;
       o[1] = op.y;
       o[2] = op.z;
+      dir[0] = ds.x;
+      dir[1] = ds.y;
+      dir[2] = ds.z;
       inv[0] = 1.0f / ds.x;
       inv[1] = 1.0f / ds.y;
       inv[2] = 1.0f / ds.z;
@@ -359,8 +509,12 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
     if (valid) {
       for (int w = 0; w < nw; ++w) lmask[w * blockDim.x] = 0ull;
-      hits = rayDomains(box, gbox, sbox, perm, nd, o, inv, safe, spawn, lc + kind * nd, tt, ti,
-                        &kept, lmask, blockDim.x);
+      if constexpr (kGrid)
+        hits = rayDomainsGrid(box, G, coff, cids, o, dir, inv, safe, spawn, lc + kind * nd, tt, ti,
+                              &kept, lmask, blockDim.x);
+      else
+        hits = rayDomainsGroups(box, gbox, sbox, perm, nd, o, inv, safe, spawn, lc + kind * nd, tt,
+                                ti, &kept, lmask, blockDim.x);
       for (int w = 0; w < nw; ++w) P.dmask[(size_t)i * nw + w] = lmask[w * blockDim.x];
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
@@ -1351,8 +1505,9 @@ static void ensureState(Ctx* c, double grow) {
   int nd = c->pool.ndomains;
   if (nd > 1024) throw StatusError(SPRAY_ERR_CAPACITY, "the tracer supports at most 1024 domains");
   if (countShmem(nd) > 65536)  // many domains: kCount's LDS beyond the default limit
-    SPRAY_HIP(hipFuncSetAttribute((const void*)kCount, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)countShmem(nd)));
+    SPRAY_HIP(hipFuncSetAttribute(useDomainGrid(nd) ? (const void*)kCount<true>
+                                                    : (const void*)kCount<false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)countShmem(nd)));
   int depths = std::min(cfg.bounces, kHistory);
   double spec = 2.0 * grow;  // headroom for speculative re-shading
   auto cap = [](double v) { return (uint32_t)std::min<double>(3.9e9, v + 4096); };
@@ -1380,7 +1535,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->vbuf, (size_t)S * kHistory * 8);
   need(T, T->rcand, (size_t)T->r_cap + 64);
   need(T, T->dmask, ((size_t)T->r_cap + T->s_cap) * ((nd + 63) / 64) * 8 + 64);
-  need(T, T->dbox, (size_t)nd * 32 + (size_t)(numGroups(nd) + numSupers(nd)) * 32 + nd * 2 + 64);
+  need(T, T->dbox, dboxBytes(nd));
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
   need(T, T->cnt, (size_t)2 * nd * 4 + 64);
@@ -1515,12 +1670,16 @@ static TraceParams makeParams(Ctx* c) {
   return P;
 }
 
-static size_t countShmem(int nd) { return countLds(nd, kBlock).total; }
+static size_t countShmem(int nd) { return countLds(nd, kBlock, useDomainGrid(nd)).total; }
 
-// Domain boxes for the ray/domain slab tests, plus the culling groups of
-// kCount: domains sorted by the Morton code of their box centres (ties by
-// id), kDomGroup consecutive ones per group, each group with its union box.
+// Domain boxes for the ray/domain slab tests, plus kCount's culling data:
+// domains sorted by the Morton code of their box centres (ties by id),
+// kDomGroup consecutive ones per group with its union box, supergroup
+// boxes; and, from kGridMinDomains domains, the domain grid (header, cell
+// offsets, the cells' domain ids in ascending order).  Without a grid the
+// header's resolution is 0.
 static void domainBoxesUpload(Ctx* c, hipStream_t s) {
+  const bool use_grid = useDomainGrid(c->pool.ndomains);
   TracerState* T = c->tracer;
   const int nd = c->pool.ndomains, ng = numGroups(nd), ns = numSupers(nd);
   const auto& D = c->pool.desc;
@@ -1542,7 +1701,77 @@ static void domainBoxesUpload(Ctx* c, hipStream_t s) {
     key[d] = {code, d};
   }
   std::sort(key.begin(), key.end());
-  const size_t bytes = (size_t)nd * 32 + (size_t)(ng + ns) * 32 + (size_t)nd * 2;
+  float glo[3] = {0.f, 0.f, 0.f}, ghi[3] = {0.f, 0.f, 0.f}, h[3] = {1.f, 1.f, 1.f},
+        inv[3] = {1.f, 1.f, 1.f};
+  int r[3] = {0, 0, 0};
+  std::vector<uint32_t> off;
+  std::vector<uint16_t> ids;
+  if (use_grid) {
+    // grid over the union of the domain boxes, resolution from the mean box
+    // extent per axis (one cell per domain on a lattice of domains)
+    float avg[3];
+    for (int k = 0; k < 3; ++k) {
+      glo[k] = kInfF;
+      ghi[k] = -kInfF;
+      double sum = 0;
+      for (int d = 0; d < nd; ++d) {
+        glo[k] = std::min(glo[k], D[d].lo[k]);
+        ghi[k] = std::max(ghi[k], D[d].hi[k]);
+        sum += (double)D[d].hi[k] - D[d].lo[k];
+      }
+      avg[k] = (float)(sum / nd);
+    }
+    for (int k = 0; k < 3; ++k) {
+      const float ext = ghi[k] - glo[k];
+      r[k] = (ext > 0.f && avg[k] > 0.f) ? (int)std::lround(ext / avg[k]) : 1;
+      r[k] = std::max(1, std::min(kGridMaxRes, r[k]));
+    }
+    for (;;) {
+      while ((long long)r[0] * r[1] * r[2] > kGridCells) *std::max_element(r, r + 3) /= 2;
+      for (int k = 0; k < 3; ++k) {
+        const float ext = ghi[k] - glo[k];
+        h[k] = ext > 0.f ? ext / (float)r[k] : 1.f;
+        inv[k] = 1.f / h[k];
+      }
+      const int ncell = r[0] * r[1] * r[2];
+      std::vector<uint32_t> cnt(ncell + 1, 0);
+      auto range = [&](int d, int k, int* c0, int* c1) {
+        const float g = kGridGrow * h[k];
+        *c0 = std::max(0, std::min(r[k] - 1, (int)std::floor((D[d].lo[k] - g - glo[k]) * inv[k])));
+        *c1 = std::max(0, std::min(r[k] - 1, (int)std::floor((D[d].hi[k] + g - glo[k]) * inv[k])));
+      };
+      long long total = 0;
+      for (int d = 0; d < nd; ++d) {
+        int a[3], b[3];
+        for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
+        total += (long long)(b[0] - a[0] + 1) * (b[1] - a[1] + 1) * (b[2] - a[2] + 1);
+      }
+      if (total > (long long)kGridIdsPerDomain * nd && r[0] * r[1] * r[2] > 1) {
+        *std::max_element(r, r + 3) /= 2;  // domains span too many cells: coarser grid
+        continue;
+      }
+      for (int d = 0; d < nd; ++d) {
+        int a[3], b[3];
+        for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
+        for (int z = a[2]; z <= b[2]; ++z)
+          for (int y = a[1]; y <= b[1]; ++y)
+            for (int x = a[0]; x <= b[0]; ++x) ++cnt[(z * r[1] + y) * r[0] + x];
+      }
+      off.assign(ncell + 1, 0);
+      for (int i = 0; i < ncell; ++i) off[i + 1] = off[i] + cnt[i];
+      ids.assign(off[ncell], 0);
+      std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+      for (int d = 0; d < nd; ++d) {  // ascending domain ids within a cell
+        int a[3], b[3];
+        for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
+        for (int z = a[2]; z <= b[2]; ++z)
+          for (int y = a[1]; y <= b[1]; ++y)
+            for (int x = a[0]; x <= b[0]; ++x) ids[cur[(z * r[1] + y) * r[0] + x]++] = (uint16_t)d;
+      }
+      break;
+    }
+  }
+  const size_t bytes = use_grid ? gridIdsBytes(nd) + ids.size() * 2 : gridOffBytes(nd);
   T->h_box.assign((bytes + 3) / 4, 0.f);
   float* box = T->h_box.data();
   float* gbox = box + (size_t)nd * 8;
@@ -1580,8 +1809,23 @@ static void domainBoxesUpload(Ctx* c, hipStream_t s) {
         sb[4 + k] = std::max(sb[4 + k], gbox[(size_t)g * 8 + 4 + k]);
       }
   }
+  float* gh = (float*)((char*)box + groupBytes(nd));
+  for (int k = 0; k < 3; ++k) {
+    gh[k] = glo[k];
+    gh[4 + k] = ghi[k];
+    gh[8 + k] = h[k];
+    gh[12 + k] = inv[k];
+  }
+  std::memcpy(&gh[3], &r[0], 4);
+  std::memcpy(&gh[7], &r[1], 4);
+  std::memcpy(&gh[11], &r[2], 4);
+  if (use_grid) {
+    std::memcpy((char*)box + gridOffBytes(nd), off.data(), off.size() * 4);
+    std::memcpy((char*)box + gridIdsBytes(nd), ids.data(), ids.size() * 2);
+  }
   SPRAY_HIP(hipMemcpyAsync(T->dbox.p, T->h_box.data(), bytes, hipMemcpyHostToDevice, s));
 }
+
 static size_t scatterShmem(int nd) { return (size_t)nd * (8 + 4) * 4; }  // boxes, slots, bases
 
 static void ensureOverflow(TracerState* T, size_t lanes) { need(T, T->ovf, lanes * kOverflow * 4); }
@@ -1593,7 +1837,10 @@ static void passQueues(Ctx* c, const TraceParams& P) {
   int nd = P.ndomains;
   kPassBegin<<<1, 1, 0, s>>>(P);
   kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
-  kCount<<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+  if (useDomainGrid(nd))
+    kCount<true><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+  else
+    kCount<false><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
   kSchedule<<<1, 1024, 0, s>>>(P);
   kBlockBases<<<2 * nd, kBlock, 0, s>>>(P);
   kScatter<<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);

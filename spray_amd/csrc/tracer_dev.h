@@ -58,9 +58,9 @@ struct TraceParams {
   DevLight lights[kMaxLights];
   // pool
   const DomainDesc* desc;
-  // per domain (lo, 0), (hi, 0); then per culling group of kDomGroup
-  // domains the union box; then per supergroup of kGroupsPerSuper groups
-  // the union box; then the domain ids in group order (uint16)
+  // per domain (lo, 0), (hi, 0); then the domain grid: header (4 float4),
+  // cell offsets (uint32, at gridOffBytes) and cell domain ids (uint16, at
+  // gridIdsBytes)
   const float4* dbox;
   const BNode4* nodes;
   const LTri* tris;
