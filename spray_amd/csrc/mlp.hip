@@ -163,13 +163,25 @@ __device__ __forceinline__ uint32_t orderedF(float f) {
 // W2 [128][128] (out, in) with padded row strides so that the 16-byte
 // B-fragment reads of the 16 lanes sharing a k-slice hit distinct banks.
 // Each wave runs 32 queue entries (two 16-row blocks) per iteration, so
-// every B fragment read from LDS feeds two MFMAs.
+// every B fragment read from LDS feeds two MFMAs.  A wave's feature rows and
+// hidden rows share one LDS area (the features are in registers before the
+// hidden rows are written), which leaves room for 12 waves per block.
 constexpr int kPredRows = 32;
 constexpr int kLdX = kInPad + 8;   // bf16 row stride of x and W1 in LDS
 constexpr int kLdH = kHid + 8;     // bf16 row stride of h1 and W2 in LDS
-constexpr int kPredWaves = 8;
+constexpr int kPredWaves = 12;
 constexpr size_t kPredLdsWeights = (size_t)kHid * kLdX + (size_t)kHid * kLdH;  // bf16 elements
-constexpr size_t kPredLdsPerWave = (size_t)kPredRows * kLdX + (size_t)kPredRows * kLdH;
+constexpr size_t kPredLdsPerWave = (size_t)kPredRows * kLdH;
+static_assert(kLdX <= kLdH, "feature rows fit the hidden-row area");
+
+// One queue entry's inputs for inference, gathered ahead of their use.
+struct PredIn {
+  QEntry e;
+  int dom;
+  bool valid;  // a queue entry (not past the end)
+  int rc;      // the ray's candidate domains this pass
+  float4 op, ds;
+};
 constexpr size_t kPredLdsBytes =
     (kPredLdsWeights + kPredWaves * kPredLdsPerWave) * 2 + 3 * kHid * 4 + 16;
 
@@ -201,25 +213,48 @@ __global__ void __launch_bounds__(64 * kPredWaves) kPredictKeys(TraceParams P, M
   __syncthreads();
   const uint32_t n = c->q_total[0];
   const int w = threadIdx.x >> 6, l = __lane_id();
-  __bf16* xs = wave_base + w * kPredLdsPerWave;  // [32][kLdX]
-  __bf16* hs = xs + kPredRows * kLdX;            // [32][kLdH]
-  for (uint32_t base = (blockIdx.x * kPredWaves + w) * (uint32_t)kPredRows; base < n;
-       base += gridDim.x * kPredWaves * (uint32_t)kPredRows) {
-    // lanes 0..31 build the feature rows; rays with a single candidate
-    // domain this pass need no ordering and are skipped
-    const uint32_t q = base + (uint32_t)(l & 31);
-    bool use = false;
-    QEntry e = QEntry{0, 0.f};
-    if (l < kPredRows && q < n) {
-      e = P.q[0][q];
-      use = P.rcand[e.ray] > 1;
+  __bf16* xs = wave_base + w * kPredLdsPerWave;  // [32][kLdX], then
+  __bf16* hs = xs;                               // [32][kLdH]
+  const unsigned long long stride = (unsigned long long)gridDim.x * kPredWaves * kPredRows;
+  // Software pipeline over the wave's iterations: the queue entries two
+  // iterations ahead and the ray records one iteration ahead are loaded
+  // while the current rows go through the MFMA layers (lanes 0..31 hold one
+  // entry each; rays with a single candidate domain this pass need no
+  // ordering and are skipped).
+  auto loadEntry = [&](unsigned long long b, PredIn& g) {
+    const unsigned long long q = b + (unsigned long long)(l & 31);
+    g.e = QEntry{0, 0.f};
+    g.dom = 0;
+    g.valid = l < kPredRows && q < n;
+    if (g.valid) {
+      g.e = P.q[0][q];
+      g.dom = (int)P.qdom[0][q];
     }
+  };
+  // the candidate count and the ray record are loaded together (no wait
+  // between them); the count is tested when the entry is used
+  auto loadRay = [&](PredIn& g) {
+    g.rc = 0;
+    g.op = g.ds = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (g.valid) {
+      g.rc = P.rcand[g.e.ray];
+      g.op = P.R.op[g.e.ray];
+      g.ds = P.R.ds[g.e.ray];
+    }
+  };
+  unsigned long long base = (unsigned long long)(blockIdx.x * kPredWaves + w) * kPredRows;
+  PredIn cur, nxt, nx2;
+  loadEntry(base, cur);
+  loadRay(cur);
+  loadEntry(base + stride, nxt);
+  for (; base < n; base += stride) {
+    const bool use = cur.valid && cur.rc > 1;
+    const QEntry e = cur.e;
     if (l < kPredRows) {
       float x[kInPad];
       if (use) {
-        float4 op = P.R.op[e.ray], ds = P.R.ds[e.ray];
-        float o[3] = {op.x, op.y, op.z}, d[3] = {ds.x, ds.y, ds.z};
-        features(M, o, d, (int)P.qdom[0][q], x);
+        float o[3] = {cur.op.x, cur.op.y, cur.op.z}, d[3] = {cur.ds.x, cur.ds.y, cur.ds.z};
+        features(M, o, d, cur.dom, x);
       } else {
 #pragma unroll
         for (int k = 0; k < kInPad; ++k) x[k] = 0.f;
@@ -232,7 +267,15 @@ __global__ void __launch_bounds__(64 * kPredWaves) kPredictKeys(TraceParams P, M
         *(bf16x8*)(xs + l * kLdX + 8 * k8) = v;
       }
     }
-    if (__ballot(use) == 0ull) continue;
+    // issued after this iteration's inputs were consumed, so that waiting
+    // for those never waits for these; they land during the MFMA work
+    loadEntry(base + 2 * stride, nx2);
+    loadRay(nxt);
+    if (__ballot(use) == 0ull) {
+      cur = nxt;
+      nxt = nx2;
+      continue;
+    }
     waveSync();
     // layer 1 (K = 32): h1 = relu(x W1^T + b1) -> bf16 rows
     f32x4 acc[2][8];
@@ -242,6 +285,7 @@ __global__ void __launch_bounds__(64 * kPredWaves) kPredictKeys(TraceParams P, M
       for (int nb = 0; nb < 8; ++nb) acc[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     {
       bf16x8 a0 = ldsA(xs, kLdX, 0), a1 = ldsA(xs + 16 * kLdX, kLdX, 0);
+      waveSync();  // the hidden rows below overwrite the feature rows
 #pragma unroll
       for (int nb = 0; nb < 8; ++nb) {
         bf16x8 bw = ldB(w1s, kLdX, nb, 0);
@@ -314,6 +358,8 @@ __global__ void __launch_bounds__(64 * kPredWaves) kPredictKeys(TraceParams P, M
           ((unsigned long long)orderedF(zr) << 32) | (uint32_t)(~orderedF(e.tdom));
       atomicMax(&rkey[e.ray], key);
     }
+    cur = nxt;
+    nxt = nx2;
     waveSync();
   }
 }
@@ -767,7 +813,7 @@ void predictorPrepare(Ctx* c, size_t r_cap) {
     SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
     SPRAY_HIP(hipFuncSetAttribute((const void*)kPredictKeys,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPredLdsBytes));
-    c->pred->pred_grid = prop.multiProcessorCount;  // one 8-wave block per CU
+    c->pred->pred_grid = prop.multiProcessorCount;  // one 12-wave block per CU
   }
 }
 
