@@ -7,9 +7,10 @@
 // Training data are (origin, direction, domain, label) samples logged by
 // the tracer.  Inference orders speculative domain visits: per radiance ray,
 // the domains entered no later than the most likely hit domain run in phase
-// A of a pass; the rest are deferred to the end of the pass's launch, where
-// the tdom <= history filter prunes most of them.  Exact traversal stays the judge, so the image
-// does not depend on the network.
+// A of a pass; the rest are deferred to the end of the pass's launch (or,
+// in small passes, to a second launch), where the tdom <= history filter
+// prunes most of them.  Exact traversal still decides every hit, so the
+// image does not depend on the network.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
