@@ -34,29 +34,21 @@ constexpr int kBlock = 256;
 // grid of kCount and kScatter (they must match: blocks own queue slots)
 constexpr int kQueueGrid = 1024;
 constexpr int kShadeRec = 5;  // float4s per shade record
-constexpr int kDomGroup = 8;  // domains per culling group of the domain-list pass
-constexpr int kGroupsPerSuper = 8;  // culling groups per supergroup (64 domains)
-__host__ __device__ inline int numGroups(int nd) { return (nd + kDomGroup - 1) / kDomGroup; }
-__host__ __device__ inline int numSupers(int nd) {
-  return (numGroups(nd) + kGroupsPerSuper - 1) / kGroupsPerSuper;
-}
-// Scenes with at least kGridMinDomains domains use a domain grid instead of
-// the groups: a uniform grid over the domain boxes (at most kGridCells
-// cells, kGridMaxRes per axis), each cell listing the domains whose box,
-// grown by kGridGrow of a cell, overlaps it; at most kGridIdsPerDomain list
-// entries per domain on average (the grid is coarsened until that holds).
-constexpr int kGridMinDomains = 512;
+// Domain grid of the domain-list pass: a uniform grid over the domain boxes
+// (at most kGridCells cells, kGridMaxRes per axis), each cell listing the
+// domains whose box
+// overlaps it with positive extent (cells [floor(lo), ceil(hi) - 1] per
+// axis, so a box that fills a cell is listed once); at most
+// kGridIdsPerDomain list entries per domain on average (the grid is
+// coarsened until that holds).  Contacts on cell faces, edges and corners
+// are the walk's business (kGridTol).
 constexpr int kGridCells = 32768;
 constexpr int kGridMaxRes = 64;
 constexpr int kGridIdsPerDomain = 32;
-constexpr float kGridGrow = 1e-3f;
-// dbox bytes: domain boxes, group boxes, supergroup boxes, domain ids in
-// group order; then the grid header (4 float4), cell offsets, cell ids
-__host__ __device__ inline size_t groupBytes(int nd) {
-  return ((size_t)nd * 32 + (size_t)(numGroups(nd) + numSupers(nd)) * 32 + (size_t)nd * 2 + 15) &
-         ~(size_t)15;
-}
-__host__ __device__ inline size_t gridOffBytes(int nd) { return groupBytes(nd) + 64; }
+constexpr float kGridTol = 1e-4f;
+// dbox bytes: domain boxes, grid header (4 float4), cell offsets, cell ids
+__host__ __device__ inline size_t gridHeadBytes(int nd) { return (size_t)nd * 32; }
+__host__ __device__ inline size_t gridOffBytes(int nd) { return gridHeadBytes(nd) + 64; }
 __host__ __device__ inline size_t gridIdsBytes(int nd) {
   return gridOffBytes(nd) + (size_t)(kGridCells + 1) * 4;
 }
@@ -64,28 +56,23 @@ __host__ __device__ inline size_t dboxBytes(int nd) {
   return gridIdsBytes(nd) + (size_t)kGridIdsPerDomain * nd * 2 + 64;
 }
 
-// kCount's dynamic LDS (byte offsets): domain boxes, group boxes and
-// supergroup boxes (group form only), counts, scores, the block's hit-mask
-// words ([word][thread]), domain permutation (group form only).
+// kCount's dynamic LDS (byte offsets): domain boxes, counts, scores, the
+// block's hit-mask words ([word][thread]).
 struct CountLds {
-  size_t gbox, sbox, lc, ls, mask, perm, total;
+  size_t lc, ls, mask, total;
 };
-__host__ __device__ inline CountLds countLds(int nd, int block, bool grid) {
-  const int ng = grid ? 0 : numGroups(nd), ns = grid ? 0 : numSupers(nd), nw = (nd + 63) / 64;
+__host__ __device__ inline CountLds countLds(int nd, int block) {
+  const int nw = (nd + 63) / 64;
   CountLds L;
-  L.gbox = (size_t)nd * 32;
-  L.sbox = L.gbox + (size_t)ng * 32;
-  L.lc = L.sbox + (size_t)ns * 32;
+  L.lc = (size_t)nd * 32;
   L.ls = L.lc + (size_t)nd * 8;
   L.mask = (L.ls + (size_t)nd * 4 + 7) & ~(size_t)7;
-  L.perm = L.mask + (size_t)nw * block * 8;
-  L.total = L.perm + (grid ? 0 : (size_t)nd * 2);
+  L.total = L.mask + (size_t)nw * block * 8;
   return L;
 }
-__host__ __device__ inline bool useDomainGrid(int nd) { return nd >= kGridMinDomains; }
 
 // Grid header as uploaded: (lo, res x bits), (hi, res y bits), (cell size,
-// res z bits), (1 / cell size, 0); resolution 0: no grid (groups).
+// res z bits), (1 / cell size, 0).
 struct DomGrid {
   float lo[3], hi[3], h[3], inv[3];
   int r[3];
@@ -264,88 +251,19 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
 // One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
 // entries sorted (registers), the domains hit as a bit mask (one word per 64
 // domains, this lane's words at mask[w * stride]) so that the scatter
-// re-tests only those, and the number of domains hit.  Below kGridMinDomains
-// domains are visited in groups of kDomGroup (Morton order of their
-// centres), groups in
-// supergroups of kGroupsPerSuper: a supergroup's groups are tested when some
-// lane of the wave hits its box, and each lane then tests the members of
-// only the groups it hits itself.  The member tests are the exact ones and
-// the list is kept sorted by (t, id), so the result does not depend on the
-// grouping or on the order the groups are visited in.
-__device__ __forceinline__ int rayDomainsGroups(const float4* box, const float4* gbox,
-                                                const float4* sbox, const uint16_t* perm, int nd,
-                                                const float o[3], const float inv[3], bool safe,
-                                                int spawn, uint32_t* lc_kind, float tt[kTopDepths],
-                                                int ti[kTopDepths], int* kept_o,
-                                                unsigned long long* mask, int stride) {
-  int kept = 0, hits = 0;
-  const int ng = numGroups(nd), ns = numSupers(nd);
-  for (int sg = 0; sg < ns; ++sg) {
-    const bool hs = slabBoxClosed(sbox[2 * sg], sbox[2 * sg + 1], o, inv, safe);
-    if (__ballot(hs) == 0ull) continue;
-    const int g0 = sg * kGroupsPerSuper, g1 = min(ng, g0 + kGroupsPerSuper);
-    uint32_t gm = 0;
-    if (hs)
-      for (int g = g0; g < g1; ++g)
-        if (slabBoxClosed(gbox[2 * g], gbox[2 * g + 1], o, inv, safe)) gm |= 1u << (g - g0);
-    while (__ballot(gm != 0u) != 0ull) {
-      if (gm == 0u) continue;
-      const int g = g0 + __ffs(gm) - 1;
-      gm &= gm - 1u;
-      const int k1 = min(nd, (g + 1) * kDomGroup);
-      for (int k = g * kDomGroup; k < k1; ++k) {
-        const int dd = perm[k];
-        float tmin;
-        // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
-        // spawning domain was tested right after shading
-        if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
-        ++hits;
-        atomicAdd(&lc_kind[dd], 1u);
-        mask[(dd >> 6) * stride] |= 1ull << (dd & 63);
-        // insertion into the (t, id)-sorted top list (unrolled: stays in
-        // registers); steps past every active lane's list end are skipped
-        if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
-          float ct = tmin;
-          int ci = dd;
-#pragma unroll
-          for (int j = 0; j < kTopDepths; ++j) {
-            if (__ballot(j <= kept) == 0ull) break;
-            if (j < kept) {
-              if (ct < tt[j] || (ct == tt[j] && ci < ti[j])) {
-                float xt = tt[j];
-                int xi = ti[j];
-                tt[j] = ct;
-                ti[j] = ci;
-                ct = xt;
-                ci = xi;
-              }
-            } else if (j == kept) {
-              tt[j] = ct;
-              ti[j] = ci;
-            }
-          }
-          if (kept < kTopDepths) ++kept;
-        }
-      }
-    }
-  }
-  *kept_o = kept;
-  return hits;
-}
-
-// One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
-// entries sorted (registers), the domains hit as a bit mask (one word per 64
-// domains, this lane's words at mask[w * stride]) so that the scatter
-// re-tests only those, and the number of domains hit.  From kGridMinDomains
-// domains on, candidates come from a 3D-DDA walk of the domain grid over the
-// ray's t range inside the grid (from max(0, entry)); every domain a ray
-// hits overlaps a cell the walk
-// visits (cells list the domains of boxes grown by a thousandth of a cell,
-// which absorbs the walk's rounding at cell faces and corners).  Each
-// candidate not already marked gets the exact slab test, and the list is
-// kept sorted by (t, id), so the result does not depend on the grid or on
-// the visiting order.
-__device__ __forceinline__ int rayDomainsGrid(const float4* box, const DomGrid& G,
+// re-tests only those, and the number of domains hit.  Candidates come from
+// a 3D-DDA walk of the domain grid over the ray's t range inside the grid
+// (from max(0, entry)).  The walk is a
+// supercover where it matters: when the ray crosses two or three cell faces
+// at the same t (within kGridTol of t), it also visits the cells of the
+// partial steps, and it starts with the neighbours across the faces its
+// first point lies on (within kGridTol of a cell), so every cell whose
+// closed region the ray touches is visited and a domain touched only on a
+// cell face, edge or corner is still a candidate.  Each candidate not
+// already marked gets the exact slab test, and the list is kept sorted by
+// (t, id), so the result does not depend on the grid or on the visiting
+// order.
+__device__ __forceinline__ int rayDomains(const float4* box, const DomGrid& G,
                                               const uint32_t* __restrict__ coff,
                                               const uint16_t* __restrict__ cids, const float o[3],
                                               const float d[3], const float inv[3], bool safe,
@@ -369,65 +287,92 @@ __device__ __forceinline__ int rayDomainsGrid(const float4* box, const DomGrid& 
       in = false;
     }
   }
+  // candidates of one cell (cells outside the grid have none)
+  auto visit = [&](int x, int y, int z) {
+    if (x < 0 || y < 0 || z < 0 || x >= G.r[0] || y >= G.r[1] || z >= G.r[2]) return;
+    const int cell = (z * G.r[1] + y) * G.r[0] + x;
+    const uint32_t q1 = coff[cell + 1];
+    for (uint32_t q = coff[cell]; q < q1; ++q) {
+      const int dd = cids[q];
+      unsigned long long* mw = &mask[(dd >> 6) * stride];
+      if ((*mw >> (dd & 63)) & 1ull) continue;  // already a hit
+      float tmin;
+      // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
+      // spawning domain was tested right after shading
+      if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
+      ++hits;
+      atomicAdd(&lc_kind[dd], 1u);
+      *mw |= 1ull << (dd & 63);
+      // insertion into the (t, id)-sorted top list (unrolled: stays in
+      // registers); steps past every active lane's list end are skipped
+      if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
+        float ct = tmin;
+        int ci = dd;
+#pragma unroll
+        for (int j = 0; j < kTopDepths; ++j) {
+          if (__ballot(j <= kept) == 0ull) break;
+          if (j < kept) {
+            if (ct < tt[j] || (ct == tt[j] && ci < ti[j])) {
+              float xt = tt[j];
+              int xi = ti[j];
+              tt[j] = ct;
+              ti[j] = ci;
+              ct = xt;
+              ci = xi;
+            }
+          } else if (j == kept) {
+            tt[j] = ct;
+            ti[j] = ci;
+          }
+        }
+        if (kept < kTopDepths) ++kept;
+      }
+    }
+  };
   if (in && t0 <= t1) {
-    int c[3], st[3];
+    int c[3], st[3], nb[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float p = o[k] + t0 * d[k];
-      const int ck = (int)floorf((p - G.lo[k]) * G.inv[k]);
-      c[k] = min(max(ck, 0), G.r[k] - 1);
+      const float u = (p - G.lo[k]) * G.inv[k];
+      c[k] = min(max((int)floorf(u), 0), G.r[k] - 1);
       st[k] = !moves[k] ? 0 : (d[k] > 0.f ? 1 : -1);
+      const float fr = u - (float)c[k];
+      nb[k] = (fr < kGridTol && c[k] > 0) ? -1 : (fr > 1.f - kGridTol && c[k] < G.r[k] - 1) ? 1 : 0;
     }
+    // the neighbours across faces the first point lies on
+    const int nbm = (nb[0] != 0 ? 1 : 0) | (nb[1] != 0 ? 2 : 0) | (nb[2] != 0 ? 4 : 0);
+    for (int sub = 1; sub < 8; ++sub)
+      if ((sub & ~nbm) == 0)
+        visit(c[0] + ((sub & 1) ? nb[0] : 0), c[1] + ((sub & 2) ? nb[1] : 0),
+              c[2] + ((sub & 4) ? nb[2] : 0));
     const int max_steps = G.r[0] + G.r[1] + G.r[2] + 3;
     for (int step = 0; step < max_steps; ++step) {
-      const int cell = (c[2] * G.r[1] + c[1]) * G.r[0] + c[0];
-      const uint32_t q1 = coff[cell + 1];
-      for (uint32_t q = coff[cell]; q < q1; ++q) {
-        const int dd = cids[q];
-        unsigned long long* mw = &mask[(dd >> 6) * stride];
-        if ((*mw >> (dd & 63)) & 1ull) continue;  // already a hit
-        float tmin;
-        // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
-        // spawning domain was tested right after shading
-        if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
-        ++hits;
-        atomicAdd(&lc_kind[dd], 1u);
-        *mw |= 1ull << (dd & 63);
-        // insertion into the (t, id)-sorted top list (unrolled: stays in
-        // registers); steps past every active lane's list end are skipped
-        if (kept < kTopDepths || tmin < tt[kTopDepths - 1]) {
-          float ct = tmin;
-          int ci = dd;
-#pragma unroll
-          for (int j = 0; j < kTopDepths; ++j) {
-            if (__ballot(j <= kept) == 0ull) break;
-            if (j < kept) {
-              if (ct < tt[j] || (ct == tt[j] && ci < ti[j])) {
-                float xt = tt[j];
-                int xi = ti[j];
-                tt[j] = ct;
-                ti[j] = ci;
-                ct = xt;
-                ci = xi;
-              }
-            } else if (j == kept) {
-              tt[j] = ct;
-              ti[j] = ci;
-            }
-          }
-          if (kept < kTopDepths) ++kept;
-        }
-      }
-      // next cell: the axis whose cell face the ray crosses first
+      visit(c[0], c[1], c[2]);
+      // next cell: the axes whose cell faces the ray crosses first
       float tn[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k)
         tn[k] = st[k] == 0 ? kInfF
                            : (G.lo[k] + (float)(c[k] + (st[k] > 0 ? 1 : 0)) * G.h[k] - o[k]) * inv[k];
-      const int a = tn[0] <= tn[1] ? (tn[0] <= tn[2] ? 0 : 2) : (tn[1] <= tn[2] ? 1 : 2);
-      if (!(tn[a] <= t1)) break;
-      c[a] += st[a];
-      if (c[a] < 0 || c[a] >= G.r[a]) break;
+      const float m = fminf(fminf(tn[0], tn[1]), tn[2]);
+      if (!(m <= t1)) break;
+      const float lim = m + kGridTol * fmaxf(1.f, fabsf(m));
+      const int tm = (st[0] != 0 && tn[0] <= lim ? 1 : 0) | (st[1] != 0 && tn[1] <= lim ? 2 : 0) |
+                     (st[2] != 0 && tn[2] <= lim ? 4 : 0);
+      // crossings at (nearly) the same t: the cells of the partial steps too
+      for (int sub = 1; sub < 7; ++sub)
+        if ((sub & ~tm) == 0 && sub != tm)
+          visit(c[0] + ((sub & 1) ? st[0] : 0), c[1] + ((sub & 2) ? st[1] : 0),
+                c[2] + ((sub & 4) ? st[2] : 0));
+      bool out = false;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (tm & (1 << k)) {
+          c[k] += st[k];
+          out = out || c[k] < 0 || c[k] >= G.r[k];
+        }
+      if (out) break;
     }
   }
   *kept_o = kept;
@@ -437,27 +382,20 @@ __device__ __forceinline__ int rayDomainsGrid(const float4* box, const DomGrid& 
 // Isector::isectAll / isectWithoutCurrentDomain (ooc_isector.h:126-174) as
 // counts: entries per domain and the DomainStats score
 // (ooc_domain_stats.cc:60-88) weight max(0, 16 - position in the ray's list).
-// kGrid: the domain-grid form (scenes of kGridMinDomains domains and more),
-// else the group form; one instantiation each keeps their registers apart.
-template <bool kGrid>
 __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
-  const int nd = P.ndomains, ng = numGroups(nd), ns = numSupers(nd), nw = (nd + 63) >> 6;
-  const CountLds L = countLds(nd, blockDim.x, kGrid);
+  const int nd = P.ndomains, nw = (nd + 63) >> 6;
+  const CountLds L = countLds(nd, blockDim.x);
   unsigned char* smc = (unsigned char*)sm;
   float4* box = (float4*)smc;                                 // 2 * nd: (lo, 0), (hi, 0)
-  float4* gbox = (float4*)(smc + L.gbox);                     // 2 * ng
-  float4* sbox = (float4*)(smc + L.sbox);                     // 2 * ns
   uint32_t* lc = (uint32_t*)(smc + L.lc);                     // 2 * nd counts
   uint32_t* ls = (uint32_t*)(smc + L.ls);                     // nd score (32-bit per block)
   unsigned long long* lmask = (unsigned long long*)(smc + L.mask) + threadIdx.x;
-  uint16_t* perm = (uint16_t*)(smc + L.perm);                 // nd
-  for (int i = threadIdx.x; i < 2 * (kGrid ? nd : nd + ng + ns); i += blockDim.x) box[i] = P.dbox[i];
-  const uint16_t* dperm = (const uint16_t*)(P.dbox + 2 * (nd + ng + ns));
+  for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
   DomGrid G;
-  if (kGrid) {
-    const float4* gh = (const float4*)((const char*)P.dbox + groupBytes(nd));
+  {
+    const float4* gh = (const float4*)((const char*)P.dbox + gridHeadBytes(nd));
     const float4 a = gh[0], b = gh[1], h = gh[2], v = gh[3];
     G.lo[0] = a.x, G.lo[1] = a.y, G.lo[2] = a.z;
     G.hi[0] = b.x, G.hi[1] = b.y, G.hi[2] = b.z;
@@ -471,7 +409,6 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     lc[i] = 0;
     lc[nd + i] = 0;
     ls[i] = 0;
-    if (!kGrid) perm[i] = dperm[i];
   }
   __syncthreads();
   const DevCounters* c = P.ctr;
@@ -509,12 +446,8 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
     if (valid) {
       for (int w = 0; w < nw; ++w) lmask[w * blockDim.x] = 0ull;
-      if constexpr (kGrid)
-        hits = rayDomainsGrid(box, G, coff, cids, o, dir, inv, safe, spawn, lc + kind * nd, tt, ti,
-                              &kept, lmask, blockDim.x);
-      else
-        hits = rayDomainsGroups(box, gbox, sbox, perm, nd, o, inv, safe, spawn, lc + kind * nd, tt,
-                                ti, &kept, lmask, blockDim.x);
+      hits = rayDomains(box, G, coff, cids, o, dir, inv, safe, spawn, lc + kind * nd, tt, ti, &kept,
+                        lmask, blockDim.x);
       for (int w = 0; w < nw; ++w) P.dmask[(size_t)i * nw + w] = lmask[w * blockDim.x];
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
@@ -1505,9 +1438,8 @@ static void ensureState(Ctx* c, double grow) {
   int nd = c->pool.ndomains;
   if (nd > 1024) throw StatusError(SPRAY_ERR_CAPACITY, "the tracer supports at most 1024 domains");
   if (countShmem(nd) > 65536)  // many domains: kCount's LDS beyond the default limit
-    SPRAY_HIP(hipFuncSetAttribute(useDomainGrid(nd) ? (const void*)kCount<true>
-                                                    : (const void*)kCount<false>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)countShmem(nd)));
+    SPRAY_HIP(hipFuncSetAttribute((const void*)kCount, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)countShmem(nd)));
   int depths = std::min(cfg.bounces, kHistory);
   double spec = 2.0 * grow;  // headroom for speculative re-shading
   auto cap = [](double v) { return (uint32_t)std::min<double>(3.9e9, v + 4096); };
@@ -1670,146 +1602,90 @@ static TraceParams makeParams(Ctx* c) {
   return P;
 }
 
-static size_t countShmem(int nd) { return countLds(nd, kBlock, useDomainGrid(nd)).total; }
+static size_t countShmem(int nd) { return countLds(nd, kBlock).total; }
 
-// Domain boxes for the ray/domain slab tests, plus kCount's culling data:
-// domains sorted by the Morton code of their box centres (ties by id),
-// kDomGroup consecutive ones per group with its union box, supergroup
-// boxes; and, from kGridMinDomains domains, the domain grid (header, cell
-// offsets, the cells' domain ids in ascending order).  Without a grid the
-// header's resolution is 0.
+// Domain boxes for the ray/domain slab tests, plus kCount's domain grid:
+// header, cell offsets and the cells' domain ids (ascending).
 static void domainBoxesUpload(Ctx* c, hipStream_t s) {
-  const bool use_grid = useDomainGrid(c->pool.ndomains);
   TracerState* T = c->tracer;
-  const int nd = c->pool.ndomains, ng = numGroups(nd), ns = numSupers(nd);
+  const int nd = c->pool.ndomains;
   const auto& D = c->pool.desc;
-  float smin[3] = {kInfF, kInfF, kInfF}, smax[3] = {-kInfF, -kInfF, -kInfF};
-  for (int d = 0; d < nd; ++d)
-    for (int k = 0; k < 3; ++k) {
-      smin[k] = std::min(smin[k], 0.5f * (D[d].lo[k] + D[d].hi[k]));
-      smax[k] = std::max(smax[k], 0.5f * (D[d].lo[k] + D[d].hi[k]));
-    }
-  std::vector<std::pair<uint32_t, int>> key(nd);
-  for (int d = 0; d < nd; ++d) {
-    uint32_t code = 0;
-    for (int k = 0; k < 3; ++k) {
-      const float ext = smax[k] - smin[k];
-      const float x = ext > 0.f ? (0.5f * (D[d].lo[k] + D[d].hi[k]) - smin[k]) / ext : 0.f;
-      const uint32_t q = (uint32_t)std::min(1023.f, std::max(0.f, x * 1023.f));
-      for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + (2 - k));
-    }
-    key[d] = {code, d};
-  }
-  std::sort(key.begin(), key.end());
-  float glo[3] = {0.f, 0.f, 0.f}, ghi[3] = {0.f, 0.f, 0.f}, h[3] = {1.f, 1.f, 1.f},
-        inv[3] = {1.f, 1.f, 1.f};
-  int r[3] = {0, 0, 0};
+  float glo[3], ghi[3], h[3], inv[3];
+  int r[3];
   std::vector<uint32_t> off;
   std::vector<uint16_t> ids;
-  if (use_grid) {
-    // grid over the union of the domain boxes, resolution from the mean box
-    // extent per axis (one cell per domain on a lattice of domains)
-    float avg[3];
-    for (int k = 0; k < 3; ++k) {
-      glo[k] = kInfF;
-      ghi[k] = -kInfF;
-      double sum = 0;
-      for (int d = 0; d < nd; ++d) {
-        glo[k] = std::min(glo[k], D[d].lo[k]);
-        ghi[k] = std::max(ghi[k], D[d].hi[k]);
-        sum += (double)D[d].hi[k] - D[d].lo[k];
-      }
-      avg[k] = (float)(sum / nd);
+  // grid over the union of the domain boxes, resolution from the mean box
+  // extent per axis (one cell per domain on a lattice of domains)
+  float avg[3];
+  for (int k = 0; k < 3; ++k) {
+    glo[k] = kInfF;
+    ghi[k] = -kInfF;
+    double sum = 0;
+    for (int d = 0; d < nd; ++d) {
+      glo[k] = std::min(glo[k], D[d].lo[k]);
+      ghi[k] = std::max(ghi[k], D[d].hi[k]);
+      sum += (double)D[d].hi[k] - D[d].lo[k];
     }
+    avg[k] = (float)(sum / nd);
+  }
+  for (int k = 0; k < 3; ++k) {
+    const float ext = ghi[k] - glo[k];
+    r[k] = (ext > 0.f && avg[k] > 0.f) ? (int)std::lround(ext / avg[k]) : 1;
+    r[k] = std::max(1, std::min(kGridMaxRes, r[k]));
+  }
+  for (;;) {
+    while ((long long)r[0] * r[1] * r[2] > kGridCells) *std::max_element(r, r + 3) /= 2;
     for (int k = 0; k < 3; ++k) {
       const float ext = ghi[k] - glo[k];
-      r[k] = (ext > 0.f && avg[k] > 0.f) ? (int)std::lround(ext / avg[k]) : 1;
-      r[k] = std::max(1, std::min(kGridMaxRes, r[k]));
+      h[k] = ext > 0.f ? ext / (float)r[k] : 1.f;
+      inv[k] = 1.f / h[k];
     }
-    for (;;) {
-      while ((long long)r[0] * r[1] * r[2] > kGridCells) *std::max_element(r, r + 3) /= 2;
-      for (int k = 0; k < 3; ++k) {
-        const float ext = ghi[k] - glo[k];
-        h[k] = ext > 0.f ? ext / (float)r[k] : 1.f;
-        inv[k] = 1.f / h[k];
-      }
-      const int ncell = r[0] * r[1] * r[2];
-      std::vector<uint32_t> cnt(ncell + 1, 0);
-      auto range = [&](int d, int k, int* c0, int* c1) {
-        const float g = kGridGrow * h[k];
-        *c0 = std::max(0, std::min(r[k] - 1, (int)std::floor((D[d].lo[k] - g - glo[k]) * inv[k])));
-        *c1 = std::max(0, std::min(r[k] - 1, (int)std::floor((D[d].hi[k] + g - glo[k]) * inv[k])));
-      };
-      long long total = 0;
-      for (int d = 0; d < nd; ++d) {
-        int a[3], b[3];
-        for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
-        total += (long long)(b[0] - a[0] + 1) * (b[1] - a[1] + 1) * (b[2] - a[2] + 1);
-      }
-      if (total > (long long)kGridIdsPerDomain * nd && r[0] * r[1] * r[2] > 1) {
-        *std::max_element(r, r + 3) /= 2;  // domains span too many cells: coarser grid
-        continue;
-      }
-      for (int d = 0; d < nd; ++d) {
-        int a[3], b[3];
-        for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
-        for (int z = a[2]; z <= b[2]; ++z)
-          for (int y = a[1]; y <= b[1]; ++y)
-            for (int x = a[0]; x <= b[0]; ++x) ++cnt[(z * r[1] + y) * r[0] + x];
-      }
-      off.assign(ncell + 1, 0);
-      for (int i = 0; i < ncell; ++i) off[i + 1] = off[i] + cnt[i];
-      ids.assign(off[ncell], 0);
-      std::vector<uint32_t> cur(off.begin(), off.end() - 1);
-      for (int d = 0; d < nd; ++d) {  // ascending domain ids within a cell
-        int a[3], b[3];
-        for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
-        for (int z = a[2]; z <= b[2]; ++z)
-          for (int y = a[1]; y <= b[1]; ++y)
-            for (int x = a[0]; x <= b[0]; ++x) ids[cur[(z * r[1] + y) * r[0] + x]++] = (uint16_t)d;
-      }
-      break;
+    const int ncell = r[0] * r[1] * r[2];
+    std::vector<uint32_t> cnt(ncell + 1, 0);
+    auto range = [&](int d, int k, int* c0, int* c1) {
+      *c0 = std::max(0, std::min(r[k] - 1, (int)std::floor((D[d].lo[k] - glo[k]) * inv[k])));
+      *c1 = std::max(0, std::min(r[k] - 1, (int)std::ceil((D[d].hi[k] - glo[k]) * inv[k]) - 1));
+      *c1 = std::max(*c0, *c1);
+    };
+    long long total = 0;
+    for (int d = 0; d < nd; ++d) {
+      int a[3], b[3];
+      for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
+      total += (long long)(b[0] - a[0] + 1) * (b[1] - a[1] + 1) * (b[2] - a[2] + 1);
     }
+    if (total > (long long)kGridIdsPerDomain * nd && r[0] * r[1] * r[2] > 1) {
+      *std::max_element(r, r + 3) /= 2;  // domains span too many cells: coarser grid
+      continue;
+    }
+    for (int d = 0; d < nd; ++d) {
+      int a[3], b[3];
+      for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
+      for (int z = a[2]; z <= b[2]; ++z)
+        for (int y = a[1]; y <= b[1]; ++y)
+          for (int x = a[0]; x <= b[0]; ++x) ++cnt[(z * r[1] + y) * r[0] + x];
+    }
+    off.assign(ncell + 1, 0);
+    for (int i = 0; i < ncell; ++i) off[i + 1] = off[i] + cnt[i];
+    ids.assign(off[ncell], 0);
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (int d = 0; d < nd; ++d) {  // ascending domain ids within a cell
+      int a[3], b[3];
+      for (int k = 0; k < 3; ++k) range(d, k, &a[k], &b[k]);
+      for (int z = a[2]; z <= b[2]; ++z)
+        for (int y = a[1]; y <= b[1]; ++y)
+          for (int x = a[0]; x <= b[0]; ++x) ids[cur[(z * r[1] + y) * r[0] + x]++] = (uint16_t)d;
+    }
+    break;
   }
-  const size_t bytes = use_grid ? gridIdsBytes(nd) + ids.size() * 2 : gridOffBytes(nd);
+  const size_t bytes = gridIdsBytes(nd) + ids.size() * 2;
   T->h_box.assign((bytes + 3) / 4, 0.f);
   float* box = T->h_box.data();
-  float* gbox = box + (size_t)nd * 8;
-  float* sbox = gbox + (size_t)ng * 8;
-  uint16_t* perm = (uint16_t*)(sbox + (size_t)ns * 8);
   for (int d = 0; d < nd; ++d)
     for (int k = 0; k < 3; ++k) {
       box[(size_t)d * 8 + k] = D[d].lo[k];
       box[(size_t)d * 8 + 4 + k] = D[d].hi[k];
     }
-  for (int g = 0; g < ng; ++g) {
-    float* gb = gbox + (size_t)g * 8;
-    for (int k = 0; k < 3; ++k) {
-      gb[k] = kInfF;
-      gb[4 + k] = -kInfF;
-    }
-    for (int j = g * kDomGroup; j < std::min(nd, (g + 1) * kDomGroup); ++j) {
-      const int d = key[j].second;
-      perm[j] = (uint16_t)d;
-      for (int k = 0; k < 3; ++k) {
-        gb[k] = std::min(gb[k], D[d].lo[k]);
-        gb[4 + k] = std::max(gb[4 + k], D[d].hi[k]);
-      }
-    }
-  }
-  for (int q = 0; q < ns; ++q) {  // supergroups: unions of consecutive groups
-    float* sb = sbox + (size_t)q * 8;
-    for (int k = 0; k < 3; ++k) {
-      sb[k] = kInfF;
-      sb[4 + k] = -kInfF;
-    }
-    for (int g = q * kGroupsPerSuper; g < std::min(ng, (q + 1) * kGroupsPerSuper); ++g)
-      for (int k = 0; k < 3; ++k) {
-        sb[k] = std::min(sb[k], gbox[(size_t)g * 8 + k]);
-        sb[4 + k] = std::max(sb[4 + k], gbox[(size_t)g * 8 + 4 + k]);
-      }
-  }
-  float* gh = (float*)((char*)box + groupBytes(nd));
+  float* gh = (float*)((char*)box + gridHeadBytes(nd));
   for (int k = 0; k < 3; ++k) {
     gh[k] = glo[k];
     gh[4 + k] = ghi[k];
@@ -1819,10 +1695,8 @@ static void domainBoxesUpload(Ctx* c, hipStream_t s) {
   std::memcpy(&gh[3], &r[0], 4);
   std::memcpy(&gh[7], &r[1], 4);
   std::memcpy(&gh[11], &r[2], 4);
-  if (use_grid) {
-    std::memcpy((char*)box + gridOffBytes(nd), off.data(), off.size() * 4);
-    std::memcpy((char*)box + gridIdsBytes(nd), ids.data(), ids.size() * 2);
-  }
+  std::memcpy((char*)box + gridOffBytes(nd), off.data(), off.size() * 4);
+  std::memcpy((char*)box + gridIdsBytes(nd), ids.data(), ids.size() * 2);
   SPRAY_HIP(hipMemcpyAsync(T->dbox.p, T->h_box.data(), bytes, hipMemcpyHostToDevice, s));
 }
 
@@ -1837,10 +1711,7 @@ static void passQueues(Ctx* c, const TraceParams& P) {
   int nd = P.ndomains;
   kPassBegin<<<1, 1, 0, s>>>(P);
   kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
-  if (useDomainGrid(nd))
-    kCount<true><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
-  else
-    kCount<false><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+  kCount<<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
   kSchedule<<<1, 1024, 0, s>>>(P);
   kBlockBases<<<2 * nd, kBlock, 0, s>>>(P);
   kScatter<<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);

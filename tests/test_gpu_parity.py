@@ -225,10 +225,11 @@ def test_long_domain_lists(tmp_path):
     assert rmse <= 1e-4 and mx <= 1e-3
 
 
-def test_partial_domain_groups(tmp_path):
-    """27 domains (3x3x3): the domain-list pass culls in groups of 8 in Morton
-    order, so here the groups straddle the grid and the last one is partial.
-    Path-traced frames (batched and plain) match the oracle."""
+def test_odd_domain_lattice(tmp_path):
+    """27 domains (3x3x3), boxes that fill their lattice cells: the domain
+    grid lists each domain once and rays graze cell faces, edges and corners
+    of neighbouring domains.  Path-traced frames (batched and plain) match
+    the oracle."""
     import spray_amd as sa
     d = str(tmp_path)
     sa.scenegen_write(d, "g27", 3, 3, 3, 24, rgb=0xB3B3B3,
@@ -281,9 +282,9 @@ def test_axis_parallel_rays(ctx_c1, ref_c1):
 
 @pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "batched"])
 def test_domain_grid_frame_matches_oracle(tmp_path, schedule):
-    """512 domains (8x8x8): the domain-list pass walks the domain grid (scenes
-    of 512 domains and more) instead of the domain groups; the frame matches
-    the oracle's brute-force domain lists."""
+    """512 domains (8x8x8), the largest scene among the parity tests: the
+    domain-list pass walks the domain grid; the frame matches the oracle's
+    brute-force domain lists."""
     import spray_amd as sa
     d = str(tmp_path)
     sa.scenegen_write(d, "g", 8, 8, 8, 8, rgb=0xB3B3B3, extra_lines="light rect 1 1 8.5 0 6 0 6 0 0 4 4 4")
