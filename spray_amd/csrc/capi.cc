@@ -164,6 +164,7 @@ void spray_destroy(spray_ctx* ctx) {
   ctx->c.pool.release();
   ctx->c.q_in.release();
   ctx->c.q_out.release();
+  ctx->c.gen_store.release();
   if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }

@@ -156,8 +156,10 @@ struct Ctx {
   std::string err;
   // scratch for spray_trace_rays / spray_domain_list
   DevBuf q_in, q_out;
-  // a scene made by the GPU generator (spray_scenegen_load) can be remeshed
+  // a scene made by the GPU generator (spray_scenegen_load) can be remeshed;
+  // the generator's per-domain buffers stay allocated between remeshes
   bool generated = false;
+  DevBuf gen_store;
   spray_scenegen_params gen{};
   std::string gen_name, gen_extra;
 };

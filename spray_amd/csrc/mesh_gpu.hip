@@ -372,10 +372,6 @@ struct McCorners {  // per local edge: corner offset and axis
   int dx[12], dy[12], dz[12], axis[12];
 };
 
-__device__ __forceinline__ double mcF(const double* f, const McGrid& g, int i, int j, int k) {
-  return f[((long long)k * g.N + j) * g.N + i];
-}
-
 // position of edge (axis, i, j, k) in the axis' edge numbering
 __device__ __forceinline__ long long mcEdgeIndex(const McGrid& g, int axis, int i, int j, int k) {
   if (axis == 0) return ((long long)k * g.N + j) * g.n + i;
@@ -395,20 +391,23 @@ __device__ __forceinline__ void mcEdgeDecode(const McGrid& g, long long e, int* 
   *k = (int)(r / nj);
 }
 
-// Field values, and their signs as bytes (1: inside, f < 0) for the cut
-// and case tests, which then read 1 byte per grid point instead of 8.
-__global__ void __launch_bounds__(kBlk) kMcField(McGrid g, double* __restrict__ f,
-                                                 uint8_t* __restrict__ sgn) {
+// Field value at grid point (i, j, k), in the CPU generator's operations
+// and order (the file is built without contraction).
+__device__ __forceinline__ double mcFieldAt(const McGrid& g, int i, int j, int k) {
+  const double x = g.ox + (double)i * g.h, y = g.oy + (double)j * g.h, z = g.oz + (double)k * g.h;
+  return sin(g.a * x + g.phase) * sin(g.a * y) * sin(g.a * z) +
+         0.3 * sin(g.b * (x + y + z) + g.phase);
+}
+
+// Field signs as bytes (1: inside, f < 0): the cut and case tests read one
+// byte per grid point; field values are evaluated again only at the ends
+// of cut edges.
+__global__ void __launch_bounds__(kBlk) kMcSign(McGrid g, uint8_t* __restrict__ sgn) {
   const long long np = (long long)g.N * g.N * g.N;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < np;
        t += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(t % g.N), j = (int)((t / g.N) % g.N), k = (int)(t / ((long long)g.N * g.N));
-    const double x = g.ox + (double)i * g.h, y = g.oy + (double)j * g.h,
-                 z = g.oz + (double)k * g.h;
-    const double v = sin(g.a * x + g.phase) * sin(g.a * y) * sin(g.a * z) +
-                     0.3 * sin(g.b * (x + y + z) + g.phase);
-    f[t] = v;
-    sgn[t] = v < 0.0 ? 1 : 0;
+    sgn[t] = mcFieldAt(g, i, j, k) < 0.0 ? 1 : 0;
   }
 }
 
@@ -427,13 +426,11 @@ __device__ __forceinline__ void mcVertex(const McGrid& g, int axis, int i, int j
 // Edge cuts as bits: bit e of the mask is set where the field changes sign
 // along edge e; edges go in blocks of 256 (four 64-bit words), bcnt[blk] =
 // cuts in the block (bcnt[nblk] = 0, for the exclusive scan of the block
-// counts); with `part`, also the per-block bounds of the edge vertices.
-__global__ void __launch_bounds__(kBlk) kMcEdgeBits(McGrid g, const double* __restrict__ f,
-                                                    const uint8_t* __restrict__ sgn,
+// counts).
+__global__ void __launch_bounds__(kBlk) kMcEdgeBits(McGrid g, const uint8_t* __restrict__ sgn,
                                                     unsigned long long* __restrict__ bits,
-                                                    uint32_t* __restrict__ bcnt, float* part) {
+                                                    uint32_t* __restrict__ bcnt) {
   __shared__ uint32_t wsum[kBlk / 64];
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   const long long ne = 3 * g.E, nblk = (ne + kBlk - 1) / kBlk;
   const int w = threadIdx.x >> 6;
   for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
@@ -443,16 +440,6 @@ __global__ void __launch_bounds__(kBlk) kMcEdgeBits(McGrid g, const double* __re
       int axis, i, j, k;
       mcEdgeDecode(g, e, &axis, &i, &j, &k);
       cut = mcS(sgn, g, i, j, k) != mcS(sgn, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
-      if (part && cut) {
-        const double f0 = mcF(f, g, i, j, k);
-        const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
-        float p[3];
-        mcVertex(g, axis, i, j, k, f0, f1, p);
-        for (int c = 0; c < 3; ++c) {
-          lo[c] = fminf(lo[c], p[c]);
-          hi[c] = fmaxf(hi[c], p[c]);
-        }
-      }
     }
     const unsigned long long m = __ballot(cut);
     if ((threadIdx.x & 63) == 0) {
@@ -468,7 +455,6 @@ __global__ void __launch_bounds__(kBlk) kMcEdgeBits(McGrid g, const double* __re
     __syncthreads();
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) bcnt[nblk] = 0u;
-  if (part) blockBounds(lo, hi, part);
 }
 
 // Vertex id of a cut edge: its block's offset plus the cuts before it in
@@ -482,25 +468,47 @@ __device__ __forceinline__ uint32_t mcEdgeId(const unsigned long long* __restric
   return id + (uint32_t)__popcll(bits[blk * (kBlk / 64) + w] & ((1ull << b) - 1ull));
 }
 
-__global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g, const double* __restrict__ f,
+// Vertices of the cut edges, one thread per 64-bit word of the cut mask
+// (its cuts in edge order), so a wave evaluates the field only as often as
+// its busiest word has cuts.  With v == nullptr only the per-block bounds of
+// the vertices are produced (part, one slot per block).
+__global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g,
                                                  const unsigned long long* __restrict__ bits,
                                                  const uint32_t* __restrict__ eoff, uint32_t color,
-                                                 float* __restrict__ v,
-                                                 uint32_t* __restrict__ rgb) {
-  const long long ne = 3 * g.E;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ne;
-       e += (long long)gridDim.x * blockDim.x) {
-    if (!((bits[e >> 6] >> (e & 63)) & 1ull)) continue;
-    const uint32_t id = mcEdgeId(bits, eoff, e);
-    int axis, i, j, k;
-    mcEdgeDecode(g, e, &axis, &i, &j, &k);
-    const double f0 = mcF(f, g, i, j, k);
-    const double f1 = mcF(f, g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
-    float p[3];
-    mcVertex(g, axis, i, j, k, f0, f1, p);
-    for (int c = 0; c < 3; ++c) v[(size_t)id * 3 + c] = p[c];
-    rgb[id] = color;
+                                                 float* __restrict__ v, uint32_t* __restrict__ rgb,
+                                                 float* part) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  const long long nwords = ((3 * g.E + kBlk - 1) / kBlk) * (kBlk / 64);
+  for (long long wi = blockIdx.x * (long long)blockDim.x + threadIdx.x; wi < nwords;
+       wi += (long long)gridDim.x * blockDim.x) {
+    unsigned long long m = bits[wi];
+    if (m == 0ull) continue;
+    const long long blk = wi / (kBlk / 64);
+    uint32_t id = eoff[blk];
+    for (long long q = blk * (kBlk / 64); q < wi; ++q) id += (uint32_t)__popcll(bits[q]);
+    while (m) {
+      const int b = __ffsll((long long)m) - 1;
+      m &= m - 1ull;
+      const long long e = wi * 64 + b;
+      int axis, i, j, k;
+      mcEdgeDecode(g, e, &axis, &i, &j, &k);
+      const double f0 = mcFieldAt(g, i, j, k);
+      const double f1 = mcFieldAt(g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
+      float p[3];
+      mcVertex(g, axis, i, j, k, f0, f1, p);
+      if (v) {
+        for (int c = 0; c < 3; ++c) v[(size_t)id * 3 + c] = p[c];
+        rgb[id] = color;
+      } else {
+        for (int c = 0; c < 3; ++c) {
+          lo[c] = fminf(lo[c], p[c]);
+          hi[c] = fmaxf(hi[c], p[c]);
+        }
+      }
+      ++id;
+    }
   }
+  if (part) blockBounds(lo, hi, part);
 }
 
 __device__ __forceinline__ int mcCase(const McGrid& g, const uint8_t* sgn, int i, int j, int k) {
@@ -590,18 +598,63 @@ __global__ void kMcTotals(const uint32_t* vtotal, const uint32_t* ftotal, uint32
   out[1] = *ftotal;
 }
 
+// Per domain, the count pass leaves what the emission needs: sign bytes,
+// cut bits and the scanned block offsets of edges and cells.  When those
+// buffers for every domain would exceed kMcKeepBudget (or a quarter of the
+// free HBM), they are kept for none and the emission recomputes them per
+// domain into scratch.
+constexpr size_t kMcKeepBudget = (size_t)16 << 30;
+
 struct McSource : MeshSource {
+  struct Parts {  // byte offsets of one domain's buffers
+    size_t sgn, bits, eoff, coff;
+  };
   spray_scenegen_params p;
   int nd;
+  bool keep = false;
   std::vector<McGrid> grid;
+  std::vector<Parts> parts;
   std::vector<long long> nv, nf;
   std::vector<float> bounds;  // 6 per domain: lo xyz, hi xyz
-  DevBuf field, sign, ebits, ecnt, ccnt, table, ntri, part, dev_out;
+  DevBuf& store;  // the context's generator store (kept between remeshes)
+  DevBuf table, ntri, part, dev_out;
   McCorners ec;
-  McSource(Ctx* c, const spray_scenegen_params& prm) : p(prm) {
+  static long long edgeBlocks(const McGrid& g) { return (3 * g.E + kBlk - 1) / kBlk; }
+  static long long cellBlocks(const McGrid& g) {
+    return ((long long)g.n * g.n * g.n + kBlk - 1) / kBlk;
+  }
+  static Parts sizes(const McGrid& g) {  // bytes of each buffer, 256-aligned
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return Parts{al((size_t)g.N * g.N * g.N), al((size_t)edgeBlocks(g) * (kBlk / 64) * 8),
+                 al((size_t)(edgeBlocks(g) + 1) * 4), al((size_t)(cellBlocks(g) + 1) * 4)};
+  }
+  // sign bytes, cut bits and scanned block offsets of domain d; with bnd,
+  // also its vertex bounds, and with tot its vertex and face totals
+  void count(const McGrid& g, const Parts& at, float* bnd, uint32_t* tot, hipStream_t s) {
+    uint8_t* sgn = store.as<uint8_t>() + at.sgn;
+    unsigned long long* bits = (unsigned long long*)(store.as<char>() + at.bits);
+    uint32_t* eoff = (uint32_t*)(store.as<char>() + at.eoff);
+    uint32_t* coff = (uint32_t*)(store.as<char>() + at.coff);
+    const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
+    const long long eb = edgeBlocks(g), cb = cellBlocks(g);
+    kMcSign<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, sgn);
+    kMcEdgeBits<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, sgn, bits, eoff);
+    exclusiveScan(eoff, eoff, eb + 1, s);
+    if (bnd) {
+      const int vgrid = gridOf(eb * (kBlk / 64), kBlk, kBoundBlocks);
+      kMcVerts<<<vgrid, kBlk, 0, s>>>(g, bits, eoff, 0u, nullptr, nullptr, part.as<float>());
+      kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), vgrid, bnd);
+    }
+    kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sgn, ntri.as<uint8_t>(), coff);
+    exclusiveScan(coff, coff, cb + 1, s);
+    if (tot) kMcTotals<<<1, 1, 0, s>>>(eoff + eb, coff + cb, tot);
+  }
+  McSource(Ctx* c, const spray_scenegen_params& prm) : p(prm), store(c->gen_store) {
     nd = p.nx * p.ny * p.nz;
     grid.resize(nd);
-    long long max_pts = 0, max_e = 0, max_c = 0;
+    parts.resize(nd);
+    Parts mx{0, 0, 0, 0};
+    size_t total = 0;
     for (int d = 0; d < nd; ++d) {
       double a, b;
       int n;
@@ -617,18 +670,29 @@ struct McSource : MeshSource {
       g.a = a;
       g.b = b;
       g.phase = p.phase;
-      max_pts = std::max(max_pts, (long long)g.N * g.N * g.N);
-      max_e = std::max(max_e, 3 * g.E + 1);
-      max_c = std::max(max_c, (long long)n * n * n + 1);
+      if (3 * g.E + 1 >= (1ll << 32) || (long long)n * n * n + 1 >= (1ll << 32))
+        throw StatusError(SPRAY_ERR_CAPACITY, "scenegen: grid too fine for 32-bit numbering");
+      const Parts z = sizes(g);
+      mx = Parts{std::max(mx.sgn, z.sgn), std::max(mx.bits, z.bits), std::max(mx.eoff, z.eoff),
+                 std::max(mx.coff, z.coff)};
+      total += z.sgn + z.bits + z.eoff + z.coff;
     }
-    if (max_e >= (1ll << 32) || max_c >= (1ll << 32))
-      throw StatusError(SPRAY_ERR_CAPACITY, "scenegen: grid too fine for 32-bit numbering");
-    field.reserve((size_t)max_pts * 8 + 64);
-    sign.reserve((size_t)max_pts + 64);
-    const long long eblk = (max_e + kBlk - 1) / kBlk, cblk = (max_c + kBlk - 1) / kBlk;
-    ebits.reserve((size_t)eblk * (kBlk / 64) * 8 + 64);
-    ecnt.reserve((size_t)(eblk + 1) * 4 + 64);
-    ccnt.reserve((size_t)(cblk + 1) * 4 + 64);
+    size_t free_b = 0, total_b = 0;
+    SPRAY_HIP(hipMemGetInfo(&free_b, &total_b));
+    keep = total <= std::min(kMcKeepBudget, free_b / 4);
+    if (keep) {
+      size_t at = 0;
+      for (int d = 0; d < nd; ++d) {
+        const Parts z = sizes(grid[d]);
+        parts[d] = Parts{at, at + z.sgn, at + z.sgn + z.bits, at + z.sgn + z.bits + z.eoff};
+        at += z.sgn + z.bits + z.eoff + z.coff;
+      }
+      store.reserve(at + 64);
+    } else {  // one scratch set, sized for the largest domain
+      const Parts one{0, mx.sgn, mx.sgn + mx.bits, mx.sgn + mx.bits + mx.eoff};
+      for (int d = 0; d < nd; ++d) parts[d] = one;
+      store.reserve(mx.sgn + mx.bits + mx.eoff + mx.coff + 64);
+    }
     const int8_t* T = mcTriTable();
     uint8_t counts[256];
     for (int cs = 0; cs < 256; ++cs) {
@@ -647,22 +711,7 @@ struct McSource : MeshSource {
     // count pass: vertex and face totals and vertex bounds of every domain
     uint32_t* tot = dev_out.as<uint32_t>();
     float* bnd = (float*)(tot + 2 * (size_t)nd);
-    for (int d = 0; d < nd; ++d) {
-      const McGrid& g = grid[d];
-      const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
-      kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>());
-      const long long eb = (ne + kBlk - 1) / kBlk, cb = (nc + kBlk - 1) / kBlk;
-      const int egrid = gridOf(ne, kBlk, kBoundBlocks);
-      kMcEdgeBits<<<egrid, kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>(),
-                                         ebits.as<unsigned long long>(),
-                                         ecnt.as<uint32_t>(), part.as<float>());
-      kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), egrid, bnd + 6 * d);
-      exclusiveScan(ecnt.as<uint32_t>(), ecnt.as<uint32_t>(), eb + 1, s);
-      kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sign.as<uint8_t>(), ntri.as<uint8_t>(),
-                                                    ccnt.as<uint32_t>());
-      exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), cb + 1, s);
-      kMcTotals<<<1, 1, 0, s>>>(ecnt.as<uint32_t>() + eb, ccnt.as<uint32_t>() + cb, tot + 2 * d);
-    }
+    for (int d = 0; d < nd; ++d) count(grid[d], parts[d], bnd + 6 * d, tot + 2 * d, s);
     SPRAY_HIP(hipGetLastError());
     std::vector<uint32_t> h((size_t)nd * 2);
     bounds.resize((size_t)nd * 6);
@@ -677,28 +726,23 @@ struct McSource : MeshSource {
     }
   }
   ~McSource() override {
-    for (DevBuf* b : {&field, &sign, &ebits, &ecnt, &ccnt, &table, &ntri, &part, &dev_out})
-      b->release();
+    for (DevBuf* b : {&table, &ntri, &part, &dev_out}) b->release();
   }
   long long numVerts(int d) const override { return nv[d]; }
   long long numFaces(int d) const override { return nf[d]; }
   void fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) override {
     (void)c;
     const McGrid& g = grid[d];
-    const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
-    kMcField<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>());
-    const long long eb = (ne + kBlk - 1) / kBlk, cb = (nc + kBlk - 1) / kBlk;
-    unsigned long long* bits = ebits.as<unsigned long long>();
-    kMcEdgeBits<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), sign.as<uint8_t>(), bits,
-                                                  ecnt.as<uint32_t>(), nullptr);
-    exclusiveScan(ecnt.as<uint32_t>(), ecnt.as<uint32_t>(), eb + 1, s);
-    kMcVerts<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, field.as<double>(), bits, ecnt.as<uint32_t>(),
-                                              p.rgb, v, rgb);
-    kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sign.as<uint8_t>(), ntri.as<uint8_t>(),
-                                                  ccnt.as<uint32_t>());
-    exclusiveScan(ccnt.as<uint32_t>(), ccnt.as<uint32_t>(), cb + 1, s);
-    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sign.as<uint8_t>(), table.as<int8_t>(), ec, bits,
-                                              ecnt.as<uint32_t>(), ccnt.as<uint32_t>(), f);
+    const Parts& at = parts[d];
+    if (!keep) count(g, at, nullptr, nullptr, s);
+    const uint8_t* sgn = store.as<uint8_t>() + at.sgn;
+    const unsigned long long* bits = (const unsigned long long*)(store.as<char>() + at.bits);
+    const uint32_t* eoff = (const uint32_t*)(store.as<char>() + at.eoff);
+    const uint32_t* coff = (const uint32_t*)(store.as<char>() + at.coff);
+    const long long ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
+    kMcVerts<<<gridOf(edgeBlocks(g) * (kBlk / 64), kBlk), kBlk, 0, s>>>(g, bits, eoff, p.rgb, v, rgb,
+                                                                       nullptr);
+    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sgn, table.as<int8_t>(), ec, bits, eoff, coff, f);
     SPRAY_HIP(hipGetLastError());
   }
 };
@@ -715,6 +759,7 @@ long long domainVerts(const DomainPool& P, int d) { return P.nverts[d]; }
 void loadScenePly(Ctx* c, HostScene& scene) {
   c->scene = std::move(scene);
   c->generated = false;
+  c->gen_store.release();
   PlySource src(c);
   requireLight(c, c->scene);
   tracerRelease(c);
