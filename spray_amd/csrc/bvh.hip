@@ -542,6 +542,18 @@ __global__ void kCollapse(const FrontierItem* __restrict__ fin, const int* __res
   }
 }
 
+// Collapse state of one domain: node counter 1 (the root), no overflow,
+// one frontier item (the binary root) at level 0, empty later levels.
+__global__ void kCollapseInit(int* cc, int nwords, FrontierItem* fa, int root) {
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) cc[i] = (i == 0 || i == 2) ? 1 : 0;
+  if (threadIdx.x == 0) fa[0] = FrontierItem{root, 0};
+}
+
+// Centroid bounds before their atomic min / max (as ordered bits).
+__global__ void kBoundsInit(uint32_t* b) {
+  if (threadIdx.x < 6) b[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
+}
+
 // Root for tiny domains (<= kLeafMax triangles): one leaf child.
 __global__ void kTinyRoot(int nf, const float* __restrict__ leafbox, BNode4* nodes) {
   BNode4 nd;
@@ -859,6 +871,27 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     }
     src.check(c);
   }
+  // A whole-scene LBVH build collapses each domain's tree on a second stream
+  // (cstream) while the next domain builds, with two scratch sets; PLOC and
+  // paged builds stay in one stream with a host check per domain.
+  const bool deferred = !paged && c->bvh_builder == 0 && nd > 1;
+  hipStream_t cstream = nullptr;
+  hipEvent_t pre_done[2] = {nullptr, nullptr}, coll_done[2] = {nullptr, nullptr};
+  DevBuf lvl_all;
+  struct Release {  // stream, events and level counts of the deferred collapse
+    hipStream_t& st;
+    hipEvent_t* a;
+    hipEvent_t* b;
+    DevBuf& lvl;
+    ~Release() {
+      if (st) (void)hipStreamDestroy(st);
+      for (int k = 0; k < 2; ++k) {
+        if (a[k]) (void)hipEventDestroy(a[k]);
+        if (b[k]) (void)hipEventDestroy(b[k]);
+      }
+      lvl.release();
+    }
+  } release_deferred{cstream, pre_done, coll_done, lvl_all};
   // one domain's normals, shading records and BVH, issued on stream s
   auto buildOne = [&](BuildScratch& S, int d, hipStream_t s) {
     DomainDesc& D = P.desc[d];
@@ -890,8 +923,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
                                                  P.fshade.as<float4>() + D.shade_off * 3);
     // Morton codes
     uint32_t* bmm = S.scal.as<uint32_t>();
-    uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-    SPRAY_HIP(hipMemcpyAsync(bmm, init, 24, hipMemcpyHostToDevice, s));
+    kBoundsInit<<<1, 64, 0, s>>>(bmm);
     kCentroidBounds<<<std::min(gridFor(nf, 256), kBoundsGrid), 256, 0, s>>>(vtx, face, nf, bmm,
                                                                             bmm + 3);
     kMorton<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, bmm, bmm + 3, S.keys.as<uint32_t>(),
@@ -1053,6 +1085,24 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       }
       kRangeBoxes<<<gridFor(nf - 1, 256), 256, 0, s>>>(nf - 1, F, Lst, leafbox, S.rpre.as<float>(),
                                                        S.rsuf.as<float>(), tab, RL, inner);
+      if (deferred) {
+        // the collapse runs on the collapse stream while this stream builds
+        // the next domain; its level counts are checked after the last domain
+        const int slot = d & 1;
+        SPRAY_HIP(hipEventRecord(pre_done[slot], s));
+        SPRAY_HIP(hipStreamWaitEvent(cstream, pre_done[slot], 0));
+        int* cc = lvl_all.as<int>() + (size_t)d * kLevelWords;
+        kCollapseInit<<<1, 64, 0, cstream>>>(cc, kLevelWords, S.fa.as<FrontierItem>(), 0);
+        const int grid = std::min(gridFor(std::max(1, nf / 8), 256), 2048);
+        for (int l = 0; l < kMaxLevels; ++l) {
+          FrontierItem* fin = (l & 1) ? S.fb.as<FrontierItem>() : S.fa.as<FrontierItem>();
+          FrontierItem* fout = (l & 1) ? S.fa.as<FrontierItem>() : S.fb.as<FrontierItem>();
+          kCollapse<<<grid, 256, 0, cstream>>>(fin, cc + 2 + l, fout, cc + 3 + l, cc, node_cap, L, R,
+                                              F, Lst, leafbox, inner, nodes, cc + 1);
+        }
+        SPRAY_HIP(hipEventRecord(coll_done[slot], cstream));
+        return;
+      }
       depth = collapse(0);
       if (!fitsStack(depth))
         throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
@@ -1061,10 +1111,45 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     pageOut();
   };
   P.bvh_depth.assign(nd, 0);
-  {
+  if (!deferred) {
     BuildScratch S;
     S.reserve(maxf, maxv, c->bvh_builder == 1);
     for (int d = 0; d < nd; ++d) buildOne(S, d, s);
+  } else {
+    // two scratch sets: domain d + 1 builds while domain d collapses
+    BuildScratch S[2];
+    for (auto& x : S) x.reserve(maxf, maxv, false);
+    lvl_all.reserve((size_t)nd * kLevelWords * 4 + 64);
+    SPRAY_HIP(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+      SPRAY_HIP(hipEventCreateWithFlags(&pre_done[k], hipEventDisableTiming));
+      SPRAY_HIP(hipEventCreateWithFlags(&coll_done[k], hipEventDisableTiming));
+    }
+    struct Drain {  // the collapse stream finishes before the scratch sets go
+      hipStream_t st;
+      ~Drain() {
+        if (st) (void)hipStreamSynchronize(st);
+      }
+    } drain{cstream};
+    for (int d = 0; d < nd; ++d) {
+      if (d >= 2) SPRAY_HIP(hipStreamWaitEvent(s, coll_done[d & 1], 0));
+      buildOne(S[d & 1], d, s);
+    }
+    SPRAY_HIP(hipStreamSynchronize(cstream));
+    SPRAY_HIP(hipStreamSynchronize(s));
+    std::vector<int> h((size_t)nd * kLevelWords);
+    SPRAY_HIP(hipMemcpy(h.data(), lvl_all.p, h.size() * 4, hipMemcpyDeviceToHost));
+    for (int d = 0; d < nd; ++d) {
+      if (P.desc[d].num_tris <= kLeafMax) continue;  // one-leaf root, built directly
+      const int* cc = &h[(size_t)d * kLevelWords];
+      if (cc[1]) throw StatusError(SPRAY_ERR_CAPACITY, "BVH node capacity exceeded");
+      int depth = 0;
+      while (depth < kMaxLevels && cc[2 + depth] > 0) ++depth;
+      if (cc[2 + kMaxLevels] > 0 || 3 * depth + 1 > kRing + kOverflow)
+        throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
+      P.desc[d].num_nodes = cc[0];
+      P.bvh_depth[d] = depth;
+    }
   }
   SPRAY_HIP(hipStreamSynchronize(s));
   long long used = 0;
