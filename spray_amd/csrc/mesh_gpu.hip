@@ -550,17 +550,40 @@ __global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const uint8_t* __
   if (blockIdx.x == 0 && threadIdx.x == 0) bcnt[nblk] = 0u;
 }
 
+// Grid cap of kMcCells: eight blocks per CU, each stages the case tables
+// into LDS once.
+constexpr long long kMcCellBlocks = 2048;
+
 // Triangles of every cell, in cell order: the block's offset plus an
-// exclusive scan of the per-cell counts inside the block.
+// exclusive scan of the per-cell counts inside the block.  The vertices of a
+// wave's triangles are then spread over its lanes, one vertex per lane, its
+// cell found by a binary search of the wave's scan, so the edge-id loads of
+// all of them are in flight together and the index stores are contiguous.
 __global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const uint8_t* __restrict__ sgn,
-                                                 const int8_t* __restrict__ table, McCorners ec,
+                                                 const int8_t* __restrict__ table,
+                                                 const uint8_t* __restrict__ ntri, McCorners ec,
                                                  const unsigned long long* __restrict__ bits,
                                                  const uint32_t* __restrict__ eoff,
                                                  const uint32_t* __restrict__ coff,
                                                  uint32_t* __restrict__ out) {
+  static_assert(kBlk == 256, "one case per thread when the tables are staged");
+  __shared__ int8_t tab[256 * 24];  // the table's rows without their terminator
+  __shared__ uint8_t cnt[256];
+  __shared__ int4 corner[12];  // dx, dy, dz, axis of each local edge
   __shared__ uint32_t wsum[kBlk / 64];
+  __shared__ uint32_t excl[kBlk];
+  __shared__ int ci[kBlk], cj[kBlk], ck[kBlk];
+  __shared__ uint8_t ccase[kBlk];
+  for (int x = threadIdx.x; x < 256 * 24; x += kBlk) tab[x] = table[(x / 24) * 25 + x % 24];
+  cnt[threadIdx.x] = ntri[threadIdx.x];
+  if (threadIdx.x < 12) {
+    const int e = threadIdx.x;
+    corner[e] = make_int4(ec.dx[e], ec.dy[e], ec.dz[e], ec.axis[e]);
+  }
+  __syncthreads();
   const long long nc = (long long)g.n * g.n * g.n, nblk = (nc + kBlk - 1) / kBlk;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t* ex = excl + w * 64;
   for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const long long t = blk * kBlk + threadIdx.x;
     int i = 0, j = 0, k = 0, cs = 0;
@@ -568,8 +591,7 @@ __global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const uint8_t* __rest
     if (t < nc) {
       mcCellDecode(g, t, &i, &j, &k);
       cs = mcCase(g, sgn, i, j, k);
-      const int8_t* tr = table + cs * 25;
-      while (tr[3 * n] >= 0) ++n;
+      n = cnt[cs];
     }
     uint32_t incl = n;
     for (int o = 1; o < 64; o <<= 1) {
@@ -577,19 +599,29 @@ __global__ void __launch_bounds__(kBlk) kMcCells(McGrid g, const uint8_t* __rest
       if (lane >= o) incl += x;
     }
     if (lane == 63) wsum[w] = incl;
+    excl[threadIdx.x] = incl - n;
+    ci[threadIdx.x] = i;
+    cj[threadIdx.x] = j;
+    ck[threadIdx.x] = k;
+    ccase[threadIdx.x] = (uint8_t)cs;
     __syncthreads();
-    uint32_t base = coff[blk] + incl - n;
+    uint32_t base = coff[blk];
     for (int q = 0; q < w; ++q) base += wsum[q];
-    const int8_t* tr = table + cs * 25;
-    for (uint32_t q = 0; q < n; ++q)
-      for (int r = 0; r < 3; ++r) {
-        const int e = tr[3 * q + r];
-        const int ax = ec.axis[e];
-        const long long idx =
-            ax * g.E + mcEdgeIndex(g, ax, i + ec.dx[e], j + ec.dy[e], k + ec.dz[e]);
-        out[((size_t)base + q) * 3 + r] = mcEdgeId(bits, eoff, idx);
-      }
-    __syncthreads();  // wsum is rewritten by the next block of cells
+    const uint32_t nv = 3u * (uint32_t)__shfl(incl, 63, 64);
+    for (uint32_t vtx = lane; vtx < nv; vtx += 64) {
+      const uint32_t tri = vtx / 3u, r = vtx - 3u * tri;
+      // last lane whose scan start is <= tri: it owns triangle tri (n > 0)
+      int L = 0;
+      for (int step = 32; step; step >>= 1)
+        if (ex[L + step] <= tri) L += step;
+      const int c = w * 64 + L;
+      const int e = tab[ccase[c] * 24 + 3 * (int)(tri - ex[L]) + (int)r];
+      const int4 cr = corner[e];
+      const long long idx =
+          cr.w * g.E + mcEdgeIndex(g, cr.w, ci[c] + cr.x, cj[c] + cr.y, ck[c] + cr.z);
+      out[(size_t)base * 3 + vtx] = mcEdgeId(bits, eoff, idx);
+    }
+    __syncthreads();  // the LDS rows are rewritten by the next block of cells
   }
 }
 
@@ -742,7 +774,9 @@ struct McSource : MeshSource {
     const long long ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
     kMcVerts<<<gridOf(edgeBlocks(g) * (kBlk / 64), kBlk), kBlk, 0, s>>>(g, bits, eoff, p.rgb, v, rgb,
                                                                        nullptr);
-    kMcCells<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sgn, table.as<int8_t>(), ec, bits, eoff, coff, f);
+    kMcCells<<<gridOf(nc, kBlk, kMcCellBlocks), kBlk, 0, s>>>(g, sgn, table.as<int8_t>(),
+                                                             ntri.as<uint8_t>(), ec, bits, eoff,
+                                                             coff, f);
     SPRAY_HIP(hipGetLastError());
   }
 };
