@@ -468,28 +468,72 @@ __device__ __forceinline__ uint32_t mcEdgeId(const unsigned long long* __restric
   return id + (uint32_t)__popcll(bits[blk * (kBlk / 64) + w] & ((1ull << b) - 1ull));
 }
 
-// Vertices of the cut edges, one thread per 64-bit word of the cut mask
-// (its cuts in edge order), so a wave evaluates the field only as often as
-// its busiest word has cuts.  With v == nullptr only the per-block bounds of
-// the vertices are produced (part, one slot per block).
+// LDS written by some lanes and read by others of the same wave
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ int mcNthBit(unsigned long long m, uint32_t k) {
+  int pos = 0;
+  for (int wd = 32; wd; wd >>= 1) {
+    const unsigned long long low = m & ((1ull << wd) - 1ull);
+    const uint32_t c = (uint32_t)__popcll(low);
+    if (k >= c) {
+      k -= c;
+      m >>= wd;
+      pos += wd;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+// Vertices of the cut edges.  Each wave takes 64 words of the cut mask
+// (16 blocks of 256 edges), scans their cut counts, and gives each lane one
+// cut (its word found by a binary search of the scan, its bit by a
+// popcount search), so the field is evaluated for 64 cuts at a time and the
+// vertex stores are contiguous.  With v == nullptr only the per-block
+// bounds of the vertices are produced (part, one slot per block).
 __global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g,
                                                  const unsigned long long* __restrict__ bits,
                                                  const uint32_t* __restrict__ eoff, uint32_t color,
                                                  float* __restrict__ v, uint32_t* __restrict__ rgb,
                                                  float* part) {
+  static_assert(kBlk % 256 == 0, "a wave's 64 words are whole blocks of edges");
+  __shared__ uint32_t scan[kBlk];
+  __shared__ unsigned long long word[kBlk];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   const long long nwords = ((3 * g.E + kBlk - 1) / kBlk) * (kBlk / 64);
-  for (long long wi = blockIdx.x * (long long)blockDim.x + threadIdx.x; wi < nwords;
-       wi += (long long)gridDim.x * blockDim.x) {
-    unsigned long long m = bits[wi];
-    if (m == 0ull) continue;
-    const long long blk = wi / (kBlk / 64);
-    uint32_t id = eoff[blk];
-    for (long long q = blk * (kBlk / 64); q < wi; ++q) id += (uint32_t)__popcll(bits[q]);
-    while (m) {
-      const int b = __ffsll((long long)m) - 1;
-      m &= m - 1ull;
-      const long long e = wi * 64 + b;
+  const long long ngroups = (nwords + 63) / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t* ex = scan + w * 64;
+  unsigned long long* wd = word + w * 64;
+  for (long long gi = (long long)blockIdx.x * (kBlk / 64) + w; gi < ngroups;
+       gi += (long long)gridDim.x * (kBlk / 64)) {
+    const long long wi = gi * 64 + lane;
+    const unsigned long long m = wi < nwords ? bits[wi] : 0ull;
+    const uint32_t c = (uint32_t)__popcll(m);
+    uint32_t incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t x = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += x;
+    }
+    const uint32_t total = (uint32_t)__shfl(incl, 63, 64);
+    if (total == 0) continue;  // uniform over the wave
+    waveSync();  // the previous group's reads of ex and wd are done
+    ex[lane] = incl - c;
+    wd[lane] = m;
+    waveSync();
+    const uint32_t base = eoff[gi * 64 / (kBlk / 64)];  // first block of the group
+    for (uint32_t r = lane; r < total; r += 64) {
+      int L = 0;  // last word whose scan start is <= r: it holds cut r
+      for (int step = 32; step; step >>= 1)
+        if (ex[L + step] <= r) L += step;
+      const int b = mcNthBit(wd[L], r - ex[L]);
+      const long long e = (gi * 64 + L) * 64 + b;
       int axis, i, j, k;
       mcEdgeDecode(g, e, &axis, &i, &j, &k);
       const double f0 = mcFieldAt(g, i, j, k);
@@ -497,15 +541,15 @@ __global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g,
       float p[3];
       mcVertex(g, axis, i, j, k, f0, f1, p);
       if (v) {
-        for (int c = 0; c < 3; ++c) v[(size_t)id * 3 + c] = p[c];
+        const uint32_t id = base + r;
+        for (int q = 0; q < 3; ++q) v[(size_t)id * 3 + q] = p[q];
         rgb[id] = color;
       } else {
-        for (int c = 0; c < 3; ++c) {
-          lo[c] = fminf(lo[c], p[c]);
-          hi[c] = fmaxf(hi[c], p[c]);
+        for (int q = 0; q < 3; ++q) {
+          lo[q] = fminf(lo[q], p[q]);
+          hi[q] = fmaxf(hi[q], p[q]);
         }
       }
-      ++id;
     }
   }
   if (part) blockBounds(lo, hi, part);
