@@ -361,9 +361,28 @@ struct PlySource : MeshSource {
 
 // ---------------------------------------------------------------------------
 // Marching cubes (scenegen.cc's mcGenerate on the GPU)
+// x / d for 32-bit x as (umulhi(x, m) + x) >> s, s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1 (round-up multiplier, Granlund and
+// Montgomery 1994): point, edge and cell numbers are split into (i, j, k)
+// without integer division.
+struct McDiv {
+  uint32_t m, s;
+};
+
+McDiv mcDiv(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  return McDiv{(uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1), s};
+}
+
+__device__ __forceinline__ uint32_t mcDivide(uint32_t x, const McDiv& v) {
+  return (uint32_t)(((unsigned long long)__umulhi(x, v.m) + x) >> v.s);
+}
+
 struct McGrid {
   int n, N;              // cells and grid points per axis
   long long E;           // grid edges per axis: n * N * N
+  McDiv dn, dN;          // division by n and by N
   double ox, oy, oz, h;  // domain origin and cell size
   double a, b, phase;
 };
@@ -381,14 +400,15 @@ __device__ __forceinline__ long long mcEdgeIndex(const McGrid& g, int axis, int 
 
 __device__ __forceinline__ void mcEdgeDecode(const McGrid& g, long long e, int* axis, int* i,
                                              int* j, int* k) {
-  const int ax = (int)(e / g.E);
-  long long r = e - ax * g.E;
-  const int ni = ax == 0 ? g.n : g.N, nj = ax == 1 ? g.n : g.N;
+  const int ax = e < g.E ? 0 : (e < 2 * g.E ? 1 : 2);
+  const uint32_t r = (uint32_t)(e - ax * g.E);
+  const uint32_t ni = ax == 0 ? g.n : g.N, nj = ax == 1 ? g.n : g.N;
+  const uint32_t q = mcDivide(r, ax == 0 ? g.dn : g.dN);
+  const uint32_t kk = mcDivide(q, ax == 1 ? g.dn : g.dN);
   *axis = ax;
-  *i = (int)(r % ni);
-  r /= ni;
-  *j = (int)(r % nj);
-  *k = (int)(r / nj);
+  *i = (int)(r - q * ni);
+  *j = (int)(q - kk * nj);
+  *k = (int)kk;
 }
 
 // Field value at grid point (i, j, k), in the CPU generator's operations
@@ -406,7 +426,8 @@ __global__ void __launch_bounds__(kBlk) kMcSign(McGrid g, uint8_t* __restrict__ 
   const long long np = (long long)g.N * g.N * g.N;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < np;
        t += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % g.N), j = (int)((t / g.N) % g.N), k = (int)(t / ((long long)g.N * g.N));
+    const uint32_t q = mcDivide((uint32_t)t, g.dN), k = mcDivide(q, g.dN);
+    const int i = (int)((uint32_t)t - q * g.N), j = (int)(q - k * g.N);
     sgn[t] = mcFieldAt(g, i, j, k) < 0.0 ? 1 : 0;
   }
 }
@@ -562,9 +583,10 @@ __device__ __forceinline__ int mcCase(const McGrid& g, const uint8_t* sgn, int i
 }
 
 __device__ __forceinline__ void mcCellDecode(const McGrid& g, long long t, int* i, int* j, int* k) {
-  *i = (int)(t % g.n);
-  *j = (int)((t / g.n) % g.n);
-  *k = (int)(t / ((long long)g.n * g.n));
+  const uint32_t q = mcDivide((uint32_t)t, g.dn), kk = mcDivide(q, g.dn);
+  *i = (int)((uint32_t)t - q * g.n);
+  *j = (int)(q - kk * g.n);
+  *k = (int)kk;
 }
 
 // Triangles per block of 256 cells (bcnt[nblk] = 0).
@@ -739,6 +761,8 @@ struct McSource : MeshSource {
       g.n = n;
       g.N = n + 1;
       g.E = (long long)n * g.N * g.N;
+      g.dn = mcDiv((uint32_t)n);
+      g.dN = mcDiv((uint32_t)g.N);
       g.ox = (double)(d % p.nx);
       g.oy = (double)((d / p.nx) % p.ny);
       g.oz = (double)(d / (p.nx * p.ny));
