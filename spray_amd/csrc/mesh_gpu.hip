@@ -411,24 +411,44 @@ __device__ __forceinline__ void mcEdgeDecode(const McGrid& g, long long e, int* 
   *k = (int)kk;
 }
 
+// Kernels that evaluate the field take this many bytes of dynamic LDS for
+// its separable factors.
+size_t mcTabBytes(const McGrid& g) { return (size_t)3 * g.N * sizeof(double); }
+
+// The field's separable factors sin(a x + phase), sin(a y) and sin(a z) at
+// every grid coordinate of the domain, into LDS (3 N doubles), the
+// coordinates formed as in mcFieldAt.
+__device__ void mcAxisTables(const McGrid& g, double* tab) {
+  for (int q = threadIdx.x; q < 3 * g.N; q += blockDim.x) {
+    const int ax = q / g.N, i = q - ax * g.N;
+    const double c = (ax == 0 ? g.ox : (ax == 1 ? g.oy : g.oz)) + (double)i * g.h;
+    tab[q] = ax == 0 ? sin(g.a * c + g.phase) : sin(g.a * c);
+  }
+  __syncthreads();
+}
+
 // Field value at grid point (i, j, k), in the CPU generator's operations
-// and order (the file is built without contraction).
-__device__ __forceinline__ double mcFieldAt(const McGrid& g, int i, int j, int k) {
+// and order (the file is built without contraction):
+// sin(a x + phase) sin(a y) sin(a z) + 0.3 sin(b (x + y + z) + phase), the
+// first three factors read from the axis tables.
+__device__ __forceinline__ double mcFieldAt(const McGrid& g, const double* tab, int i, int j,
+                                            int k) {
   const double x = g.ox + (double)i * g.h, y = g.oy + (double)j * g.h, z = g.oz + (double)k * g.h;
-  return sin(g.a * x + g.phase) * sin(g.a * y) * sin(g.a * z) +
-         0.3 * sin(g.b * (x + y + z) + g.phase);
+  return tab[i] * tab[g.N + j] * tab[2 * g.N + k] + 0.3 * sin(g.b * (x + y + z) + g.phase);
 }
 
 // Field signs as bytes (1: inside, f < 0): the cut and case tests read one
 // byte per grid point; field values are evaluated again only at the ends
 // of cut edges.
 __global__ void __launch_bounds__(kBlk) kMcSign(McGrid g, uint8_t* __restrict__ sgn) {
+  extern __shared__ double tab[];
+  mcAxisTables(g, tab);
   const long long np = (long long)g.N * g.N * g.N;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < np;
        t += (long long)gridDim.x * blockDim.x) {
     const uint32_t q = mcDivide((uint32_t)t, g.dN), k = mcDivide(q, g.dN);
     const int i = (int)((uint32_t)t - q * g.N), j = (int)(q - k * g.N);
-    sgn[t] = mcFieldAt(g, i, j, k) < 0.0 ? 1 : 0;
+    sgn[t] = mcFieldAt(g, tab, i, j, k) < 0.0 ? 1 : 0;
   }
 }
 
@@ -526,6 +546,8 @@ __global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g,
   static_assert(kBlk % 256 == 0, "a wave's 64 words are whole blocks of edges");
   __shared__ uint32_t scan[kBlk];
   __shared__ unsigned long long word[kBlk];
+  extern __shared__ double tab[];
+  mcAxisTables(g, tab);
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   const long long nwords = ((3 * g.E + kBlk - 1) / kBlk) * (kBlk / 64);
   const long long ngroups = (nwords + 63) / 64;
@@ -557,8 +579,8 @@ __global__ void __launch_bounds__(kBlk) kMcVerts(McGrid g,
       const long long e = (gi * 64 + L) * 64 + b;
       int axis, i, j, k;
       mcEdgeDecode(g, e, &axis, &i, &j, &k);
-      const double f0 = mcFieldAt(g, i, j, k);
-      const double f1 = mcFieldAt(g, i + (axis == 0), j + (axis == 1), k + (axis == 2));
+      const double f0 = mcFieldAt(g, tab, i, j, k);
+      const double f1 = mcFieldAt(g, tab, i + (axis == 0), j + (axis == 1), k + (axis == 2));
       float p[3];
       mcVertex(g, axis, i, j, k, f0, f1, p);
       if (v) {
@@ -616,8 +638,8 @@ __global__ void __launch_bounds__(kBlk) kMcCellCount(McGrid g, const uint8_t* __
   if (blockIdx.x == 0 && threadIdx.x == 0) bcnt[nblk] = 0u;
 }
 
-// Grid cap of kMcCells: eight blocks per CU, each stages the case tables
-// into LDS once.
+// Grid cap of kMcSign and kMcCells: eight blocks per CU, each stages its
+// tables into LDS once.
 constexpr long long kMcCellBlocks = 2048;
 
 // Triangles of every cell, in cell order: the block's offset plus an
@@ -735,12 +757,13 @@ struct McSource : MeshSource {
     uint32_t* coff = (uint32_t*)(store.as<char>() + at.coff);
     const long long np = (long long)g.N * g.N * g.N, ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
     const long long eb = edgeBlocks(g), cb = cellBlocks(g);
-    kMcSign<<<gridOf(np, kBlk), kBlk, 0, s>>>(g, sgn);
+    kMcSign<<<gridOf(np, kBlk, kMcCellBlocks), kBlk, mcTabBytes(g), s>>>(g, sgn);
     kMcEdgeBits<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, sgn, bits, eoff);
     exclusiveScan(eoff, eoff, eb + 1, s);
     if (bnd) {
       const int vgrid = gridOf(eb * (kBlk / 64), kBlk, kBoundBlocks);
-      kMcVerts<<<vgrid, kBlk, 0, s>>>(g, bits, eoff, 0u, nullptr, nullptr, part.as<float>());
+      kMcVerts<<<vgrid, kBlk, mcTabBytes(g), s>>>(g, bits, eoff, 0u, nullptr, nullptr,
+                                                 part.as<float>());
       kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), vgrid, bnd);
     }
     kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sgn, ntri.as<uint8_t>(), coff);
@@ -840,8 +863,8 @@ struct McSource : MeshSource {
     const uint32_t* eoff = (const uint32_t*)(store.as<char>() + at.eoff);
     const uint32_t* coff = (const uint32_t*)(store.as<char>() + at.coff);
     const long long ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
-    kMcVerts<<<gridOf(edgeBlocks(g) * (kBlk / 64), kBlk), kBlk, 0, s>>>(g, bits, eoff, p.rgb, v, rgb,
-                                                                       nullptr);
+    kMcVerts<<<gridOf(edgeBlocks(g) * (kBlk / 64), kBlk), kBlk, mcTabBytes(g), s>>>(
+        g, bits, eoff, p.rgb, v, rgb, nullptr);
     kMcCells<<<gridOf(nc, kBlk, kMcCellBlocks), kBlk, 0, s>>>(g, sgn, table.as<int8_t>(),
                                                              ntri.as<uint8_t>(), ec, bits, eoff,
                                                              coff, f);
