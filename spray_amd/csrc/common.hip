@@ -1,7 +1,9 @@
 // Device-wide exclusive scan of uint32 counts: per-block scans of 2048
 // values, a single-block scan of the block sums (chunked, any length), and
-// a final add.  Used for vertex adjacency offsets, radix-sort digit
-// offsets and per-domain ray queue offsets.
+// a final add (up to 1024 tiles, each block of the add sums the tile sums
+// before it instead).  Used for vertex adjacency offsets, radix-sort digit
+// offsets, PLOC merge positions and the marching-cubes edge and cell block
+// offsets.
 #include <algorithm>
 
 #include "ctx.h"
@@ -72,6 +74,26 @@ __global__ void kScanSums(uint32_t* sums, long long m) {
   }
 }
 
+// Up to this many tiles, kScanAddSum adds the prefix of the tile sums that
+// each block forms itself, and the separate tile-sum scan is skipped.
+constexpr long long kScanFuseTiles = 1024;
+
+__global__ void kScanAddSum(uint32_t* out, long long n, const uint32_t* sums) {
+  __shared__ uint32_t pre;
+  if (threadIdx.x < 64) {
+    uint32_t x = 0;
+    for (long long q = threadIdx.x; q < blockIdx.x; q += 64) x += sums[q];
+    for (int o = 32; o; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (threadIdx.x == 0) pre = x;
+  }
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kScanTile;
+  for (int j = threadIdx.x; j < kScanTile; j += blockDim.x) {
+    const long long i = base + j;
+    if (i < n) out[i] += pre;
+  }
+}
+
 __global__ void kScanAdd(uint32_t* out, long long n, const uint32_t* sums) {
   long long base = (long long)blockIdx.x * kScanTile;
   uint32_t add = sums[blockIdx.x];
@@ -93,7 +115,9 @@ void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s
   sums.b.reserve(std::max<size_t>((size_t)nb * 4 + 64, (size_t)4 << 20));
   uint32_t* tile_sums = sums.b.as<uint32_t>();
   kScanTiles<<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, tile_sums);
-  if (nb > 1) {
+  if (nb > 1 && nb <= kScanFuseTiles) {
+    kScanAddSum<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, tile_sums);
+  } else if (nb > 1) {
     kScanSums<<<1, 1024, 0, s>>>(tile_sums, nb);
     kScanAdd<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, tile_sums);
   }
