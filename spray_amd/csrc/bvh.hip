@@ -24,18 +24,19 @@ namespace spray {
 // three vertices in face order.  A vertex-parallel gather over the faces of
 // each vertex, visited in ascending face order, reproduces that float
 // summation order exactly.
+// Face corners are numbered in 32 bits: a domain has fewer than 2^24 faces.
 __global__ void kVertCount(const uint32_t* __restrict__ face, long long nf, uint32_t* cnt) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nf * 3;
-       i += (long long)gridDim.x * blockDim.x)
+  const uint32_t nc = (uint32_t)nf * 3u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
     atomicAdd(&cnt[face[i]], 1u);
 }
 __global__ void kVertFill(const uint32_t* __restrict__ face, long long nf,
                           const uint32_t* __restrict__ off, uint32_t* cursor, uint32_t* adj) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nf * 3;
-       i += (long long)gridDim.x * blockDim.x) {
-    uint32_t v = face[i];
-    uint32_t slot = atomicAdd(&cursor[v], 1u);
-    adj[off[v] + slot] = (uint32_t)(i / 3);
+  const uint32_t nc = (uint32_t)nf * 3u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    const uint32_t v = face[i];
+    const uint32_t slot = atomicAdd(&cursor[v], 1u);
+    adj[off[v] + slot] = i / 3u;
   }
 }
 __global__ void kVertNormal(const float* __restrict__ vtx, const uint32_t* __restrict__ face,
@@ -908,10 +909,9 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     const uint32_t* face = P.face.as<uint32_t>() + D.face_off * 3;
     // normals
     uint32_t* cnt = S.cnts.as<uint32_t>();
-    uint32_t* off = cnt + (nv + 1);
-    uint32_t* cur = off + (nv + 1);
-    SPRAY_HIP(hipMemsetAsync(cnt, 0, (size_t)(nv + 1) * 4, s));
-    SPRAY_HIP(hipMemsetAsync(cur, 0, (size_t)(nv + 1) * 4, s));
+    uint32_t* cur = cnt + (nv + 1);  // cleared with cnt in one fill
+    uint32_t* off = cur + (nv + 1);
+    SPRAY_HIP(hipMemsetAsync(cnt, 0, (size_t)(nv + 1) * 8, s));
     kVertCount<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, cnt);
     exclusiveScan(cnt, off, nv + 1, s);
     kVertFill<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, off, cur,
