@@ -386,6 +386,31 @@ __global__ void kRangeLevel(const float* __restrict__ src, float* __restrict__ d
     }
 }
 
+// Levels j and j + 1 of the sparse table in one launch, from level j - 1
+// (h = 2^(j-1) blocks per entry): level j entry i = union(src[i], src[i + h])
+// for i < mj, and level j + 1 entry i = union(level j [i], level j [i + 2h])
+// for i < mj2, each level-j box formed as the one-level kernel forms it.
+__global__ void kRangeLevel2(const float* __restrict__ src, float* __restrict__ dst,
+                             float* __restrict__ dst2, long long mj, long long mj2, long long h) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < mj;
+       i += (long long)gridDim.x * blockDim.x) {
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(src[i * 6 + k], src[(i + h) * 6 + k]);
+      hi[k] = fmaxf(src[i * 6 + 3 + k], src[(i + h) * 6 + 3 + k]);
+      dst[i * 6 + k] = lo[k];
+      dst[i * 6 + 3 + k] = hi[k];
+    }
+    if (i < mj2) {
+      const long long a = i + 2 * h, b = i + 3 * h;
+      for (int k = 0; k < 3; ++k) {
+        dst2[i * 6 + k] = fminf(lo[k], fminf(src[a * 6 + k], src[b * 6 + k]));
+        dst2[i * 6 + 3 + k] = fmaxf(hi[k], fmaxf(src[a * 6 + 3 + k], src[b * 6 + 3 + k]));
+      }
+    }
+  }
+}
+
 __global__ void kRangeBoxes(int ninner, const int* __restrict__ first, const int* __restrict__ last,
                             const float* __restrict__ leafbox, const float* __restrict__ pre,
                             const float* __restrict__ suf, const float* __restrict__ table,
@@ -1078,10 +1103,15 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       float* tab = S.rtab.as<float>();
       kLeafScan<<<gridFor(nblk * 64, 256), 256, 0, s>>>(leafbox, nf, S.rpre.as<float>(),
                                                         S.rsuf.as<float>(), tab);
-      for (int j = 1; j < nlev; ++j) {
+      for (int j = 1; j < nlev; j += 2) {  // two levels per launch
         const long long m = nblk - (1ll << j) + 1;
-        kRangeLevel<<<gridFor(m, 256), 256, 0, s>>>(tab + RL.off[j - 1], tab + RL.off[j], m,
-                                                    1ll << (j - 1));
+        if (j + 1 < nlev)
+          kRangeLevel2<<<gridFor(m, 256), 256, 0, s>>>(tab + RL.off[j - 1], tab + RL.off[j],
+                                                       tab + RL.off[j + 1], m,
+                                                       nblk - (1ll << (j + 1)) + 1, 1ll << (j - 1));
+        else
+          kRangeLevel<<<gridFor(m, 256), 256, 0, s>>>(tab + RL.off[j - 1], tab + RL.off[j], m,
+                                                      1ll << (j - 1));
       }
       kRangeBoxes<<<gridFor(nf - 1, 256), 256, 0, s>>>(nf - 1, F, Lst, leafbox, S.rpre.as<float>(),
                                                        S.rsuf.as<float>(), tab, RL, inner);
