@@ -216,6 +216,9 @@ def test_binary_ply_errors(tmp_path, case, message):
 GEN = [
     dict(nx=2, ny=2, nz=2, res=24),
     dict(nx=3, ny=2, nz=1, res=0, per_domain_random=True, seed=11, tri_total=150000.0),
+    # one domain of 128^3 cells: the marching-cubes kernels' grid-stride loops
+    # (capped grids of 2048 blocks) run several passes
+    dict(nx=1, ny=1, nz=1, res=128),
 ]
 
 
@@ -246,7 +249,7 @@ def _same_scene(a, b):
         assert np.array_equal(va, vb), (d, int((va != vb).sum()))
 
 
-@pytest.mark.parametrize("gen", GEN, ids=["lattice", "log-uniform"])
+@pytest.mark.parametrize("gen", GEN, ids=["lattice", "log-uniform", "fine-domain"])
 def test_gpu_generator_matches_cpu_generator(tmp_path, gen):
     """Marching cubes on the GPU gives the CPU generator's meshes, vertex and
     face order, domain bounds and counts (then the same frame)."""
