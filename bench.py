@@ -1,17 +1,20 @@
 """Benchmark: SpRay per-rank tracing on one MI355X.
 
-Default workload (the second configuration of the benchmark set, "c2"): 64
-domains in a 4x4x4 grid of unit cubes, each a marching-cubes isosurface of
-f = sin(6x)sin(6y)sin(6z) + 0.3 sin(17(x+y+z)) on a 64^3 grid (~91k triangles
-per domain, ~5.8M total), path tracing with albedo 0.7 (vertex colour 179),
-one rectangular area light, 1024x1024, 8 spp, depth 5, Russian roulette from
-bounce 3, fp32 rays.  A step is one frame (8 blocking tiles of 1,048,576
-samples).  Synthetic geometry generated in-process.
+Default workload ("c3", the 256-domain path-tracing scene): 256 domains in an
+8x8x4 grid of unit cubes, each a marching-cubes isosurface of
+f = sin(a x)sin(a y)sin(a z) + 0.3 sin(b(x+y+z)) with per-domain a in [4, 8],
+b in [12, 20] and a log-uniform triangle budget in 10k..2M (seeded, rescaled
+to ~120M triangles in total), path tracing with albedo 0.7 (vertex colour
+179), one rectangular area light, 1920x1080, 16 spp, depth 6, Russian
+roulette from bounce 3, fp32 rays.  A step is one frame (33 blocking tiles of
+at most 1,048,576 samples).  Synthetic geometry generated in-process.  The
+learned schedule's predictor is trained on 50M (ray, domain, hit) samples
+logged by plain-schedule frames.
 
-Other workloads (--workload): "c1" AO on the same geometry (512x512, 1 spp,
-8 AO rays, max distance 0.5) and "c3" path tracing on 256 domains in an
-8x8x4 grid with log-uniform per-domain triangle budgets (10k..2M, rescaled to
-~120M in total), 1920x1080, 16 spp, depth 6; "c5" the in-situ stress case:
+Other workloads (--workload): "c2" the 64-domain scene (4x4x4 grid, 64^3
+cells per domain, a = 6, b = 17, ~5.8M triangles) path traced at 1024x1024,
+8 spp, depth 5; "c1" AO on that geometry (512x512, 1 spp, 8 AO rays, max
+distance 0.5); "c5" the in-situ stress case:
 1024 domains in a 16x8x8 grid, ~500k triangles each (160^3 cells), generated
 on the GPU and remeshed on the GPU before every timed frame with the field
 phase advancing 0.05 per frame, 3840x2160, 16 spp, depth 8; the learned
@@ -25,6 +28,7 @@ import argparse
 import json
 import math
 import os
+import shutil
 import statistics
 import sys
 import tempfile
@@ -36,34 +40,37 @@ sys.path.insert(0, ROOT)
 C2_CAM = dict(pos=[2.0, 2.0, 2.0 + 2.0 * math.sqrt(3.0)], lookat=[2.0, 2.0, 2.0], up=[0.0, 1.0, 0.0])
 RECT_LIGHT = "light rect 0.5 4.5 0.5 3 0 0 0 0 3 4 4 4"
 HBM_PEAK_GBS = 8000.0
+TRAIN_SAMPLES = 50_000_000  # (ray, domain, hit) samples the predictor trains on
 
 # name -> scene generator arguments, render settings, camera (None: the
-# reference's default framing of the scene bound), CPU tiles timed by default
+# reference's default framing of the scene bound), the blocking tiles
+# [begin, end) the CPU baseline times by default (C3: two tiles across the
+# middle of the image, about 2 x 1.01M samples)
 WORKLOADS = {
     "c1": dict(grid=(4, 4, 4), gen=dict(res=64, rgb=0xFFFFFF, extra_lines="light point 2 6 2 1 1 1"),
                render=dict(image_w=512, image_h=512, pixel_samples=1, ao_mode=1, ao_samples=8, bounces=1,
                            ao_max_dist=0.5),
-               cam=C2_CAM, cpu_tiles=1,
+               cam=C2_CAM, cpu_tiles=(0, 1),
                title="C1 ambient occlusion: 64 domains (4x4x4 MC isosurfaces, {tris} tris), 512x512, 1 spp, "
                      "8 AO rays, max dist 0.5"),
     "c2": dict(grid=(4, 4, 4), gen=dict(res=64, rgb=0xB3B3B3, extra_lines=RECT_LIGHT),
                render=dict(image_w=1024, image_h=1024, pixel_samples=8, ao_mode=0, ao_samples=1, bounces=5,
                            rr_start=3),
-               cam=C2_CAM, cpu_tiles=8,
+               cam=C2_CAM, cpu_tiles=(0, 8),
                title="C2 path tracing: 64 domains (4x4x4 MC isosurfaces, {tris} tris), {w}x{h}, {spp} spp, "
                      "depth 5, RR>=3, rect light"),
     "c3": dict(grid=(8, 8, 4), gen=dict(res=0, per_domain_random=True, seed=0xC3, tri_total=120e6, rgb=0xB3B3B3,
                                         extra_lines="light rect 1 8.5 0.5 6 0 0 0 0 3 4 4 4"),
                render=dict(image_w=1920, image_h=1080, pixel_samples=16, ao_mode=0, ao_samples=1, bounces=6,
                            rr_start=3),
-               cam=None, cpu_tiles=1,
+               cam=None, cpu_tiles=(16, 18),
                title="C3 path tracing: 256 domains (8x8x4 MC isosurfaces, log-uniform 10k-2M tris each, "
                      "{tris} tris), {w}x{h}, {spp} spp, depth 6, RR>=3, rect light"),
     "c5": dict(grid=(16, 8, 8), gen=dict(res=160, rgb=0xB3B3B3,
                                          extra_lines="light rect 2 1 8.5 0 6 0 12 0 0 4 4 4"),
                render=dict(image_w=3840, image_h=2160, pixel_samples=16, ao_mode=0, ao_samples=1, bounces=8,
                            rr_start=3),
-               cam=None, cpu_tiles=0, remesh=0.05,
+               cam=None, cpu_tiles=None, remesh=0.05,
                title="C5 in-situ: 1024 domains (16x8x8 MC isosurfaces remeshed on the GPU every frame, "
                      "phase +0.05 per frame, {tris} tris), {w}x{h}, {spp} spp, depth 8, RR>=3, rect light facing the "
                      "scene's front (+z) face"),
@@ -73,7 +80,7 @@ WORKLOADS = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=0, help="square image size (0: the workload's)")
@@ -81,7 +88,7 @@ def parse():
     ap.add_argument("--schedules", default="plain,batched,learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-tiles", type=int, default=0,
-                    help="blocking tiles timed on the CPU (0: the workload's default)")
+                    help="blocking tiles timed on the CPU, from the first (0: the workload's default range)")
     ap.add_argument("--cache-domains", type=int, default=-1,
                     help="domains kept in HBM (< 0: all); a partial cache pages domains (plain schedule)")
     ap.add_argument("--tiles-in-flight", type=int, default=0,
@@ -179,9 +186,14 @@ def main():
                 # train the predictor on hit logs of plain-schedule frames
                 # (in-situ: of a batched frame, the plain one taking minutes)
                 ctx.set_schedule(1 if remesh_step else 0)
-                ctx.predictor_log(True, 16_000_000)
-                ctx.trace_frame(want_image=False)
+                ctx.predictor_log(True, TRAIN_SAMPLES)
+                for _ in range(4):  # plain frames until the log holds its 50M samples
+                    ctx.trace_frame(want_image=False)
+                    if ctx.predictor_num_samples() >= TRAIN_SAMPLES:
+                        break
                 ctx.predictor_log(False)
+                print(f"# learned: training on {ctx.predictor_num_samples()} logged samples", file=sys.stderr,
+                      flush=True)
                 ctx.predictor_train(2, 4096, 2e-3)
                 ctx.set_schedule(sched)
             if name != "plain" and not args.no_autotune:
@@ -281,7 +293,8 @@ def main():
 
     cpu = None
     if not args.no_cpu and not remesh_step:
-        cpu = cpu_baseline(tmp, args.workload, kw, cam, args.cpu_tiles or wl["cpu_tiles"])
+        tiles = (0, args.cpu_tiles) if args.cpu_tiles else wl["cpu_tiles"]
+        cpu = cpu_baseline(tmp, args.workload, kw, cam, tiles)
 
     line = {
         "metric": "Mrays/s (AO + path) on 1 MI355X; learned vs plain schedule at equal image",
@@ -295,11 +308,22 @@ def main():
                    "schedule": best, "triangles": tris, "scene_gen_s": round(gen_s, 2),
                    "load_build_s": round(load_s, 2)},
         "schedules": results,
+        # the metric's comparison: Mrays/s of each schedule over the plain one (equal images above)
+        "learned_vs_plain": ratio(ok, "learned", "plain"),
+        "batched_vs_plain": ratio(ok, "batched", "plain"),
+        "learned_vs_batched": ratio(ok, "learned", "batched"),
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
     ctx.close()
+    shutil.rmtree(tmp, ignore_errors=True)
+
+
+def ratio(ok, a, b):
+    if a in ok and b in ok and ok[b]["mrays_s"] > 0:
+        return round(ok[a]["mrays_s"] / ok[b]["mrays_s"], 4)
+    return None
 
 
 def measured_traffic(workload, schedule):
@@ -341,20 +365,36 @@ def default_framing(ctx):
     return dict(pos=pos, lookat=[float(x) for x in look], up=[0.0, 1.0, 0.0])
 
 
-def cpu_baseline(scene_dir, name, kw, cam, ntiles):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene_dir, name, kw, cam, tiles):
     """The oracle (the CPU restatement of the reference's OpenMP tracer) on
-    the first `ntiles` blocking tiles of the same frame, on the host cores."""
+    blocking tiles [tiles[0], tiles[1]) of the same frame, on the host cores
+    this process may run on (at most 16: the GPU box's share)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
-    threads = min(16, os.cpu_count() or 1)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.time()
     ref = orc.Scene(os.path.join(scene_dir, name + ".spray"), scene_dir, nthreads=threads)
+    load_s = time.time() - t0
+    b, e = tiles
     _, st = ref.render(nthreads=threads, cam_pos=cam["pos"], cam_lookat=cam["lookat"],
-                       cam_up=cam["up"], tile_begin=0, tile_end=ntiles, **kw)
+                       cam_up=cam["up"], tile_begin=b, tile_end=e, **kw)
     return {"value": round(st["useful_rays"] / st["seconds"] / 1e6, 4), "unit": "Mrays/s",
-            "cores": threads, "kind": "port",
-            "sample": f"first {ntiles} of {orc.num_tiles(**kw)} blocking tiles of the same frame "
-                      f"({st['useful_rays']} useful rays, {st['seconds']:.1f} s, speculative "
-                      f"OpenMP restatement with its own SAH BVH)"}
+            "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"blocking tiles {b}..{e - 1} of {orc.num_tiles(**kw)} of the same frame "
+                      f"({st['useful_rays']} useful rays in {st['seconds']:.1f} s; scene load and SAH "
+                      f"BVH build {load_s:.1f} s, not timed), speculative OpenMP restatement of the "
+                      f"reference tracer with its own SAH BVH"}
 
 
 if __name__ == "__main__":

@@ -85,6 +85,7 @@ typedef struct spray_stats {
   unsigned long long cache_hits;      /* domain loads served by the HBM cache (paged pool) */
   unsigned long long cache_misses;    /* domain loads copied from host memory */
   unsigned long long cache_bytes;     /* bytes copied host -> HBM by those loads */
+  unsigned long long regrows;         /* frame re-renders after a ray pool or queue overflowed */
 } spray_stats;
 
 typedef struct spray_rays_soa { /* n entries each; host memory */
@@ -156,8 +157,11 @@ int spray_set_schedule(spray_ctx* ctx, int schedule);
 /* Batched form: autotunes the queue chunk (how many lanes of a wave must be
  * idle before the wave fetches that many queue entries with one atomic,
  * 1..64) and the wavefront tile (waves per workgroup of the persistent trace
- * kernel, 1..4) on the loaded scene and current camera; the choice is kept
- * in the context. */
+ * kernel, 1..4) on a tuning frame (the loaded scene and current camera at
+ * half the samples per pixel); the choice is kept in the context and in a
+ * cache file (SPRAY_AUTOTUNE_CACHE, else ~/.cache/spray_amd/autotune.txt)
+ * keyed by the device, the frame settings and the scene, so a later call
+ * with the same key tunes nothing. */
 int spray_autotune(spray_ctx* ctx, int* chunk_out, int* tile_out);
 int spray_set_batch_params(spray_ctx* ctx, int chunk, int tile);
 
@@ -184,6 +188,14 @@ int spray_set_bvh_builder(spray_ctx* ctx, int builder);
  * schedule needs the whole scene resident. */
 int spray_set_cache_size(spray_ctx* ctx, int domains);
 
+/* Ray-pool headroom (new; test hook): the factor on the tracer's initial
+ * estimate of its ray pools and queues (default 1).  A frame that overflows
+ * them is rendered again with doubled capacities, up to 8 times the
+ * estimate, and spray_stats.regrows counts the re-renders; a smaller factor
+ * exercises that path.  The reference's per-thread arenas and std::queues
+ * grow without bound instead (ooc/ooc_tcontext.h:74-115). */
+int spray_set_pool_headroom(spray_ctx* ctx, double factor);
+
 /* Domain cache state (render/lru_cache.h:50-57, getCacheSize and
  * getBlocks): *capacity = domains the cache holds; lru_domains (capacity
  * entries) = the resident domains from least to most recently used, -1
@@ -200,6 +212,18 @@ int spray_cache_state(spray_ctx* ctx, int* capacity, int* lru_domains,
  * count and summed device time since the last reset. */
 int spray_profile_kernels(spray_ctx* ctx, int enable);
 int spray_kernel_time(spray_ctx* ctx, long long* launches, double* total_ms, int reset);
+
+/* Schedule test hook (DomainStats::schedule, ooc/ooc_domain_stats.cc:60-111,
+ * as PContext::isectPrims uses it, ooc/ooc_pcontext.h:127-131): with
+ * max_records > 0 the plain schedule records its first max_records passes
+ * from the next frame on (0 stops; each call clears the record).
+ * spray_schedule_read returns the record count and, per record i,
+ * meta[4i..4i+3] = {blocking tile, window, pass in the window, queue entries},
+ * scores[i*ndomains + d] = sum over the pass's entries of domain d of
+ * max(0, 16 - position in the ray's domain list), and order[i*ndomains ..]
+ * = the domains by descending score, ties by id.  Arrays may be NULL. */
+int spray_schedule_log(spray_ctx* ctx, int max_records);
+int spray_schedule_read(spray_ctx* ctx, int* nrec, int* meta, unsigned long long* scores, int* order);
 
 /* Learned form (new): domain-hit predictor.  Logging collects
  * (origin, direction, domain, hit) samples from plain-schedule frames;
@@ -227,6 +251,14 @@ int spray_mlp_logits(spray_ctx* ctx, const float* samples, int n, const float* p
  * (dir/name.spray); extra_lines (e.g. light lines) go to the header. */
 int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const char* name,
                          const char* extra_lines, long long* triangles_out);
+
+/* One domain of that scene as the CPU generator builds it (host only): the
+ * mesh spray_scenegen_write writes to name_dNNNN.ply, in the same vertex,
+ * face and colour order.  With v, f and rgb NULL only the counts are
+ * returned; otherwise the arrays hold nverts*3, nfaces*3 and nverts
+ * entries. */
+int spray_scenegen_domain(const spray_scenegen_params* p, int domain, float* v, unsigned int* f,
+                          unsigned int* rgb, long long* nverts, long long* nfaces);
 
 /* ---- on-disk formats and remeshing on the GPU (new) ----
  * spray_load_scene decodes binary little-endian PLY bodies on the GPU (the

@@ -258,6 +258,25 @@ int orc_render(void* h, const TraceConfig* cfg, float* rgba, TraceStats* st) {
     return setErr(e);
   }
 }
+// Renders like orc_render and returns the DomainStats schedule of the first
+// max_records passes (ooc_domain_stats.cc:60-111): meta[4] = {tile, window,
+// pass in window, entries queued}, scores[ndomains], order[ndomains].
+int orc_render_schedule(void* h, const TraceConfig* cfg, float* rgba, TraceStats* st, int max_records,
+                        int* nrec, int* meta, long long* scores, int* order) {
+  try {
+    SchedLog L;
+    L.max_records = max_records;
+    trace(*(Scene*)h, *cfg, rgba, st, &L);
+    const int n = L.records(), nd = L.ndomains;
+    *nrec = n;
+    std::copy(L.meta.begin(), L.meta.end(), meta);
+    for (size_t i = 0; i < (size_t)n * nd; ++i) scores[i] = (long long)L.scores[i];
+    std::copy(L.order.begin(), L.order.end(), order);
+    return 0;
+  } catch (const std::exception& e) {
+    return setErr(e);
+  }
+}
 int orc_sizeof_config() { return (int)sizeof(TraceConfig); }
 int orc_sizeof_stats() { return (int)sizeof(TraceStats); }
 

@@ -71,6 +71,7 @@ struct Ctx {
   int nlights;     // AO: ao_samples; PT: scene lights (ooc_tracer.inl:54-60)
   int tile_w = 0;  // largest blocking tile
   int tile_h = 0;
+  SchedLog* log = nullptr;
 };
 
 struct Isect {
@@ -457,14 +458,20 @@ class DomainStats {
   void add(int id, const DomainStats& o) {
     for (int d = 0; d < kSlots; ++d) stats_[(size_t)id * kSlots + d] += o.stats_[(size_t)id * kSlots + d];
   }
+  int64_t total() const {
+    int64_t s = 0;
+    for (int64_t v : stats_) s += v;
+    return s;
+  }
   // evalScores + sort descending (ties by id: deviation Q12) + traversal order
-  void schedule(std::vector<int>* order) const {
+  void schedule(std::vector<int>* order, std::vector<int64_t>* scores = nullptr) const {
     std::vector<int64_t> score(n_, 0);
     for (int id = 0; id < n_; ++id) {
       int64_t sc = 0;
       for (int d = 0; d < kSlots; ++d) sc += stats_[(size_t)id * kSlots + d] * std::max(0, kDomainListSize - d);
       score[id] = sc;
     }
+    if (scores) *scores = score;
     order->resize(n_);
     std::iota(order->begin(), order->end(), 0);
     std::stable_sort(order->begin(), order->end(),
@@ -715,7 +722,7 @@ struct TContext {
 
 // ooc/ooc_pcontext.h:91-181 (isectPrims), run by every thread of the team
 static void isectPrims(const Ctx& C, std::vector<TContext>& tcs, DomainStats& merged,
-                       std::vector<int>& order, std::vector<int>& votes, int tid) {
+                       std::vector<int>& order, std::vector<int>& votes, int tid, int tile) {
   TContext& tc = tcs[tid];
   const int nt = (int)tcs.size();
   auto vote = [&](int slot, bool v) {
@@ -726,9 +733,11 @@ static void isectPrims(const Ctx& C, std::vector<TContext>& tcs, DomainStats& me
     return s;
   };
   int ray_depth = 0;
+  int window = 0;
   while (true) {
     tc.procPendingQ();
     tc.vbuf.reset();
+    int pass = 0;
     while (true) {
       bool sqs_in_empty = tc.anyEmpty(*tc.sqs_in);
       bool rqs_empty = tc.anyEmpty(tc.rqs);
@@ -742,7 +751,18 @@ static void isectPrims(const Ctx& C, std::vector<TContext>& tcs, DomainStats& me
         for (int id = 0; id < C.ndomains; ++id) merged.add(id, tc.rstats);
 #pragma omp barrier
 #pragma omp single
-        merged.schedule(&order);
+        {
+          SchedLog* L = C.log;
+          if (L && L->records() < L->max_records) {
+            std::vector<int64_t> sc;
+            merged.schedule(&order, &sc);
+            L->meta.insert(L->meta.end(), {tile, window, pass, (int)merged.total()});
+            L->scores.insert(L->scores.end(), sc.begin(), sc.end());
+            L->order.insert(L->order.end(), order.begin(), order.end());
+          } else {
+            merged.schedule(&order);
+          }
+        }
         for (int i = 0; i < C.ndomains; ++i) {
           int id = order[i];
           tc.filterQs(id);
@@ -753,11 +773,13 @@ static void isectPrims(const Ctx& C, std::vector<TContext>& tcs, DomainStats& me
 #pragma omp critical(orc_retire)
       tc.retire();
       tc.swapQs();
+      ++pass;
     }
     tc.censusWindow();
     tc.mem_in->reset();
     std::swap(tc.mem_in, tc.mem_out);
     ray_depth += kHistory;
+    ++window;
     if (tid == 0) tc.st.windows++;
     if (vote(3, tc.pending_q.empty()) == nt) break;
   }
@@ -798,7 +820,7 @@ static void traceSpeculative(const Ctx& C, float* rgba, TraceStats* out) {
       tc.rstats.reset();
 #pragma omp for schedule(static, 1)
       for (long i = 0; i < (long)neyes; ++i) tc.isectAll(&eyes[i]);
-      isectPrims(C, tcs, merged, order, votes, tid);
+      isectPrims(C, tcs, merged, order, votes, tid, ti);
     }
   }
   for (auto& tc : tcs)
@@ -918,9 +940,11 @@ static void traceCanonical(const Ctx& C, float* rgba, TraceStats* out) {
   out->tiles += (uint64_t)(t1 - t0);
 }
 
-void trace(const Scene& s, const TraceConfig& cfg, float* rgba, TraceStats* st) {
+void trace(const Scene& s, const TraceConfig& cfg, float* rgba, TraceStats* st, SchedLog* log) {
   Ctx C;
   C.s = &s;
+  C.log = log;
+  if (log) log->ndomains = (int)s.domains.size();
   C.cfg = cfg;
   C.ndomains = (int)s.domains.size();
   if (C.ndomains <= 0) throw std::runtime_error("empty scene");
@@ -929,8 +953,7 @@ void trace(const Scene& s, const TraceConfig& cfg, float* rgba, TraceStats* st) 
   if (cfg.nthreads <= 0) throw std::runtime_error("nthreads must be positive");
   if (cfg.ao_mode) {
     C.nlights = cfg.ao_samples;
-  } else {
-    if (s.lights.empty()) throw std::runtime_error("path tracing needs at least one light");
+  } else {  // a scene without lights renders black (ooc_shader_pt.h:114 loops over none)
     C.nlights = (int)s.lights.size();
   }
   C.cam.init(v3(cfg.cam_pos[0], cfg.cam_pos[1], cfg.cam_pos[2]),

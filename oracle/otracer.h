@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "oscene.h"
 
@@ -41,8 +42,22 @@ struct TraceStats {
   double seconds = 0;
 };
 
+// DomainStats schedule of every pass of the speculative tracer (test hook):
+// per record meta = {tile, window, pass in window, entries queued}, the
+// score of every domain and the visit order.
+struct SchedLog {
+  int max_records = 0;
+  int ndomains = 0;
+  std::vector<int> meta;        // 4 per record
+  std::vector<int64_t> scores;  // ndomains per record
+  std::vector<int> order;       // ndomains per record
+  int records() const { return (int)meta.size() / 4; }
+};
+
 // Renders into rgba (image_w*image_h*4 floats, cleared by the caller).
-void trace(const Scene& s, const TraceConfig& cfg, float* rgba, TraceStats* st);
+// `log` (may be null) receives the schedule of the first passes.
+void trace(const Scene& s, const TraceConfig& cfg, float* rgba, TraceStats* st,
+           SchedLog* log = nullptr);
 
 // Number of blocking tiles (render/tile.cc:317-391) for this configuration.
 int numTiles(const TraceConfig& cfg);

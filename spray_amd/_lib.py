@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libspray_gpu.so")
+# SPRAY_LIB names another build of the same library (e.g. the host-sanitized
+# one the CPU suite can run against)
+LIB_PATH = os.environ.get("SPRAY_LIB") or os.path.join(_HERE, "libspray_gpu.so")
 
 SPRAY_OK = 0
 SPRAY_ERR_INVALID = 1
@@ -56,6 +58,7 @@ class spray_stats(C.Structure):
         ("node_visits", C.c_ulonglong), ("tri_tests", C.c_ulonglong),
         ("trace_ms", C.c_double), ("build_ms", C.c_double),
         ("cache_hits", C.c_ulonglong), ("cache_misses", C.c_ulonglong), ("cache_bytes", C.c_ulonglong),
+        ("regrows", C.c_ulonglong),
     ]
 
 
@@ -105,9 +108,12 @@ PROTOTYPES = {
     "spray_set_tiles_in_flight": (C.c_int, [_P, C.c_int]),
     "spray_set_bvh_builder": (C.c_int, [_P, C.c_int]),
     "spray_set_cache_size": (C.c_int, [_P, C.c_int]),
+    "spray_set_pool_headroom": (C.c_int, [_P, C.c_double]),
     "spray_cache_state": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_ulonglong)]),
     "spray_profile_kernels": (C.c_int, [_P, C.c_int]),
     "spray_kernel_time": (C.c_int, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.c_int]),
+    "spray_schedule_log": (C.c_int, [_P, C.c_int]),
+    "spray_schedule_read": (C.c_int, [_P, _IP, _IP, C.POINTER(C.c_ulonglong), _IP]),
     "spray_predictor_log": (C.c_int, [_P, C.c_int, C.c_longlong]),
     "spray_predictor_num_samples": (C.c_longlong, [_P]),
     "spray_predictor_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, _FP]),
@@ -119,6 +125,8 @@ PROTOTYPES = {
     "spray_mlp_logits": (C.c_int, [_P, _FP, C.c_int, _FP, _FP]),
     "spray_write_ppm": (C.c_int, [_FP, C.c_int, C.c_int, C.c_char_p]),
     "spray_scenegen_write": (C.c_int, [C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_longlong)]),
+    "spray_scenegen_domain": (C.c_int, [C.POINTER(spray_scenegen_params), C.c_int, _FP, C.POINTER(C.c_uint),
+                                        C.POINTER(C.c_uint), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "spray_scenegen_load": (C.c_int, [_P, C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p]),
     "spray_remesh": (C.c_int, [_P, C.c_double]),
     "spray_domain_mesh": (C.c_int, [_P, C.c_int, _FP, C.POINTER(C.c_uint), C.POINTER(C.c_uint),

@@ -59,6 +59,25 @@ def scenegen_write(directory, name, nx, ny, nz, res, a=6.0, b=17.0, phase=0.0, r
     return tris.value
 
 
+def scenegen_domain(domain, nx, ny, nz, res, a=6.0, b=17.0, phase=0.0, rgb=0xFFFFFF,
+                    per_domain_random=False, seed=0, tri_total=0.0):
+    """One domain's mesh from the CPU generator: (verts[n, 3], faces[m, 3], rgb[n])."""
+    lib = L.load()
+    p = _gen_params(nx, ny, nz, res, a, b, phase, rgb, per_domain_random, seed, tri_total)
+    nv, nf = C.c_longlong(0), C.c_longlong(0)
+    rc = lib.spray_scenegen_domain(C.byref(p), domain, None, None, None, C.byref(nv), C.byref(nf))
+    if rc != L.SPRAY_OK:
+        raise SprayError(rc, "spray_scenegen_domain")
+    v = np.zeros((nv.value, 3), np.float32)
+    f = np.zeros((nf.value, 3), np.uint32)
+    c = np.zeros(nv.value, np.uint32)
+    rc = lib.spray_scenegen_domain(C.byref(p), domain, _fp(v), f.ctypes.data_as(C.POINTER(C.c_uint)),
+                                   c.ctypes.data_as(C.POINTER(C.c_uint)), C.byref(nv), C.byref(nf))
+    if rc != L.SPRAY_OK:
+        raise SprayError(rc, "spray_scenegen_domain")
+    return v, f, c
+
+
 def write_ply_info(infile, outfile, info):
     """The ply_header_reader's output file (apps/ply_header_reader.cc:90-104):
     file, vertex and face lines and the bound line with 16 significant
@@ -251,6 +270,28 @@ class Context:
         n, ms = C.c_longlong(), C.c_double()
         self._check(self.lib.spray_kernel_time(self.h, C.byref(n), C.byref(ms), int(reset)))
         return n.value, ms.value
+
+    def set_pool_headroom(self, factor):
+        """Factor on the initial ray-pool estimate (smaller: overflow and regrow)."""
+        self._check(self.lib.spray_set_pool_headroom(self.h, float(factor)))
+
+    def schedule_log(self, max_records):
+        """Records the plain schedule's first max_records passes from the next
+        frame on (DomainStats scores and visit order per pass)."""
+        self._check(self.lib.spray_schedule_log(self.h, int(max_records)))
+        self._slog_max = int(max_records)
+
+    def schedule_read(self):
+        """(meta[n, 4] = (tile, window, pass, entries), scores[n, nd], order[n, nd])."""
+        nd, m = self.num_domains, max(1, getattr(self, "_slog_max", 0))
+        n = C.c_int(0)
+        meta = np.zeros((m, 4), np.int32)
+        scores = np.zeros((m, nd), np.uint64)
+        order = np.zeros((m, nd), np.int32)
+        self._check(self.lib.spray_schedule_read(self.h, C.byref(n), _ip(meta),
+                                                 scores.ctypes.data_as(C.POINTER(C.c_ulonglong)), _ip(order)))
+        k = n.value
+        return meta[:k], scores[:k].astype(np.int64), order[:k]
 
     # learned schedule
     def predictor_log(self, enable, max_samples=0):

@@ -53,6 +53,17 @@ void validateConfig(const spray_config& c) {
     throw StatusError(SPRAY_ERR_INVALID, "max_samples_per_rank too small for one image row");
 }
 
+// Rays the reference would hand to Embree with an undefined result (a NaN
+// or all-zero direction) are rejected with the first offending index.
+void checkRays(const float* dx, const float* dy, const float* dz, size_t stride, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    const float x = dx[i * stride], y = dy[i * stride], z = dz[i * stride];
+    if (std::isnan(x) || std::isnan(y) || std::isnan(z) || (x == 0.f && y == 0.f && z == 0.f))
+      throw StatusError(SPRAY_ERR_INVALID,
+                        "ray " + std::to_string(i) + " has a NaN or zero direction");
+  }
+}
+
 // render/camera.h:128-166 (Camera::init) on the host
 void cameraInit(Ctx* c, const float pos[3], const float look[3], const float up[3], float vfov) {
   const spray_config& cfg = c->cfg;
@@ -321,6 +332,7 @@ int spray_trace_rays(spray_ctx* ctx, const spray_rays_soa* rays, size_t n, spray
     if (!hits->prim || (mode == SPRAY_QUERY_CLOSEST && (!hits->t || !hits->u || !hits->v)))
       throw StatusError(SPRAY_ERR_INVALID, "missing output arrays");
     if (n == 0) return;
+    checkRays(rays->dx, rays->dy, rays->dz, 1, n);
     SPRAY_HIP(hipSetDevice(c->device));
     c->q_in.reserve(n * 48 + 256);  // 8 input + 4 output floats per ray
     float* d_org = c->q_in.as<float>();
@@ -368,6 +380,7 @@ int spray_domain_list(spray_ctx* ctx, const float* org, const float* dir, size_t
     needScene(c);
     if (c->pool.ndomains > 1024) throw StatusError(SPRAY_ERR_CAPACITY, "domain list query supports <= 1024 domains");
     if (n == 0) return;
+    checkRays(dir, dir + 1, dir + 2, 3, n);
     SPRAY_HIP(hipSetDevice(c->device));
     size_t bytes = n * 24 + n * 4 + n * (size_t)maxk * 8 + 256;
     c->q_in.reserve(bytes);
@@ -413,8 +426,8 @@ int spray_autotune(spray_ctx* ctx, int* chunk_out, int* tile_out) {
   return guarded(ctx, [&] {
     needScene(&ctx->c);
     SPRAY_HIP(hipSetDevice(ctx->c.device));
-    int ch = 0, tl = 0;
-    tracerAutotune(&ctx->c, &ch, &tl);
+    int ch = 0, tl = 0, cached = 0;
+    tracerAutotune(&ctx->c, &ch, &tl, &cached);
     if (chunk_out) *chunk_out = ch;
     if (tile_out) *tile_out = tl;
   });
@@ -446,6 +459,14 @@ int spray_set_cache_size(spray_ctx* ctx, int domains) {
     // render/config.cc:200-202 takes any value; lru_cache.cc:44 rejects 0
     if (domains == 0) throw StatusError(SPRAY_ERR_INVALID, "cache size must not be 0");
     ctx->c.cache_size = domains;
+  });
+}
+
+int spray_set_pool_headroom(spray_ctx* ctx, double factor) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (!(factor > 0.0 && factor <= 8.0)) throw StatusError(SPRAY_ERR_INVALID, "headroom must be in (0, 8]");
+    ctx->c.pool_headroom = factor;
   });
 }
 
@@ -492,10 +513,43 @@ int spray_profile_kernels(spray_ctx* ctx, int enable) {
   return SPRAY_OK;
 }
 
+int spray_schedule_log(spray_ctx* ctx, int max_records) {
+  if (!ctx || max_records < 0) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (!ctx->c.loaded) throw StatusError(SPRAY_ERR_STATE, "no scene loaded");
+    tracerScheduleLog(&ctx->c, max_records);
+  });
+}
+
+int spray_schedule_read(spray_ctx* ctx, int* nrec, int* meta, unsigned long long* scores, int* order) {
+  if (!ctx || !nrec) return SPRAY_ERR_INVALID;
+  *nrec = tracerScheduleRead(&ctx->c, meta, scores, order);
+  return SPRAY_OK;
+}
+
 int spray_kernel_time(spray_ctx* ctx, long long* launches, double* total_ms, int reset) {
   if (!ctx || !launches || !total_ms) return SPRAY_ERR_INVALID;
   tracerKernelTime(&ctx->c, launches, total_ms, reset);
   return SPRAY_OK;
+}
+
+int spray_scenegen_domain(const spray_scenegen_params* p, int domain, float* v, unsigned int* f,
+                          unsigned int* rgb, long long* nverts, long long* nfaces) {
+  if (!p || domain < 0 || domain >= p->nx * p->ny * p->nz || p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
+    return SPRAY_ERR_INVALID;
+  if (!p->per_domain_random && p->res <= 0) return SPRAY_ERR_INVALID;
+  try {
+    HostMesh m;
+    mcGenerate(*p, domain, &m);
+    if (nverts) *nverts = (long long)m.v.size() / 3;
+    if (nfaces) *nfaces = (long long)m.f.size() / 3;
+    if (v) std::copy(m.v.begin(), m.v.end(), v);
+    if (f) std::copy(m.f.begin(), m.f.end(), f);
+    if (rgb) std::copy(m.rgb.begin(), m.rgb.end(), rgb);
+    return SPRAY_OK;
+  } catch (const std::bad_alloc&) {
+    return SPRAY_ERR_OOM;
+  }
 }
 
 int spray_scenegen_write(const spray_scenegen_params* p, const char* dir, const char* name,

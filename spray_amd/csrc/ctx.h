@@ -153,6 +153,7 @@ struct Ctx {
   int batch_tile = 4;      // waves per block of the persistent kernel (block = 64 * tile)
   int param_version = 0;   // bumped when camera / config change (graph re-capture)
   bool profile = false;    // time every trace-kernel launch with hipEvents
+  double pool_headroom = 1.0;  // factor on the initial ray-pool estimate (test hook)
   std::string err;
   // scratch for spray_trace_rays / spray_domain_list
   DevBuf q_in, q_out;
@@ -192,8 +193,10 @@ void domainListDevice(Ctx* c, const float* d_org, const float* d_dir, size_t n, 
 // tracer.hip
 void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st);
 void tracerRelease(Ctx* c);
-void tracerAutotune(Ctx* c, int* chunk, int* tile);
+void tracerAutotune(Ctx* c, int* chunk, int* tile, int* cached);
 void tracerKernelTime(Ctx* c, long long* launches, double* ms, int reset);
+void tracerScheduleLog(Ctx* c, int max_records);
+int tracerScheduleRead(Ctx* c, int* meta, unsigned long long* scores, int* order);
 void tracerSetLogging(Ctx* c, bool on, long long max_samples);
 long long tracerLogCount(const Ctx* c);
 const float* tracerLogBuffer(Ctx* c);

@@ -422,7 +422,7 @@ __device__ void mcAxisTables(const McGrid& g, double* tab) {
   for (int q = threadIdx.x; q < 3 * g.N; q += blockDim.x) {
     const int ax = q / g.N, i = q - ax * g.N;
     const double c = (ax == 0 ? g.ox : (ax == 1 ? g.oy : g.oz)) + (double)i * g.h;
-    tab[q] = ax == 0 ? sin(g.a * c + g.phase) : sin(g.a * c);
+    tab[q] = ax == 0 ? sinD(g.a * c + g.phase) : sinD(g.a * c);
   }
   __syncthreads();
 }
@@ -434,7 +434,7 @@ __device__ void mcAxisTables(const McGrid& g, double* tab) {
 __device__ __forceinline__ double mcFieldAt(const McGrid& g, const double* tab, int i, int j,
                                             int k) {
   const double x = g.ox + (double)i * g.h, y = g.oy + (double)j * g.h, z = g.oz + (double)k * g.h;
-  return tab[i] * tab[g.N + j] * tab[2 * g.N + k] + 0.3 * sin(g.b * (x + y + z) + g.phase);
+  return tab[i] * tab[g.N + j] * tab[2 * g.N + k] + 0.3 * sinD(g.b * (x + y + z) + g.phase);
 }
 
 // Field signs as bytes (1: inside, f < 0): the cut and case tests read one
@@ -872,11 +872,6 @@ struct McSource : MeshSource {
   }
 };
 
-void requireLight(const Ctx* c, const HostScene& scene) {
-  if (c->cfg.ao_mode == 0 && scene.lights.empty())
-    throw StatusError(SPRAY_ERR_INVALID, "path tracing needs at least one light");
-}
-
 long long domainVerts(const DomainPool& P, int d) { return P.nverts[d]; }
 
 }  // namespace
@@ -886,7 +881,6 @@ void loadScenePly(Ctx* c, HostScene& scene) {
   c->generated = false;
   c->gen_store.release();
   PlySource src(c);
-  requireLight(c, c->scene);
   tracerRelease(c);
   buildPool(c, src);
 }
@@ -913,7 +907,6 @@ void generateScene(Ctx* c, const spray_scenegen_params& p, const std::string& na
   std::string err;
   int rc = parseSceneText(text, name + ".spray", "", &scene, &err);
   if (rc != SPRAY_OK) throw StatusError(rc, err);
-  requireLight(c, scene);
   c->scene = std::move(scene);
   if (remesh) {
     c->param_version++;  // pool addresses change: re-capture the pass graphs

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "dmath.h"
 #include "spray_internal.h"
 
 namespace spray {
@@ -171,8 +172,7 @@ void mcGenerate(const spray_scenegen_params& p, int d, HostMesh* m) {
       for (int i = 0; i < N; ++i) {
         double x = X(i, di), y = X(j, dj), z = X(k, dk);
         f[((size_t)k * N + j) * N + i] =
-            std::sin(a * x + p.phase) * std::sin(a * y) * std::sin(a * z) +
-            0.3 * std::sin(b * (x + y + z) + p.phase);
+            sinD(a * x + p.phase) * sinD(a * y) * sinD(a * z) + 0.3 * sinD(b * (x + y + z) + p.phase);
       }
   auto F = [&](int i, int j, int k) { return f[((size_t)k * N + j) * N + i]; };
   // edges along axis: count of edges per axis = n * N * N

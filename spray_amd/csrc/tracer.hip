@@ -21,9 +21,15 @@
 // launch and captures a whole tile in a hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <string>
 
 #include "ctx.h"
 #include "tracer_dev.h"
@@ -1292,12 +1298,26 @@ struct TracerState {
   bool logging = false;
   DevBuf log_ctr;
   int grid_persistent = 0;
-  hipGraphExec_t graph = nullptr;
-  int graph_key[6] = {0, 0, 0, 0, 0, 0};
+  // captured tile graphs by key (schedule, windows, batch parameters,
+  // logging, configuration version).  A graph whose buffers were reallocated
+  // is retired, never launched again, and destroyed only when the tracer is
+  // released, so no graph is destroyed while the context is tracing.
+  struct KeyedGraph {
+    int key[6];
+    hipGraphExec_t exec;
+  };
+  std::vector<KeyedGraph> graphs;
+  std::vector<hipGraphExec_t> retired;
   std::vector<uint32_t> h_cnt;
   std::vector<float> h_box;
   std::vector<int> h_order;
   unsigned long long paged_passes = 0;  // alternates the chunk direction of paged passes
+  // schedule log of the plain form (test hook): where the frame is, and per
+  // recorded pass {tile, window, pass, entries}, the scores and the order
+  int cur_tile = 0, cur_window = 0, cur_pass = 0;
+  int slog_max = 0;
+  std::vector<int> slog_meta, slog_order;
+  std::vector<unsigned long long> slog_scores;
   size_t heldBytes() const {
     size_t b = log_ctr.bytes;
     for (const DevBuf* x : {R, Pn, Sh})
@@ -1335,16 +1355,33 @@ struct TracerState {
     }
     ev_used = 0;
   }
+  hipGraphExec_t findGraph(const int key[6]) const {
+    for (const KeyedGraph& g : graphs)
+      if (std::memcmp(g.key, key, sizeof(g.key)) == 0) return g.exec;
+    return nullptr;
+  }
+  void trimRetired() {  // a bound on what a long-lived context keeps (the oldest go first)
+    while (retired.size() > 64) {
+      (void)hipGraphExecDestroy(retired.front());
+      retired.erase(retired.begin());
+    }
+  }
   void releaseGraph() {
-    if (graph) (void)hipGraphExecDestroy(graph);
-    graph = nullptr;
+    for (const KeyedGraph& g : graphs) retired.push_back(g.exec);
+    graphs.clear();
+    trimRetired();
+  }
+  void destroyGraphs() {
+    releaseGraph();
+    for (hipGraphExec_t e : retired) (void)hipGraphExecDestroy(e);
+    retired.clear();
   }
 };
 
 void tracerRelease(Ctx* c) {
   if (!c->tracer) return;
   TracerState* T = c->tracer;
-  T->releaseGraph();
+  T->destroyGraphs();
   for (hipEvent_t e : T->ev) (void)hipEventDestroy(e);
   for (auto* arr : {T->R, T->Pn, T->Sh})
     for (int k = 0; k < 4; ++k) arr[k].release();
@@ -1756,6 +1793,15 @@ static bool passPlain(Ctx* c, const TraceParams& P) {
   SPRAY_HIP(hipMemcpyAsync(T->h_order.data(), P.order, nd * 4, hipMemcpyDeviceToHost, s));
   SPRAY_HIP(hipStreamSynchronize(s));
   if (!h.active) return false;
+  if ((int)T->slog_meta.size() / 4 < T->slog_max) {
+    unsigned long long entries = 0;
+    for (int k = 0; k < 2 * nd; ++k) entries += T->h_cnt[k];
+    T->slog_meta.insert(T->slog_meta.end(), {T->cur_tile, T->cur_window, T->cur_pass, (int)entries});
+    const size_t at = T->slog_scores.size();
+    T->slog_scores.resize(at + nd);
+    SPRAY_HIP(hipMemcpy(T->slog_scores.data() + at, P.score, (size_t)nd * 8, hipMemcpyDeviceToHost));
+    T->slog_order.insert(T->slog_order.end(), T->h_order.begin(), T->h_order.begin() + nd);
+  }
   hipDeviceProp_t prop;
   SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
   const int max_grid = prop.multiProcessorCount * (1536 / kTravBlock);
@@ -1898,6 +1944,7 @@ static void tileBody(Ctx* c, const TraceParams& P, bool batched, int windows, in
   kTileStart<<<1, 1, 0, s>>>(P);
   kGenEyes<<<eye_grid, 256, 0, s>>>(P);
   for (int w = 0; w < windows; ++w) {
+    c->tracer->cur_window = w;
     if (w > 0) {
       kWindowStart<<<1, 1, 0, s>>>(P);
       kRestartPending<<<1024, 256, 0, s>>>(P);
@@ -1905,6 +1952,7 @@ static void tileBody(Ctx* c, const TraceParams& P, bool batched, int windows, in
     }
     kVbufReset<<<eye_grid, 256, 0, s>>>(P);
     for (int p = 0; p <= kHistory; ++p) {
+      c->tracer->cur_pass = p;
       if (batched && c->pool.paged) {
         if (!passPagedBatched(c, P)) break;
       } else if (batched) {
@@ -1928,8 +1976,8 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
   tileList(cfg, &ys, &hs);
   const unsigned long long hits0 = c->pool.hits, misses0 = c->pool.misses,
                            bytes0 = c->pool.bytes_in;
-  double grow = 1.0;
-  for (int attempt = 0; attempt < 4; ++attempt) {
+  double grow = c->pool_headroom;
+  for (unsigned long long regrows = 0;; ++regrows) {
     ensureState(c, grow);
     TracerState* T = c->tracer;
     hipStream_t s = c->stream;
@@ -1955,24 +2003,38 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
                   P.log_on + 2 * c->param_version};
     // profiling times each launch instead; a paged pool is driven from the host
     const bool use_graph = batched && !c->profile && !c->pool.paged;
-    if (use_graph && (T->graph == nullptr || std::memcmp(key, T->graph_key, sizeof(key)) != 0)) {
-      T->releaseGraph();
+    hipGraphExec_t exec = use_graph ? T->findGraph(key) : nullptr;
+    if (use_graph && exec == nullptr) {
       hipGraph_t g;
       SPRAY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       tileBody(c, P, true, windows, nullptr);
       SPRAY_HIP(hipStreamEndCapture(s, &g));
-      SPRAY_HIP(hipGraphInstantiate(&T->graph, g, nullptr, nullptr, 0));
+      SPRAY_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
       SPRAY_HIP(hipGraphDestroy(g));
-      std::memcpy(T->graph_key, key, sizeof(key));
+      // graphs of an older configuration version (remeshed pool, new camera)
+      // are never launched again
+      std::vector<TracerState::KeyedGraph> keep;
+      for (const TracerState::KeyedGraph& g : T->graphs)
+        if ((g.key[5] >> 1) == (key[5] >> 1))
+          keep.push_back(g);
+        else
+          T->retired.push_back(g.exec);
+      T->graphs.swap(keep);
+      T->trimRetired();
+      TracerState::KeyedGraph kg;
+      std::memcpy(kg.key, key, sizeof(key));
+      kg.exec = exec;
+      T->graphs.push_back(kg);
     }
     const int per_batch = batched ? tilesPerBatch(c, hs) : 1;
     for (size_t b0 = 0; b0 < ys.size(); b0 += per_batch) {
       int rows = 0;
       for (size_t ti = b0; ti < std::min(ys.size(), b0 + (size_t)per_batch); ++ti) rows += hs[ti];
       int rec[4] = {ys[b0], rows, 0, 0};
+      T->cur_tile = (int)b0;
       SPRAY_HIP(hipMemcpyAsync(T->tile_rec.p, rec, sizeof(rec), hipMemcpyHostToDevice, s));
       if (use_graph)
-        SPRAY_HIP(hipGraphLaunch(T->graph, s));
+        SPRAY_HIP(hipGraphLaunch(exec, s));
       else
         tileBody(c, P, batched, windows, nullptr);
     }
@@ -1987,7 +2049,8 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     (void)hipEventDestroy(e1);
     if (h.err & 4u) throw StatusError(SPRAY_ERR_CAPACITY, "traversal stack overflow");
     if (h.err & 1u) {  // a pool or queue bound was hit: grow and render again
-      grow *= 2.0;
+      if (grow >= 8.0) throw StatusError(SPRAY_ERR_CAPACITY, "ray pools exceeded after growing 8x");
+      grow = std::min(8.0, 2.0 * grow);
       continue;
     }
     if (rgba_out)
@@ -2011,9 +2074,9 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     st->cache_hits = c->pool.hits - hits0;
     st->cache_misses = c->pool.misses - misses0;
     st->cache_bytes = c->pool.bytes_in - bytes0;
+    st->regrows = regrows;
     return;
   }
-  throw StatusError(SPRAY_ERR_CAPACITY, "ray pools exceeded after growing 8x");
 }
 
 void tracerSetLogging(Ctx* c, bool on, long long max_samples) {
@@ -2056,34 +2119,147 @@ void tracerKernelTime(Ctx* c, long long* launches, double* ms, int reset) {
   }
 }
 
-void tracerAutotune(Ctx* c, int* chunk_out, int* tile_out) {
-  int saved_sched = c->cfg.schedule;
+void tracerScheduleLog(Ctx* c, int max_records) {
+  ensureState(c, 1.0);
+  TracerState* T = c->tracer;
+  T->slog_max = std::max(0, max_records);
+  T->slog_meta.clear();
+  T->slog_scores.clear();
+  T->slog_order.clear();
+}
+
+int tracerScheduleRead(Ctx* c, int* meta, unsigned long long* scores, int* order) {
+  const TracerState* T = c->tracer;
+  if (!T) return 0;
+  const int n = (int)T->slog_meta.size() / 4;
+  if (meta) std::copy(T->slog_meta.begin(), T->slog_meta.end(), meta);
+  if (scores) std::copy(T->slog_scores.begin(), T->slog_scores.end(), scores);
+  if (order) std::copy(T->slog_order.begin(), T->slog_order.end(), order);
+  return n;
+}
+
+// Autotuner of the batched forms: times every (queue chunk, waves per
+// block) pair on a tuning frame and keeps the fastest.  The tuning frame is
+// the loaded scene and camera at half the samples per pixel (at least one),
+// so the pair is not chosen on the frames it then renders.  Results are kept
+// in a text cache (SPRAY_AUTOTUNE_CACHE, else $HOME/.cache/spray_amd/
+// autotune.txt), one line per key: the device (name, compute units), the
+// frame settings (image, samples, bounces, AO, Russian roulette, schedule,
+// tiles in flight, BVH builder) and a hash of the scene (every domain's
+// triangle count and bounds).  A key found in the cache is used without
+// tuning.
+namespace {
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = (const unsigned char*)p;
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
+}
+
+std::string autotuneKey(Ctx* c) {
+  hipDeviceProp_t prop;
+  SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
+  uint64_t h = 1469598103934665603ull;
+  for (const DomainDesc& D : c->pool.desc) {
+    h = fnv(h, &D.num_tris, sizeof(D.num_tris));
+    h = fnv(h, D.lo, sizeof(D.lo));
+    h = fnv(h, D.hi, sizeof(D.hi));
+    h = fnv(h, &D.mtl, sizeof(D.mtl));
+  }
+  const spray_config& g = c->cfg;
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%s|cu%d|%dx%d|spp%d|b%d|ao%d,%d,%.9g|rr%d|s%d|tif%d|bvh%d|scene%016llx",
+                prop.name, prop.multiProcessorCount, g.image_w, g.image_h, g.pixel_samples, g.bounces,
+                g.ao_mode, g.ao_samples, (double)g.ao_max_dist, g.rr_start, g.schedule, c->tiles_in_flight,
+                c->bvh_builder, (unsigned long long)h);
+  std::string key(buf);
+  for (char& ch : key)
+    if (ch == ' ' || ch == '\t' || ch == '\n') ch = '_';
+  return key;
+}
+
+std::string autotuneCachePath() {
+  if (const char* p = std::getenv("SPRAY_AUTOTUNE_CACHE")) return p;
+  const char* home = std::getenv("HOME");
+  return std::string(home ? home : "/tmp") + "/.cache/spray_amd/autotune.txt";
+}
+
+bool autotuneLookup(const std::string& path, const std::string& key, int* chunk, int* tile) {
+  std::ifstream in(path);
+  std::string k;
+  int ch = 0, tl = 0;
+  double ms = 0;
+  bool found = false;
+  while (in >> k >> ch >> tl >> ms)
+    if (k == key && ch >= 1 && ch <= 64 && tl >= 1 && tl <= 4) {
+      *chunk = ch;  // the last line of a key wins
+      *tile = tl;
+      found = true;
+    }
+  return found;
+}
+
+void autotuneStore(const std::string& path, const std::string& key, int chunk, int tile, double ms) {
+  const size_t slash = path.find_last_of('/');
+  if (slash != std::string::npos && slash > 0) {
+    std::string dir = path.substr(0, slash);
+    for (size_t i = 1; i <= dir.size(); ++i)
+      if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+  }
+  std::ofstream out(path, std::ios::app);
+  if (out) out << key << ' ' << chunk << ' ' << tile << ' ' << ms << '\n';  // a cache: failures are not fatal
+}
+}  // namespace
+
+void tracerAutotune(Ctx* c, int* chunk_out, int* tile_out, int* cached_out) {
+  const int saved_sched = c->cfg.schedule, saved_spp = c->cfg.pixel_samples;
   if (c->cfg.schedule == SPRAY_SCHEDULE_PLAIN) c->cfg.schedule = SPRAY_SCHEDULE_BATCHED;
+  const std::string key = autotuneKey(c), path = autotuneCachePath();
+  int bc = c->batch_chunk, bt = c->batch_tile;
+  if (autotuneLookup(path, key, &bc, &bt)) {
+    c->batch_chunk = bc;
+    c->batch_tile = bt;
+    c->cfg.schedule = saved_sched;
+    *chunk_out = bc;
+    *tile_out = bt;
+    *cached_out = 1;
+    return;
+  }
   // queue chunk = lanes a wave refills at once; tile = waves per block
   const int chunks[] = {8, 16, 32, 64};
   const int tiles[] = {1, 2, 4};
   double best = 1e30;
-  int bc = c->batch_chunk, bt = c->batch_tile;
-  for (int ch : chunks)
-    for (int tl : tiles) {
-      c->batch_chunk = ch;
-      c->batch_tile = tl;
-      spray_stats st;
-      std::memset(&st, 0, sizeof(st));
-      tracerTraceFrame(c, nullptr, &st);  // warm (graph capture)
-      tracerTraceFrame(c, nullptr, &st);
-      if (st.trace_ms < best) {
-        best = st.trace_ms;
-        bc = ch;
-        bt = tl;
+  c->cfg.pixel_samples = std::max(1, saved_spp / 2);
+  c->param_version++;
+  try {
+    for (int ch : chunks)
+      for (int tl : tiles) {
+        c->batch_chunk = ch;
+        c->batch_tile = tl;
+        spray_stats st;
+        std::memset(&st, 0, sizeof(st));
+        tracerTraceFrame(c, nullptr, &st);  // warm (graph capture)
+        tracerTraceFrame(c, nullptr, &st);
+        if (st.trace_ms < best) {
+          best = st.trace_ms;
+          bc = ch;
+          bt = tl;
+        }
       }
-    }
+  } catch (...) {
+    c->cfg.pixel_samples = saved_spp;
+    c->cfg.schedule = saved_sched;
+    c->param_version++;
+    throw;
+  }
+  c->cfg.pixel_samples = saved_spp;
+  c->cfg.schedule = saved_sched;
+  c->param_version++;
   c->batch_chunk = bc;
   c->batch_tile = bt;
-  c->cfg.schedule = saved_sched;
-  if (c->tracer) c->tracer->releaseGraph();
+  autotuneStore(path, key, bc, bt, best);
   *chunk_out = bc;
   *tile_out = bt;
+  *cached_out = 0;
 }
 
 }  // namespace spray

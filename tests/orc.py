@@ -6,7 +6,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "oracle", "liborc.so")
+LIB = os.environ.get("ORC_LIB") or os.path.join(ROOT, "oracle", "liborc.so")
 
 
 class TraceConfig(C.Structure):
@@ -77,6 +77,8 @@ def lib():
                                       C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p]
         l.orc_num_tiles.argtypes = [C.POINTER(TraceConfig)]
         l.orc_render.argtypes = [C.c_void_p, C.POINTER(TraceConfig), C.c_void_p, C.POINTER(TraceStats)]
+        l.orc_render_schedule.argtypes = [C.c_void_p, C.POINTER(TraceConfig), C.c_void_p, C.POINTER(TraceStats),
+                                          C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         l.orc_ply_read.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         assert l.orc_sizeof_config() == C.sizeof(TraceConfig), "TraceConfig layout mismatch"
         assert l.orc_sizeof_stats() == C.sizeof(TraceStats), "TraceStats layout mismatch"
@@ -188,6 +190,28 @@ class Scene:
         if self.l.orc_render(self.h, C.byref(cfg), p(img), C.byref(st)) != 0:
             raise OracleError(self.l.orc_last_error().decode())
         return img, {f: getattr(st, f) for f, _ in TraceStats._fields_}
+
+
+def _render_schedule(self, max_records=256, **kw):
+    """render() plus the DomainStats schedule of the first max_records passes:
+    (img, stats, meta[n, 4] = (tile, window, pass, entries), scores[n, nd],
+    order[n, nd])."""
+    cfg = make_config(**kw)
+    img = np.zeros((cfg.image_h, cfg.image_w, 4), np.float32)
+    st = TraceStats()
+    nd = self.num_domains
+    nrec = C.c_int(0)
+    meta = np.zeros((max_records, 4), np.int32)
+    scores = np.zeros((max_records, nd), np.int64)
+    order = np.zeros((max_records, nd), np.int32)
+    if self.l.orc_render_schedule(self.h, C.byref(cfg), p(img), C.byref(st), max_records, C.byref(nrec),
+                                  p(meta), p(scores), p(order)) != 0:
+        raise OracleError(self.l.orc_last_error().decode())
+    n = nrec.value
+    return img, {f: getattr(st, f) for f, _ in TraceStats._fields_}, meta[:n], scores[:n], order[:n]
+
+
+Scene.render_schedule = _render_schedule
 
 
 def make_config(**kw):

@@ -108,6 +108,15 @@ def test_learned_schedule_same_image_fewer_visits(scene, tmp_path):
         assert np.sqrt((diff ** 2).mean()) <= 1e-4 and diff.max() <= 1e-3
     assert learned["deferred_entries"] > 0
     assert learned["radiance_visits"] < batched["radiance_visits"]
+    # and against the oracle on the same 64-domain scene and rays
+    ref = orc.Scene(os.path.join(scene, "pt.spray"), scene, nthreads=8)
+    rimg, rst = ref.render(image_w=128, image_h=128, pixel_samples=4, ao_mode=0, ao_samples=1, bounces=5,
+                           rr_start=3, max_samples_per_rank=32768, nthreads=8, cam_pos=CAM["pos"],
+                           cam_lookat=CAM["lookat"], cam_up=CAM["up"])
+    diff = np.abs(limg[..., :3] - rimg[..., :3])
+    print(f"learned vs oracle: rmse {np.sqrt((diff ** 2).mean()):.3g} max {diff.max():.3g}")
+    assert learned["useful_rays"] == rst["useful_rays"] and learned["final_hits"] == rst["final_hits"]
+    assert np.sqrt((diff ** 2).mean()) <= 1e-4 and diff.max() <= 1e-3
     # save / load round trip keeps the network
     path = str(tmp_path / "pred.bin")
     ctx.predictor_save(path)
