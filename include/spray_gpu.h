@@ -240,7 +240,8 @@ int spray_predictor_load(spray_ctx* ctx, const char* path);
  * parameters (flat fp32: W1[128][22], b1[128], W2[128][128], b2[128],
  * w3[128], b3, emb[ndomains][16]); returns the summed BCE loss and the
  * gradient of the mean BCE (n a multiple of 32).  spray_mlp_logits runs the
- * forward only (params may be NULL: current weights). */
+ * forward only, with the learned schedule's inference arithmetic (params
+ * may be NULL: current weights). */
 int spray_mlp_num_params(spray_ctx* ctx);
 int spray_mlp_grad(spray_ctx* ctx, const float* samples, int n, const float* params,
                    float* loss_out, float* grad_out);

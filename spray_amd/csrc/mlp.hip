@@ -5,18 +5,22 @@
 // (v_mfma_f32_16x16x32_bf16), fp32 accumulation, fp32 master weights, Adam.
 //
 // Training data are (origin, direction, domain, label) samples logged by
-// the tracer.  Inference orders speculative domain visits: per radiance ray,
-// the domains entered no later than the most likely hit domain run in phase
-// A of a pass; the rest are deferred to the end of the pass's launch (or,
-// in small passes, to a second launch), where the tdom <= history filter
-// prunes most of them.  Exact traversal still decides every hit, so the
-// image does not depend on the network.
+// the tracer.  Inference orders speculative domain visits: per radiance ray
+// with several candidate domains, the network is asked about the nearest
+// candidate (and the second, when the nearest is predicted to miss); the
+// domains entered no later than the first one predicted to hold the hit run
+// in phase A of a pass, the rest are deferred to the end of the pass's
+// launch (or, in small passes, to a second launch), where the tdom <=
+// history filter prunes most of them.  Exact traversal still decides every
+// hit, so the image does not depend on the network.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "ctx.h"
@@ -39,6 +43,8 @@ constexpr int oEmb = oB3 + 1;              // [nd][16]
 constexpr int kDenseParams = oEmb;         // 19585
 
 struct MlpDev {
+  const __bf16* w1f;   // inference fragments of W1 and W2 (kPackFrags)
+  const __bf16* w2f;
   const __bf16* w1;    // [128][32] bf16, columns 22..31 zero
   const __bf16* w2;    // [128][128]
   const __bf16* w2t;   // [128][128] transposed (in, out)
@@ -160,217 +166,195 @@ __device__ __forceinline__ uint32_t orderedF(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// Weights live in LDS for the whole (persistent) launch: W1 [128][32] and
-// W2 [128][128] (out, in) with padded row strides so that the 16-byte
-// B-fragment reads of the 16 lanes sharing a k-slice hit distinct banks.
-// Each wave runs 32 queue entries (two 16-row blocks) per iteration, so
-// every B fragment read from LDS feeds two MFMAs.  A wave's feature rows and
-// hidden rows share one LDS area (the features are in registers before the
-// hidden rows are written), which leaves room for 12 waves per block.
-constexpr int kPredRows = 32;
-constexpr int kLdX = kInPad + 8;   // bf16 row stride of x and W1 in LDS
-constexpr int kLdH = kHid + 8;     // bf16 row stride of h1 and W2 in LDS
-constexpr int kPredWaves = 12;
-constexpr size_t kPredLdsWeights = (size_t)kHid * kLdX + (size_t)kHid * kLdH;  // bf16 elements
-constexpr size_t kPredLdsPerWave = (size_t)kPredRows * kLdH;
-static_assert(kLdX <= kLdH, "feature rows fit the hidden-row area");
+// Inference over the radiance rays of a pass (learned form).  A ray with
+// two or more candidate domains asks the network about its nearest
+// candidate and, when that one is predicted to miss and a third candidate
+// exists, about the second; the first candidate predicted to hold the hit
+// (logit > kHitLogit) bounds phase A: the ray's entries entering no later
+// than it.  A ray with no such prediction keeps every entry in phase A.
+//
+// The weights stay in registers for the whole persistent launch, laid out
+// as v_mfma_f32_16x16x32_bf16 A operands (hidden units as rows); the rays
+// are the B operand's 16 columns.  Layer 1's accumulators hold hidden
+// units 16 mt + 4 (l / 16) + i of ray l % 16 in lane l, and layer 2 takes
+// them as its B operand in place: its K-block kb reads slot j of lane group
+// g from unit 16 (2 kb + j / 4) + 4 g + j % 4, and the layer-2 weight
+// fragments are packed in that unit order (kPackFrags).  No LDS traffic
+// per MFMA; LDS holds only the biases and the output weights.
+constexpr float kHitLogit = 0.0f;
+constexpr int kInferWaves = 4;  // waves per block
 
-// One queue entry's inputs for inference, gathered ahead of their use.
-struct PredIn {
-  QEntry e;
-  int dom;
-  bool valid;  // a queue entry (not past the end)
-  int rc;      // the ray's candidate domains this pass
-  float4 op, ds;
+__device__ __forceinline__ int l2Unit(int kb, int g, int j) {
+  return 16 * (2 * kb + (j >> 2)) + 4 * g + (j & 3);
+}
+
+// w1f[mt][lane][j] = W1[16 mt + lane % 16][8 (lane / 16) + j] (features
+// padded to 32 with zeros); w2f[mt][kb][lane][j] = W2[16 mt + lane % 16]
+// [l2Unit(kb, lane / 16, j)].
+__global__ void kPackFrags(const float* p, __bf16* w1f, __bf16* w2f) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 8 * 4 * 64 * 8; t += gridDim.x * blockDim.x) {
+    const int j = t & 7, lane = (t >> 3) & 63;
+    if (t < 8 * 64 * 8) {
+      const int mt = t >> 9, o = 16 * mt + (lane & 15), k = 8 * (lane >> 4) + j;
+      w1f[t] = tobf(k < kIn ? p[oW1 + o * kIn + k] : 0.f);
+    }
+    const int kb = (t >> 9) & 3, mt = t >> 11, o = 16 * mt + (lane & 15);
+    w2f[t] = tobf(p[oW2 + o * kHid + l2Unit(kb, lane >> 4, j)]);
+  }
+}
+
+// Layer-1 input slots 8 g .. 8 g + 7 of one (ray, domain) sample: the
+// normalised origin and the direction (slots 0..5), the domain embedding
+// (6..21), zeros (22..31).  Gathered as floats ahead of their use.
+__device__ __forceinline__ void loadSlots(const MlpDev& M, int g, float4 op, float4 ds, int dom,
+                                          float x[8]) {
+  const float* e = M.p + oEmb + (size_t)dom * kEmb;
+  if (g == 0) {
+    x[0] = (op.x - M.center[0]) * M.inv_half[0];
+    x[1] = (op.y - M.center[1]) * M.inv_half[1];
+    x[2] = (op.z - M.center[2]) * M.inv_half[2];
+    x[3] = ds.x;
+    x[4] = ds.y;
+    x[5] = ds.z;
+    x[6] = e[0];
+    x[7] = e[1];
+  } else if (g == 1) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = e[2 + j];
+  } else if (g == 2) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) x[j] = e[10 + j];
+    x[6] = x[7] = 0.f;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = 0.f;
+  }
+}
+
+__device__ __forceinline__ bf16x8 toSlots(const float x[8]) {
+  bf16x8 b;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) b[j] = tobf(x[j]);
+  return b;
+}
+
+// The network on 16 samples (the B operand's columns): returns the logit of
+// column l % 16 in every lane.
+__device__ __forceinline__ float inferColumns(const bf16x8 (&w1f)[8], const bf16x8 (&w2f)[8][4],
+                                              const float* b1s, const float* b2s, const float* w3s,
+                                              float b3, bf16x8 xb) {
+  const int g = __lane_id() >> 4;
+  f32x4 acc[8];
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt)
+    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1f[mt], xb, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  bf16x8 hb[4];
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    const float4 bb = *(const float4*)(b1s + 16 * mt + 4 * g);
+    const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) hb[mt >> 1][4 * (mt & 1) + i] = tobf(fmaxf(acc[mt][i] + bv[i], 0.f));
+  }
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    f32x4 a = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2f[mt][kb], hb[kb], a, 0, 0, 0);
+    acc[mt] = a;
+  }
+  float z = 0.f;
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    const float4 bb = *(const float4*)(b2s + 16 * mt + 4 * g);
+    const float4 ww = *(const float4*)(w3s + 16 * mt + 4 * g);
+    z += fmaxf(acc[mt][0] + bb.x, 0.f) * ww.x + fmaxf(acc[mt][1] + bb.y, 0.f) * ww.y +
+         fmaxf(acc[mt][2] + bb.z, 0.f) * ww.z + fmaxf(acc[mt][3] + bb.w, 0.f) * ww.w;
+  }
+  z += __shfl_xor(z, 16, 64);
+  z += __shfl_xor(z, 32, 64);
+  return z + b3;
+}
+
+// One group of 16 rays as it moves through the software pipeline: the
+// candidate count and nearest candidates first, then the feature slots.
+struct InferGroup {
+  uint32_t r;
+  int rc;
+  uint4 nr;
+  float x0[8], x1[8];
 };
-constexpr size_t kPredLdsBytes =
-    (kPredLdsWeights + kPredWaves * kPredLdsPerWave) * 2 + 3 * kHid * 4 + 16;
 
-__global__ void __launch_bounds__(64 * kPredWaves) kPredictKeys(TraceParams P, MlpDev M,
-                                                               unsigned long long* rkey) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+__global__ void __launch_bounds__(64 * kInferWaves, 1) kPredictRays(TraceParams P, MlpDev M,
+                                                                   unsigned long long* rkey) {
+  __shared__ __attribute__((aligned(16))) float b1s[kHid], b2s[kHid], w3s[kHid];
   const DevCounters* c = P.ctr;
   if (!c->active) return;
-  __bf16* w1s = (__bf16*)smem;                 // [128][kLdX]
-  __bf16* w2s = w1s + (size_t)kHid * kLdX;     // [128][kLdH]
-  __bf16* wave_base = w2s + (size_t)kHid * kLdH;
-  float* b1s = (float*)(wave_base + kPredWaves * kPredLdsPerWave);
-  float* b2s = b1s + kHid;
-  float* w3s = b2s + kHid;
-  for (int i = threadIdx.x; i < kHid * kInPad / 8; i += blockDim.x) {
-    int r = i / (kInPad / 8), k8 = i % (kInPad / 8);
-    *(bf16x8*)(w1s + r * kLdX + 8 * k8) = *(const bf16x8*)(M.w1 + r * kInPad + 8 * k8);
-  }
-  for (int i = threadIdx.x; i < kHid * kHid / 8; i += blockDim.x) {
-    int r = i / (kHid / 8), k8 = i % (kHid / 8);
-    *(bf16x8*)(w2s + r * kLdH + 8 * k8) = *(const bf16x8*)(M.w2 + r * kHid + 8 * k8);
-  }
   for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
     b1s[i] = M.p[oB1 + i];
     b2s[i] = M.p[oB2 + i];
     w3s[i] = M.p[oW3 + i];
   }
   const float b3 = M.p[oB3];
+  const int l = __lane_id(), g = l >> 4, w = threadIdx.x >> 6;
+  bf16x8 w1f[8], w2f[8][4];
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    w1f[mt] = *(const bf16x8*)(M.w1f + ((size_t)mt * 64 + l) * 8);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) w2f[mt][kb] = *(const bf16x8*)(M.w2f + (((size_t)mt * 4 + kb) * 64 + l) * 8);
+  }
   __syncthreads();
-  const uint32_t n = c->q_total[0];
-  const int w = threadIdx.x >> 6, l = __lane_id();
-  __bf16* xs = wave_base + w * kPredLdsPerWave;  // [32][kLdX], then
-  __bf16* hs = xs;                               // [32][kLdH]
-  const unsigned long long stride = (unsigned long long)gridDim.x * kPredWaves * kPredRows;
-  // Software pipeline over the wave's iterations: the queue entries two
-  // iterations ahead and the ray records one iteration ahead are loaded
-  // while the current rows go through the MFMA layers (lanes 0..31 hold one
-  // entry each; rays with a single candidate domain this pass need no
-  // ordering and are skipped).
-  auto loadEntry = [&](unsigned long long b, PredIn& g) {
-    const unsigned long long q = b + (unsigned long long)(l & 31);
-    g.e = QEntry{0, 0.f};
-    g.dom = 0;
-    g.valid = l < kPredRows && q < n;
-    if (g.valid) {
-      g.e = P.q[0][q];
-      g.dom = (int)P.qdom[0][q];
+  const uint32_t re = c->r_end;
+  const uint32_t stride = gridDim.x * kInferWaves * 16u;
+  // pipeline: the group two iterations ahead loads its counts and nearest
+  // candidates, the next one its feature slots, while this one is inferred
+  auto stageA = [&](uint32_t base, InferGroup& G) {
+    G.r = base + (uint32_t)(l & 15);
+    G.rc = 0;
+    G.nr = make_uint4(0u, 0u, 0u, 0u);
+    if (G.r < re) {
+      G.rc = P.rcand[G.r];
+      if (G.rc >= 2) G.nr = P.rnear[G.r];
     }
   };
-  // the candidate count and the ray record are loaded together (no wait
-  // between them); the count is tested when the entry is used
-  auto loadRay = [&](PredIn& g) {
-    g.rc = 0;
-    g.op = g.ds = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (g.valid) {
-      g.rc = P.rcand[g.e.ray];
-      g.op = P.R.op[g.e.ray];
-      g.ds = P.R.ds[g.e.ray];
+  auto stageB = [&](InferGroup& G) {
+    float4 op = make_float4(0.f, 0.f, 0.f, 0.f), ds = op;
+    if (G.rc >= 2) {
+      op = P.R.op[G.r];
+      ds = P.R.ds[G.r];
     }
+    loadSlots(M, g, op, ds, (int)G.nr.x, G.x0);
+    loadSlots(M, g, op, ds, (int)G.nr.z, G.x1);
   };
-  unsigned long long base = (unsigned long long)(blockIdx.x * kPredWaves + w) * kPredRows;
-  PredIn cur, nxt, nx2;
-  loadEntry(base, cur);
-  loadRay(cur);
-  loadEntry(base + stride, nxt);
-  for (; base < n; base += stride) {
-    const bool use = cur.valid && cur.rc > 1;
-    const QEntry e = cur.e;
-    if (l < kPredRows) {
-      float x[kInPad];
-      if (use) {
-        float o[3] = {cur.op.x, cur.op.y, cur.op.z}, d[3] = {cur.ds.x, cur.ds.y, cur.ds.z};
-        features(M, o, d, cur.dom, x);
-      } else {
-#pragma unroll
-        for (int k = 0; k < kInPad; ++k) x[k] = 0.f;
+  uint32_t base = c->r_beg + (blockIdx.x * kInferWaves + w) * 16u;
+  InferGroup cur, nxt, nx2;
+  stageA(base, cur);
+  stageB(cur);
+  stageA(base + stride, nxt);
+  for (; base < re; base += stride) {
+    stageA(base + 2 * stride, nx2);
+    stageB(nxt);
+    const bool ask = cur.rc >= 2;
+    unsigned long long key = 0ull;
+    if (__ballot(ask) != 0ull) {
+      const float z0 = inferColumns(w1f, w2f, b1s, b2s, w3s, b3, toSlots(cur.x0));
+      const bool hit0 = ask && z0 > kHitLogit;
+      const bool ask1 = ask && !hit0 && cur.rc >= 3;
+      bool hit1 = false;
+      if (__ballot(ask1) != 0ull) {
+        const float z1 = inferColumns(w1f, w2f, b1s, b2s, w3s, b3, toSlots(cur.x1));
+        hit1 = ask1 && z1 > kHitLogit;
       }
-#pragma unroll
-      for (int k8 = 0; k8 < kInPad / 8; ++k8) {
-        bf16x8 v;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = tobf(x[8 * k8 + j]);
-        *(bf16x8*)(xs + l * kLdX + 8 * k8) = v;
-      }
+      // bit 32: a bound exists; below it the bound's entry t (ordered bits,
+      // complemented)
+      if (hit0 || hit1)
+        key = (1ull << 32) | (uint32_t)(~orderedF(__uint_as_float(hit0 ? cur.nr.y : cur.nr.w)));
     }
-    // issued after this iteration's inputs were consumed, so that waiting
-    // for those never waits for these; they land during the MFMA work
-    loadEntry(base + 2 * stride, nx2);
-    loadRay(nxt);
-    if (__ballot(use) == 0ull) {
-      cur = nxt;
-      nxt = nx2;
-      continue;
-    }
-    waveSync();
-    // layer 1 (K = 32): h1 = relu(x W1^T + b1) -> bf16 rows
-    f32x4 acc[2][8];
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb) acc[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    {
-      bf16x8 a0 = ldsA(xs, kLdX, 0), a1 = ldsA(xs + 16 * kLdX, kLdX, 0);
-      waveSync();  // the hidden rows below overwrite the feature rows
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb) {
-        bf16x8 bw = ldB(w1s, kLdX, nb, 0);
-        acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw, acc[0][nb], 0, 0, 0);
-        acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw, acc[1][nb], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb) {
-        int col = 16 * nb + (l & 15);
-        float bb = b1s[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          hs[(16 * rb + 4 * (l >> 4) + i) * kLdH + col] = tobf(fmaxf(acc[rb][nb][i] + bb, 0.f));
-      }
-    waveSync();
-    // layer 2 (K = 128)
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb) acc[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kb = 0; kb < kHid / 32; ++kb) {
-      bf16x8 a0 = ldsA(hs, kLdH, kb), a1 = ldsA(hs + 16 * kLdH, kLdH, kb);
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb) {
-        bf16x8 bw = ldB(w2s, kLdH, nb, kb);
-        acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bw, acc[0][nb], 0, 0, 0);
-        acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bw, acc[1][nb], 0, 0, 0);
-      }
-    }
-    // logits: relu(h2 + b2) . w3 + b3, reduced over the 16 lanes of a row group
-    float z[2][4];
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
-      float sum[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb) {
-        int col = 16 * nb + (l & 15);
-        float bb = b2s[col], ww = w3s[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) sum[i] += fmaxf(acc[rb][nb][i] + bb, 0.f) * ww;
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float v = sum[i];
-        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-        z[rb][i] = v + b3;
-      }
-    }
-    // entry r (lane r < 32): row block r / 16, row r % 16 held by lanes
-    // 16 * ((r % 16) / 4) .. at slot r % 4
-    const int r = l & 31, rb = r >> 4, rl = r & 15, src = 16 * (rl >> 2);
-    float zz[2][4];
-#pragma unroll
-    for (int b2 = 0; b2 < 2; ++b2)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) zz[b2][i] = __shfl(z[b2][i], src, 64);
-    float zr = 0.f;
-#pragma unroll
-    for (int b2 = 0; b2 < 2; ++b2)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (b2 == rb && i == (rl & 3)) zr = zz[b2][i];
-    if (use) {
-      // argmax over a ray's entries of (logit, -tdom): smaller tdom wins ties
-      unsigned long long key =
-          ((unsigned long long)orderedF(zr) << 32) | (uint32_t)(~orderedF(e.tdom));
-      atomicMax(&rkey[e.ray], key);
-    }
+    if (cur.r < re && l < 16) rkey[cur.r] = key;
     cur = nxt;
     nxt = nx2;
-    waveSync();
   }
-}
-
-__global__ void kResetKeys(TraceParams P, unsigned long long* rkey) {
-  const DevCounters* c = P.ctr;
-  if (!c->active) return;
-  for (uint32_t i = c->r_beg + blockIdx.x * blockDim.x + threadIdx.x; i < c->r_end;
-       i += gridDim.x * blockDim.x)
-    rkey[i] = 0ull;
 }
 
 // Splits the radiance queue into the phase-A list and the deferred list.
@@ -670,15 +654,17 @@ __global__ void __launch_bounds__(kMlpBlock) kLogits(MlpDev M, const float* samp
 struct Predictor {
   int nd = 0;
   int nparams = 0;
-  DevBuf p, m, v, g, w1, w2, w2t, w1et, loss, rkey;
+  DevBuf p, m, v, g, w1, w2, w2t, w1et, w1f, w2f, loss, rkey;
   DevBuf xt, h1t, dh2t, dh1t, index;
   int step = 0;
   bool trained = false;
   float center[3], inv_half[3];
   int tbatch = 0;
-  int pred_grid = 0;  // persistent inference grid
+  int pred_grid = 0;  // persistent inference grid (blocks)
   MlpDev dev() const {
     MlpDev M;
+    M.w1f = w1f.as<__bf16>();
+    M.w2f = w2f.as<__bf16>();
     M.w1 = w1.as<__bf16>();
     M.w2 = w2.as<__bf16>();
     M.w2t = w2t.as<__bf16>();
@@ -695,7 +681,7 @@ struct Predictor {
 void predictorRelease(Ctx* c) {
   if (!c->pred) return;
   Predictor* R = c->pred;
-  for (DevBuf* b : {&R->p, &R->m, &R->v, &R->g, &R->w1, &R->w2, &R->w2t, &R->w1et, &R->loss,
+  for (DevBuf* b : {&R->p, &R->m, &R->v, &R->g, &R->w1, &R->w2, &R->w2t, &R->w1et, &R->w1f, &R->w2f, &R->loss,
                     &R->rkey, &R->xt, &R->h1t, &R->dh2t, &R->dh1t, &R->index})
     b->release();
   delete R;
@@ -706,6 +692,8 @@ static void packWeights(Ctx* c) {
   Predictor* R = c->pred;
   kPackWeights<<<64, 256, 0, c->stream>>>(R->p.as<float>(), R->w1.as<__bf16>(), R->w2.as<__bf16>(),
                                           R->w2t.as<__bf16>(), R->w1et.as<__bf16>());
+  SPRAY_HIP(hipGetLastError());
+  kPackFrags<<<64, 256, 0, c->stream>>>(R->p.as<float>(), R->w1f.as<__bf16>(), R->w2f.as<__bf16>());
   SPRAY_HIP(hipGetLastError());
 }
 
@@ -745,6 +733,8 @@ static Predictor* ensurePredictor(Ctx* c) {
   R->w2.reserve(kHid * kHid * 2);
   R->w2t.reserve(kHid * kHid * 2);
   R->w1et.reserve(kEmb * kHid * 2);
+  R->w1f.reserve(8 * 64 * 8 * 2);
+  R->w2f.reserve(8 * 4 * 64 * 8 * 2);
   R->loss.reserve(64);
   SPRAY_HIP(hipMemcpy(R->p.p, h.data(), (size_t)R->nparams * 4, hipMemcpyHostToDevice));
   SPRAY_HIP(hipMemset(R->m.p, 0, (size_t)R->nparams * 4));
@@ -798,9 +788,7 @@ __global__ void kBatchIndex(uint32_t* idx, int batch, unsigned long long start,
 void predictorPhase(Ctx* c, const TraceParams& P) {
   Predictor* R = c->pred;  // rkey was sized by predictorPrepare (no allocation here: graph capture)
   hipStream_t s = c->stream;
-  kResetKeys<<<1024, 256, 0, s>>>(P, R->rkey.as<unsigned long long>());
-  kPredictKeys<<<R->pred_grid, 64 * kPredWaves, kPredLdsBytes, s>>>(
-      P, R->dev(), R->rkey.as<unsigned long long>());
+  kPredictRays<<<R->pred_grid, 64 * kInferWaves, 0, s>>>(P, R->dev(), R->rkey.as<unsigned long long>());
   kAssignPhase<<<1024, 256, 0, s>>>(P, R->rkey.as<unsigned long long>());
 }
 
@@ -812,9 +800,7 @@ void predictorPrepare(Ctx* c, size_t r_cap) {
   if (!c->pred->pred_grid) {
     hipDeviceProp_t prop;
     SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
-    SPRAY_HIP(hipFuncSetAttribute((const void*)kPredictKeys,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPredLdsBytes));
-    c->pred->pred_grid = prop.multiProcessorCount;  // one 12-wave block per CU
+    c->pred->pred_grid = prop.multiProcessorCount * 2;  // two 4-wave blocks per CU
   }
 }
 
@@ -979,6 +965,15 @@ int spray_predictor_load(spray_ctx* ctx, const char* path) {
 // origin, direction, domain, label) with host-provided parameters (flat fp32,
 // layout W1[128][22], b1, W2[128][128], b2, w3, b3, emb[nd][16]); returns
 // the summed BCE and the gradient of the mean BCE.
+// Host samples name a domain of the loaded scene (their embedding row).
+static void checkSampleDomains(const float* samples, int n, int nd) {
+  for (int i = 0; i < n; ++i) {
+    const float d = samples[(size_t)i * 8 + 6];
+    if (!(d >= 0.f && d < (float)nd) || d != (float)(int)d)
+      throw StatusError(SPRAY_ERR_INVALID, "sample " + std::to_string(i) + " names no domain of the scene");
+  }
+}
+
 int spray_mlp_grad(spray_ctx* ctx, const float* samples, int n, const float* params,
                    float* loss_out, float* grad_out) {
   if (!ctx || !samples || !params || !grad_out) return SPRAY_ERR_INVALID;
@@ -986,6 +981,7 @@ int spray_mlp_grad(spray_ctx* ctx, const float* samples, int n, const float* par
     Ctx* c = &ctx->c;
     if (!c->loaded) throw StatusError(SPRAY_ERR_STATE, "no scene loaded");
     if (n < 32 || n % 32) throw StatusError(SPRAY_ERR_INVALID, "n must be a positive multiple of 32");
+    checkSampleDomains(samples, n, c->pool.ndomains);
     Predictor* R = ensurePredictor(c);
     SPRAY_HIP(hipMemcpy(R->p.p, params, (size_t)R->nparams * 4, hipMemcpyHostToDevice));
     packWeights(c);
@@ -1005,12 +1001,50 @@ int spray_mlp_grad(spray_ctx* ctx, const float* samples, int n, const float* par
   });
 }
 
+// Logits of host samples through the inference kernel's arithmetic (the
+// register-resident fragments and the chained layers of kPredictRays), so
+// that the logits the learned schedule acts on are checked against the
+// float64 oracle.
+__global__ void __launch_bounds__(64) kLogitsInfer(MlpDev M, const float* samples, int n, float* out) {
+  __shared__ __attribute__((aligned(16))) float b1s[kHid], b2s[kHid], w3s[kHid];
+  for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+    b1s[i] = M.p[oB1 + i];
+    b2s[i] = M.p[oB2 + i];
+    w3s[i] = M.p[oW3 + i];
+  }
+  __syncthreads();
+  const int l = __lane_id(), g = l >> 4;
+  bf16x8 w1f[8], w2f[8][4];
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    w1f[mt] = *(const bf16x8*)(M.w1f + ((size_t)mt * 64 + l) * 8);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) w2f[mt][kb] = *(const bf16x8*)(M.w2f + (((size_t)mt * 4 + kb) * 64 + l) * 8);
+  }
+  for (int s0 = blockIdx.x * 16; s0 < n; s0 += gridDim.x * 16) {
+    const int sidx = s0 + (l & 15);
+    float4 op = make_float4(0.f, 0.f, 0.f, 0.f), ds = op;
+    int dom = 0;
+    if (sidx < n) {
+      const float* sm = samples + (size_t)sidx * 8;
+      op = make_float4(sm[0], sm[1], sm[2], 0.f);
+      ds = make_float4(sm[3], sm[4], sm[5], 0.f);
+      dom = (int)sm[6];
+    }
+    float x[8];
+    loadSlots(M, g, op, ds, dom, x);
+    const float z = inferColumns(w1f, w2f, b1s, b2s, w3s, M.p[oB3], toSlots(x));
+    if (l < 16 && sidx < n) out[sidx] = z;
+  }
+}
+
 int spray_mlp_logits(spray_ctx* ctx, const float* samples, int n, const float* params,
                      float* logits_out) {
   if (!ctx || !samples || !logits_out || n <= 0) return SPRAY_ERR_INVALID;
   return guard(ctx, [&] {
     Ctx* c = &ctx->c;
     if (!c->loaded) throw StatusError(SPRAY_ERR_STATE, "no scene loaded");
+    checkSampleDomains(samples, n, c->pool.ndomains);
     Predictor* R = ensurePredictor(c);
     if (params) {
       SPRAY_HIP(hipMemcpy(R->p.p, params, (size_t)R->nparams * 4, hipMemcpyHostToDevice));
@@ -1020,8 +1054,8 @@ int spray_mlp_logits(spray_ctx* ctx, const float* samples, int n, const float* p
     ds.reserve((size_t)n * 32);
     out.reserve((size_t)n * 4);
     SPRAY_HIP(hipMemcpy(ds.p, samples, (size_t)n * 32, hipMemcpyHostToDevice));
-    kLogits<<<(n + 63) / 64, kMlpBlock, 0, c->stream>>>(R->dev(), ds.as<float>(), nullptr, n,
-                                                        out.as<float>());
+    kLogitsInfer<<<std::min(1024, (n + 15) / 16), 64, 0, c->stream>>>(R->dev(), ds.as<float>(), n,
+                                                                     out.as<float>());
     SPRAY_HIP(hipStreamSynchronize(c->stream));
     SPRAY_HIP(hipMemcpy(logits_out, out.p, (size_t)n * 4, hipMemcpyDeviceToHost));
   });

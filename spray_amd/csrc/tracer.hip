@@ -458,7 +458,13 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
         if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
-      if (kind == 0) P.rcand[r] = (uint8_t)min(hits, 255);
+      if (kind == 0) {
+        P.rcand[r] = (uint8_t)min(hits, 255);
+        // the two nearest candidates, for the learned form's predictor
+        if (P.learned && kept >= 2)
+          P.rnear[r] = make_uint4((uint32_t)ti[0], __float_as_uint(tt[0]), (uint32_t)ti[1],
+                                  __float_as_uint(tt[1]));
+      }
     }
   }
   __syncthreads();
@@ -1292,7 +1298,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0;
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
-      wctr, score, order, q[2], qdom[2], rcand, dmask, dbox, ovf, vlog, log, sh_rec, qa,
+      wctr, score, order, q[2], qdom[2], rcand, rnear, dmask, dbox, ovf, vlog, log, sh_rec, qa,
       qb;
   unsigned long long log_cap = 0;
   bool logging = false;
@@ -1323,7 +1329,7 @@ struct TracerState {
     for (const DevBuf* x : {R, Pn, Sh})
       for (int k = 0; k < 4; ++k) b += x[k].bytes;
     for (const DevBuf* x : {&s_tmax, &s_occ, &vbuf, &image, &ctr, &tile_rec, &cnt, &off, &dbase, &bcnt,
-                            &bbase, &wctr, &score, &order, &q[0], &q[1], &qdom[0], &qdom[1], &rcand,
+                            &bbase, &wctr, &score, &order, &q[0], &q[1], &qdom[0], &qdom[1], &rcand, &rnear,
                             &dmask, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb})
       b += x->bytes;
     return b;
@@ -1387,7 +1393,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb})
+                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->rnear, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1503,6 +1509,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->s_occ, (size_t)T->s_cap * 4);
   need(T, T->vbuf, (size_t)S * kHistory * 8);
   need(T, T->rcand, (size_t)T->r_cap + 64);
+  if (cfg.schedule == SPRAY_SCHEDULE_LEARNED) need(T, T->rnear, (size_t)T->r_cap * 16 + 64);
   need(T, T->dmask, ((size_t)T->r_cap + T->s_cap) * ((nd + 63) / 64) * 8 + 64);
   need(T, T->dbox, dboxBytes(nd));
   need(T, T->ctr, sizeof(DevCounters));
@@ -1620,6 +1627,7 @@ static TraceParams makeParams(Ctx* c) {
   P.qdom[0] = T->qdom[0].as<uint16_t>();
   P.qdom[1] = T->qdom[1].as<uint16_t>();
   P.rcand = T->rcand.as<uint8_t>();
+  P.rnear = T->rnear.as<uint4>();
   P.dmask = T->dmask.as<unsigned long long>();
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];

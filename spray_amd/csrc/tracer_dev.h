@@ -90,6 +90,8 @@ struct TraceParams {
   QEntry* q[2];       // queues (radiance, shadow)
   uint16_t* qdom[2];  // domain of each queue entry
   uint8_t* rcand;     // radiance rays: queue entries of this pass (saturating)
+  uint4* rnear;       // learned form, radiance rays with >= 2 candidates: the two nearest
+                      // candidate domains (id, entry t bits, id, entry t bits)
   uint32_t rng[4];    // range launches: radiance base, count, shadow base, count
   unsigned long long* dmask;  // [pass ray][(nd + 63) / 64]: domains hit (kCount -> kScatter)
   uint32_t q_cap[2];

@@ -60,7 +60,7 @@ def test_autotune_cache(scene, tmp_path, monkeypatch):
         assert len(lines) == 1
         key, c2, t2, ms = lines[0].split()
         assert (int(c2), int(t2)) == (chunk, tile) and float(ms) > 0
-        assert "spp1" in key and "64x64" in key  # tuned at half the samples per pixel
+        assert "spp2" in key and "64x64" in key  # the key names the frame it tunes for
         img, st = ctx.trace_frame()  # the frame itself is rendered at full samples
         assert st["useful_rays"] == rst["useful_rays"]
     with _ctx(d) as ctx:  # a new context finds the key and does not tune again

@@ -110,10 +110,13 @@ def test_pool_regrow_matches_oracle(tmp_path):
     import spray_amd as sa
     d = str(tmp_path)
     sa.scenegen_write(d, "g", 2, 2, 2, 16, rgb=0xB3B3B3, extra_lines="light rect 0.5 2.4 0.5 1 0 0 0 0 1 4 4 4")
+    # 128x128x4 samples in 4 tiles: the pools' fixed slack of 4096 rays is
+    # small against them
+    kw = dict(PT, image_w=128, image_h=128, max_samples_per_rank=16384)
     ref = orc.Scene(os.path.join(d, "g.spray"), d, nthreads=4)
-    rimg, rst = ref.render(nthreads=4, cam_pos=CAM["pos"], cam_lookat=CAM["lookat"], cam_up=CAM["up"], **PT)
+    rimg, rst = ref.render(nthreads=4, cam_pos=CAM["pos"], cam_lookat=CAM["lookat"], cam_up=CAM["up"], **kw)
     for schedule in (0, 1):
-        with _ctx(schedule=schedule) as ctx:
+        with _ctx(schedule=schedule, **kw) as ctx:
             ctx.load_scene(os.path.join(d, "g.spray"), d)
             ctx.set_pool_headroom(1.0 / 64.0)
             img, st = ctx.trace_frame()
