@@ -145,6 +145,23 @@ int spray_trace_frame(spray_ctx* ctx, float* rgba_out, spray_stats* out);
 int spray_trace_rays(spray_ctx* ctx, const spray_rays_soa* rays, size_t n,
                      spray_hits_soa* hits, int mode, int domain);
 
+/* No counterpart in the reference: spray_trace_rays for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same query and results, on caller-owned
+ * DEVICE arrays: org[n*3], dir[n*3] (xyz per ray), tnear[n], tfar[n] in;
+ * prim[n] out, and t/u/v[n] out for SPRAY_QUERY_CLOSEST (may be NULL for
+ * SPRAY_QUERY_ANY).  It calls no hipMalloc/hipFree and no allocating
+ * library routine: the traversal's scratch (overflow stacks of at most
+ * 65,536 lanes plus an error word, 29.4 MB) is a static device array of the
+ * library, one per GPU, held until the process ends.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU race; the call
+ * returns after its work on the context's stream has finished.  At most
+ * 2^31 - 1 rays per call; the scene must be wholly resident (no partial
+ * domain cache). */
+int spray_trace_rays_device(spray_ctx* ctx, const float* org, const float* dir, const float* tnear,
+                            const float* tfar, size_t n, float* t, float* u, float* v, int* prim,
+                            int mode, int domain);
+
 /* Isector::isectDomains (ooc/ooc_isector.h:116-124 -> WbvhEmbree,
  * render/wbvh_embree.cc:126-148): per-ray domain list sorted by (t, id).
  * counts[n]; ids/ts[n*maxk] (entries beyond maxk are dropped). */

@@ -125,6 +125,7 @@ PROTOTYPES = {
     "spray_mlp_logits": (C.c_int, [_P, _FP, C.c_int, _FP, _FP]),
     "spray_write_ppm": (C.c_int, [_FP, C.c_int, C.c_int, C.c_char_p]),
     "spray_scenegen_write": (C.c_int, [C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_longlong)]),
+    "spray_trace_rays_device": (C.c_int, [_P, _FP, _FP, _FP, _FP, C.c_size_t, _FP, _FP, _FP, _IP, C.c_int, C.c_int]),
     "spray_scenegen_domain": (C.c_int, [C.POINTER(spray_scenegen_params), C.c_int, _FP, C.POINTER(C.c_uint),
                                         C.POINTER(C.c_uint), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "spray_scenegen_load": (C.c_int, [_P, C.POINTER(spray_scenegen_params), C.c_char_p, C.c_char_p]),

@@ -8,6 +8,8 @@
 // (render/trimesh_buffer.cc:267-326).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -1256,6 +1258,69 @@ void traceRaysDevice(Ctx* c, const float* d_org, const float* d_dir, const float
       c->pool.nodes.as<BNode4>() + D.node_off, c->pool.tris.as<LTri>() + D.tri_off, d_org, d_dir,
       d_tnear, d_tfar, (long long)n, mode, (uint32_t)D.num_nodes, d_t, d_u, d_v, d_prim,
       c->q_out.as<uint32_t>(), err);
+  int h = 0;
+  SPRAY_HIP(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, c->stream));
+  SPRAY_HIP(hipStreamSynchronize(c->stream));
+  if (h) throw StatusError(SPRAY_ERR_CAPACITY, "traversal stack overflow or corrupt BVH");
+}
+
+// ---------------------------------------------------------------------------
+// spray_trace_rays_device: the same query on caller-owned device arrays with
+// no allocation.  Its scratch (the traversal overflow stacks of at most
+// kScratchBlocks blocks, then one error word) is this file-scope array,
+// which every GPU the library runs on holds until the process ends.  Host
+// code never uses the array's name as an address: scratchInit looks up each
+// device's copy once (hipGetSymbolAddress) and keeps the pointer per device.
+constexpr int kScratchBlocks = 256;
+constexpr size_t kScratchStackBytes = (size_t)kScratchBlocks * kQBlock * kOverflow * 4;
+constexpr size_t kScratchBytes = kScratchStackBytes + 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+__global__ void kClearWord(int* w) { *w = 0; }
+
+void scratchInit(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    throw StatusError(SPRAY_ERR_DEVICE, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return;
+  int cur = 0;
+  SPRAY_HIP(hipGetDevice(&cur));
+  SPRAY_HIP(hipSetDevice(device));
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  (void)hipSetDevice(cur);
+  if (e != hipSuccess || ptr == nullptr)
+    throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = ptr;
+}
+
+size_t scratchBytes() { return kScratchBytes; }
+
+void traceRaysNoAlloc(Ctx* c, const float* d_org, const float* d_dir, const float* d_tnear,
+                      const float* d_tfar, size_t n, int mode, int domain, float* d_t, float* d_u,
+                      float* d_v, int* d_prim) {
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch[c->device] : nullptr;
+  }
+  if (!base) throw StatusError(SPRAY_ERR_STATE, "scratch not set up on this device");
+  const DomainDesc& D = c->pool.desc[domain];
+  const int grid =
+      (int)std::min<long long>(kScratchBlocks, std::max<long long>(1, ((long long)n + kQBlock - 1) / kQBlock));
+  uint32_t* ovf = (uint32_t*)base;
+  int* err = (int*)((char*)base + kScratchStackBytes);
+  kClearWord<<<1, 1, 0, c->stream>>>(err);
+  kTraceRays<<<grid, kQBlock, 0, c->stream>>>(
+      c->pool.nodes.as<BNode4>() + D.node_off, c->pool.tris.as<LTri>() + D.tri_off, d_org, d_dir,
+      d_tnear, d_tfar, (long long)n, mode, (uint32_t)D.num_nodes, d_t, d_u, d_v, d_prim, ovf, err);
+  SPRAY_HIP(hipGetLastError());
   int h = 0;
   SPRAY_HIP(hipMemcpyAsync(&h, err, 4, hipMemcpyDeviceToHost, c->stream));
   SPRAY_HIP(hipStreamSynchronize(c->stream));

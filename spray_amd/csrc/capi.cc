@@ -16,6 +16,7 @@ using namespace spray;
 namespace {
 
 thread_local std::string g_create_err;
+constexpr long long kTraceRaysDeviceMax = 2147483647LL;
 
 template <typename F>
 int guarded(spray_ctx* ctx, F&& f) {
@@ -156,6 +157,7 @@ int spray_create(const spray_config* cfg, spray_ctx** out) {
     ctx->c.device = cfg->device;
     SPRAY_HIP(hipSetDevice(cfg->device));
     SPRAY_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    scratchInit(cfg->device);  // this device's copy of the no-allocation entry point's scratch
   });
   if (rc != SPRAY_OK) {
     g_create_err = ctx->c.err;
@@ -369,6 +371,28 @@ int spray_trace_rays(spray_ctx* ctx, const spray_rays_soa* rays, size_t n, spray
       SPRAY_HIP(hipMemcpy(hits->v, d_v, n * 4, hipMemcpyDeviceToHost));
     }
     SPRAY_HIP(hipMemcpy(hits->prim, d_p, n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int spray_trace_rays_device(spray_ctx* ctx, const float* org, const float* dir, const float* tnear,
+                            const float* tfar, size_t n, float* t, float* u, float* v, int* prim,
+                            int mode, int domain) {
+  if (!ctx) return SPRAY_ERR_INVALID;
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    needScene(c);
+    if (domain < 0 || domain >= c->pool.ndomains) throw StatusError(SPRAY_ERR_INVALID, "bad domain id");
+    if (mode != SPRAY_QUERY_CLOSEST && mode != SPRAY_QUERY_ANY)
+      throw StatusError(SPRAY_ERR_INVALID, "bad query mode");
+    if (c->pool.paged)
+      throw StatusError(SPRAY_ERR_STATE, "spray_trace_rays_device needs the whole scene resident");
+    if (n > (size_t)kTraceRaysDeviceMax)
+      throw StatusError(SPRAY_ERR_CAPACITY, "spray_trace_rays_device takes at most 2^31 - 1 rays per call");
+    if (n == 0) return;
+    if (!org || !dir || !tnear || !tfar || !prim || (mode == SPRAY_QUERY_CLOSEST && (!t || !u || !v)))
+      throw StatusError(SPRAY_ERR_INVALID, "missing device arrays");
+    SPRAY_HIP(hipSetDevice(c->device));
+    traceRaysNoAlloc(c, org, dir, tnear, tfar, n, mode, domain, t, u, v, prim);
   });
 }
 

@@ -185,6 +185,13 @@ void poolAcquire(Ctx* c, int d, hipStream_t s);
 void poolPrefetch(Ctx* c, int d, int keep);
 void poolRelease(Ctx* c, int d, hipStream_t s);
 void poolInitPaging(Ctx* c, const std::vector<std::pair<int, int>>& resident);
+// spray_trace_rays_device (bvh.hip): scratch set-up per device, and the
+// query on caller-owned device arrays using that scratch
+void scratchInit(int device);
+size_t scratchBytes();
+void traceRaysNoAlloc(Ctx* c, const float* d_org, const float* d_dir, const float* d_tnear,
+                      const float* d_tfar, size_t n, int mode, int domain, float* d_t, float* d_u,
+                      float* d_v, int* d_prim);
 void traceRaysDevice(Ctx* c, const float* d_org, const float* d_dir, const float* d_tnear,
                      const float* d_tfar, size_t n, int mode, int domain, float* d_t, float* d_u,
                      float* d_v, int* d_prim);
