@@ -767,10 +767,11 @@ static int gridFor(long long n, int block) {
 constexpr int kLevelBatch = 16;
 constexpr int kMaxLevels = 48;
 constexpr int kLevelWords = 2 + kMaxLevels + 1;
+constexpr int kBuildStreams = 4;  // domain pipelines of a whole-scene LBVH build
 
 struct BuildScratch {
   DevBuf keys, vals, keys2, vals2, hist, lb, ib, lft, rgt, fst, lst, pin, plf, fa, fb, cnts,
-      scal, adj, rpre, rsuf, rtab, lvl;
+      scal, adj, rpre, rsuf, rtab, lvl, sums;
   DevBuf pc, pc2, pb, pb2, pnn, pcnt, pmf, pkf, pmp, pkp, pperm, pord;  // PLOC
   void reserve(int maxf, long long maxv, bool ploc) {
     const size_t f4 = (size_t)maxf * 4 + 64;
@@ -784,6 +785,10 @@ struct BuildScratch {
     cnts.reserve((size_t)(maxv + 1) * 12 + 64);
     scal.reserve(64);
     lvl.reserve(kLevelWords * 4 + 64);
+    // tile sums of this set's scans (vertex offsets, radix histograms, PLOC flags)
+    const long long scan_n = std::max<long long>(
+        std::max<long long>(maxv + 1, (long long)maxf + 1), 256ll * ((maxf + kRsTile - 1) / kRsTile));
+    sums.reserve((size_t)scanSumsWords(scan_n) * 4 + 64);
     adj.reserve((size_t)maxf * 12 + 64);
     rpre.reserve((size_t)maxf * 24 + 64);
     rsuf.reserve((size_t)maxf * 24 + 64);
@@ -798,7 +803,7 @@ struct BuildScratch {
   }
   ~BuildScratch() {
     for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
-                      &plf, &fa, &fb, &cnts, &scal, &adj, &rpre, &rsuf, &rtab, &lvl, &pc, &pc2,
+                      &plf, &fa, &fb, &cnts, &scal, &adj, &rpre, &rsuf, &rtab, &lvl, &sums, &pc, &pc2,
                       &pb, &pb2, &pnn, &pcnt,
                       &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
       b->release();
@@ -899,29 +904,28 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     }
     src.check(c);
   }
-  // A whole-scene LBVH build collapses each domain's tree on a second stream
-  // (cstream) while the next domain builds, with two scratch sets; PLOC and
-  // paged builds stay in one stream with a host check per domain.
-  const bool deferred = !paged && c->bvh_builder == 0 && nd > 1;
-  hipStream_t cstream = nullptr;
-  hipEvent_t pre_done[2] = {nullptr, nullptr}, coll_done[2] = {nullptr, nullptr};
+  // A whole-scene LBVH build runs kBuildStreams domain pipelines side by side
+  // (domain d on stream d % kBuildStreams, each stream with its own scratch
+  // set), so the many small per-domain kernels of different domains overlap;
+  // each domain's collapse launches a fixed number of levels with no host
+  // wait, and the level counts of every domain are checked once after the
+  // last.  PLOC and paged builds stay in one stream with a host check per
+  // domain.
+  const bool multi = !paged && c->bvh_builder == 0 && nd > 1;
   DevBuf lvl_all;
-  struct Release {  // stream, events and level counts of the deferred collapse
-    hipStream_t& st;
-    hipEvent_t* a;
-    hipEvent_t* b;
+  std::vector<int> launched(nd, 0);  // collapse levels launched per domain (multi)
+  // a remesh of the same domain structure launches the previous depth + 4
+  // levels (a domain that needs more is rebuilt with host-checked levels)
+  std::vector<int> prev_depth;
+  if (keep_buffers && (int)P.bvh_depth.size() == nd) prev_depth = P.bvh_depth;
+  struct ReleaseLvl {
     DevBuf& lvl;
-    ~Release() {
-      if (st) (void)hipStreamDestroy(st);
-      for (int k = 0; k < 2; ++k) {
-        if (a[k]) (void)hipEventDestroy(a[k]);
-        if (b[k]) (void)hipEventDestroy(b[k]);
-      }
-      lvl.release();
-    }
-  } release_deferred{cstream, pre_done, coll_done, lvl_all};
+    ~ReleaseLvl() { lvl.release(); }
+  } release_lvl{lvl_all};
   // one domain's normals, shading records and BVH, issued on stream s
-  auto buildOne = [&](BuildScratch& S, int d, hipStream_t s) {
+  // levels > 0: collapse with that many level launches and no host wait
+  // (counts checked later); 0: host-checked collapse
+  auto buildOne = [&](BuildScratch& S, int d, hipStream_t s, int levels) {
     DomainDesc& D = P.desc[d];
     const int nf = D.num_tris;
     const long long nv = src.numVerts(d);
@@ -940,7 +944,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     uint32_t* off = cur + (nv + 1);
     SPRAY_HIP(hipMemsetAsync(cnt, 0, (size_t)(nv + 1) * 8, s));
     kVertCount<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, cnt);
-    exclusiveScan(cnt, off, nv + 1, s);
+    exclusiveScan(cnt, off, nv + 1, s, S.sums.as<uint32_t>());
     kVertFill<<<gridFor((long long)nf * 3, 256), 256, 0, s>>>(face, nf, off, cur,
                                                               S.adj.as<uint32_t>());
     kVertNormal<<<gridFor(nv, 256), 256, 0, s>>>(vtx, face, nv, off, cnt, S.adj.as<uint32_t>(),
@@ -961,7 +965,8 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
              *vb = S.vals2.as<uint32_t>();
     for (int pass = 0; pass < 4; ++pass) {
       kRsHist<<<nb, kRsBlock, 0, s>>>(ka, nf, pass * 8, S.hist.as<uint32_t>());
-      exclusiveScan(S.hist.as<uint32_t>(), S.hist.as<uint32_t>(), (long long)256 * nb, s);
+      exclusiveScan(S.hist.as<uint32_t>(), S.hist.as<uint32_t>(), (long long)256 * nb, s,
+                    S.sums.as<uint32_t>());
       kRsScatter<<<nb, kRsBlock, 0, s>>>(ka, va, kb, vb, nf, pass * 8, S.hist.as<uint32_t>());
       std::swap(ka, kb);
       std::swap(va, vb);
@@ -1056,8 +1061,8 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       while (n > 1) {
         kPlocNearest<<<(n + kPlocBlock - 1) / kPlocBlock, kPlocBlock, 0, s>>>(cbox, n, nn);
         kPlocFlags<<<gridFor(n + 1, 256), 256, 0, s>>>(nn, n, mflag, kflag);
-        exclusiveScan(mflag, mpos, n + 1, s);
-        exclusiveScan(kflag, kpos, n + 1, s);
+        exclusiveScan(mflag, mpos, n + 1, s, S.sums.as<uint32_t>());
+        exclusiveScan(kflag, kpos, n + 1, s, S.sums.as<uint32_t>());
         kPlocMerge<<<gridFor(n, 256), 256, 0, s>>>(nn, n, cid, cbox, mpos, kpos, made, cid2, cbox2,
                                                     L, R, inner, icount);
         uint32_t h[2];
@@ -1117,22 +1122,17 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       }
       kRangeBoxes<<<gridFor(nf - 1, 256), 256, 0, s>>>(nf - 1, F, Lst, leafbox, S.rpre.as<float>(),
                                                        S.rsuf.as<float>(), tab, RL, inner);
-      if (deferred) {
-        // the collapse runs on the collapse stream while this stream builds
-        // the next domain; its level counts are checked after the last domain
-        const int slot = d & 1;
-        SPRAY_HIP(hipEventRecord(pre_done[slot], s));
-        SPRAY_HIP(hipStreamWaitEvent(cstream, pre_done[slot], 0));
+      if (levels > 0) {
         int* cc = lvl_all.as<int>() + (size_t)d * kLevelWords;
-        kCollapseInit<<<1, 64, 0, cstream>>>(cc, kLevelWords, S.fa.as<FrontierItem>(), 0);
+        kCollapseInit<<<1, 64, 0, s>>>(cc, kLevelWords, S.fa.as<FrontierItem>(), 0);
         const int grid = std::min(gridFor(std::max(1, nf / 8), 256), 2048);
-        for (int l = 0; l < kMaxLevels; ++l) {
+        for (int l = 0; l < levels; ++l) {
           FrontierItem* fin = (l & 1) ? S.fb.as<FrontierItem>() : S.fa.as<FrontierItem>();
           FrontierItem* fout = (l & 1) ? S.fa.as<FrontierItem>() : S.fb.as<FrontierItem>();
-          kCollapse<<<grid, 256, 0, cstream>>>(fin, cc + 2 + l, fout, cc + 3 + l, cc, node_cap, L, R,
-                                              F, Lst, leafbox, inner, nodes, cc + 1);
+          kCollapse<<<grid, 256, 0, s>>>(fin, cc + 2 + l, fout, cc + 3 + l, cc, node_cap, L, R, F, Lst,
+                                         leafbox, inner, nodes, cc + 1);
         }
-        SPRAY_HIP(hipEventRecord(coll_done[slot], cstream));
+        launched[d] = levels;
         return;
       }
       depth = collapse(0);
@@ -1143,45 +1143,57 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     pageOut();
   };
   P.bvh_depth.assign(nd, 0);
-  if (!deferred) {
+  if (!multi) {
     BuildScratch S;
     S.reserve(maxf, maxv, c->bvh_builder == 1);
-    for (int d = 0; d < nd; ++d) buildOne(S, d, s);
+    for (int d = 0; d < nd; ++d) buildOne(S, d, s, 0);
   } else {
-    // two scratch sets: domain d + 1 builds while domain d collapses
-    BuildScratch S[2];
+    const int K = std::min(kBuildStreams, nd);
+    std::vector<BuildScratch> S(K);
     for (auto& x : S) x.reserve(maxf, maxv, false);
     lvl_all.reserve((size_t)nd * kLevelWords * 4 + 64);
-    SPRAY_HIP(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
-    for (int k = 0; k < 2; ++k) {
-      SPRAY_HIP(hipEventCreateWithFlags(&pre_done[k], hipEventDisableTiming));
-      SPRAY_HIP(hipEventCreateWithFlags(&coll_done[k], hipEventDisableTiming));
-    }
-    struct Drain {  // the collapse stream finishes before the scratch sets go
-      hipStream_t st;
-      ~Drain() {
-        if (st) (void)hipStreamSynchronize(st);
+    struct Pipes {  // the pipeline streams finish before the scratch sets go
+      std::vector<hipStream_t> st;
+      hipEvent_t ready = nullptr;
+      ~Pipes() {
+        for (hipStream_t x : st) {
+          (void)hipStreamSynchronize(x);
+          (void)hipStreamDestroy(x);
+        }
+        if (ready) (void)hipEventDestroy(ready);
       }
-    } drain{cstream};
+    } pipes;
+    pipes.st.assign(K, nullptr);
+    for (auto& x : pipes.st) SPRAY_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    SPRAY_HIP(hipEventCreateWithFlags(&pipes.ready, hipEventDisableTiming));
+    SPRAY_HIP(hipEventRecord(pipes.ready, s));  // the meshes are in place
+    for (hipStream_t x : pipes.st) SPRAY_HIP(hipStreamWaitEvent(x, pipes.ready, 0));
     for (int d = 0; d < nd; ++d) {
-      if (d >= 2) SPRAY_HIP(hipStreamWaitEvent(s, coll_done[d & 1], 0));
-      buildOne(S[d & 1], d, s);
+      const int lv = prev_depth.empty() ? kMaxLevels : std::min(kMaxLevels, prev_depth[d] + 4);
+      buildOne(S[d % K], d, pipes.st[d % K], std::max(1, lv));
     }
-    SPRAY_HIP(hipStreamSynchronize(cstream));
-    SPRAY_HIP(hipStreamSynchronize(s));
+    for (hipStream_t x : pipes.st) SPRAY_HIP(hipStreamSynchronize(x));
     std::vector<int> h((size_t)nd * kLevelWords);
     SPRAY_HIP(hipMemcpy(h.data(), lvl_all.p, h.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<int> redo;
     for (int d = 0; d < nd; ++d) {
       if (P.desc[d].num_tris <= kLeafMax) continue;  // one-leaf root, built directly
       const int* cc = &h[(size_t)d * kLevelWords];
       if (cc[1]) throw StatusError(SPRAY_ERR_CAPACITY, "BVH node capacity exceeded");
+      if (cc[2 + launched[d]] > 0) {  // items left after the last launched level
+        if (launched[d] >= kMaxLevels)
+          throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
+        redo.push_back(d);
+        continue;
+      }
       int depth = 0;
-      while (depth < kMaxLevels && cc[2 + depth] > 0) ++depth;
-      if (cc[2 + kMaxLevels] > 0 || 3 * depth + 1 > kRing + kOverflow)
+      while (depth < launched[d] && cc[2 + depth] > 0) ++depth;
+      if (3 * depth + 1 > kRing + kOverflow)
         throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
       P.desc[d].num_nodes = cc[0];
       P.bvh_depth[d] = depth;
     }
+    for (int d : redo) buildOne(S[0], d, s, 0);  // deeper than last time: host-checked levels
   }
   SPRAY_HIP(hipStreamSynchronize(s));
   long long used = 0;

@@ -103,17 +103,11 @@ __global__ void kScanAdd(uint32_t* out, long long n, const uint32_t* sums) {
   }
 }
 
-void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s) {
+long long scanSumsWords(long long n) { return (n + kScanTile - 1) / kScanTile + 16; }
+
+void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s, uint32_t* tile_sums) {
   if (n <= 0) return;
   long long nb = (n + kScanTile - 1) / kScanTile;
-  // per host thread (build workers scan concurrently); freed when the thread ends
-  struct Sums {
-    DevBuf b;
-    ~Sums() { b.release(); }
-  };
-  static thread_local Sums sums;
-  sums.b.reserve(std::max<size_t>((size_t)nb * 4 + 64, (size_t)4 << 20));
-  uint32_t* tile_sums = sums.b.as<uint32_t>();
   kScanTiles<<<(unsigned)nb, kScanBlock, 0, s>>>(in, out, n, tile_sums);
   if (nb > 1 && nb <= kScanFuseTiles) {
     kScanAddSum<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, tile_sums);
@@ -122,6 +116,19 @@ void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s
     kScanAdd<<<(unsigned)nb, kScanBlock, 0, s>>>(out, n, tile_sums);
   }
   SPRAY_HIP(hipGetLastError());
+}
+
+void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  long long nb = (n + kScanTile - 1) / kScanTile;
+  // per host thread; freed when the thread ends
+  struct Sums {
+    DevBuf b;
+    ~Sums() { b.release(); }
+  };
+  static thread_local Sums sums;
+  sums.b.reserve(std::max<size_t>((size_t)nb * 4 + 64, (size_t)4 << 20));
+  exclusiveScan(in, out, n, s, sums.b.as<uint32_t>());
 }
 
 }  // namespace spray

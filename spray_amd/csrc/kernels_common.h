@@ -27,7 +27,12 @@ __device__ __forceinline__ unsigned long long waveSumULL(unsigned long long v) {
   return v;
 }
 
-// out[i] = sum(in[0..i-1]) for n uint32 values; in == out is allowed.
+// out[i] = sum(in[0..i-1]) for n uint32 values; in == out is allowed.  The
+// first form keeps its tile sums in a buffer of the calling host thread; the
+// second takes them from the caller (scanSumsWords(n) words), so scans on
+// several streams of one thread do not share it.
 void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s);
+void exclusiveScan(const uint32_t* in, uint32_t* out, long long n, hipStream_t s, uint32_t* tile_sums);
+long long scanSumsWords(long long n);
 
 }  // namespace spray
