@@ -22,32 +22,62 @@ def _rays(rng, n):
     return org, dirs, tnear, tfar
 
 
-def _device_query(ctx, torch, org, dirs, tnear, tfar, mode, dom):
-    dev = torch.device("cuda:0")
+class _Hip:
+    """Device arrays from the HIP runtime the library itself links
+    (libamdhip64.so.7, already loaded with it), not from a second runtime."""
+
+    def __init__(self):
+        self.rt = C.CDLL("libamdhip64.so.7")
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.bufs = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.rt.hipMalloc(C.byref(p), max(a.nbytes, 4)) == 0
+        self.bufs.append(p)
+        assert self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return p
+
+    def get(self, p, like):
+        out = np.empty_like(like)
+        assert self.rt.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        for p in self.bufs:
+            self.rt.hipFree(p)
+        self.bufs = []
+
+
+def _device_query(ctx, org, dirs, tnear, tfar, mode, dom):
+    hip = _Hip()
     n = len(org)
-    t_org = torch.from_numpy(org).to(dev).contiguous()
-    t_dir = torch.from_numpy(dirs).to(dev).contiguous()
-    t_tn = torch.from_numpy(tnear).to(dev)
-    t_tf = torch.from_numpy(tfar).to(dev)
-    # outputs start as garbage so that nothing unwritten can pass
-    t_t = torch.full((n,), -7.0, device=dev)
-    t_u = torch.full((n,), -7.0, device=dev)
-    t_v = torch.full((n,), -7.0, device=dev)
-    t_p = torch.full((n,), -7, dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
-    fp = lambda x: C.cast(C.c_void_p(x.data_ptr()), C.POINTER(C.c_float))
-    ip = lambda x: C.cast(C.c_void_p(x.data_ptr()), C.POINTER(C.c_int))
-    closest = mode == 0
-    rc = ctx.lib.spray_trace_rays_device(ctx.h, fp(t_org), fp(t_dir), fp(t_tn), fp(t_tf), n,
-                                         fp(t_t) if closest else None, fp(t_u) if closest else None,
-                                         fp(t_v) if closest else None, ip(t_p), mode, dom)
-    assert rc == 0, ctx.lib.spray_last_error(ctx.h)
-    torch.cuda.synchronize()
-    return t_t.cpu().numpy(), t_u.cpu().numpy(), t_v.cpu().numpy(), t_p.cpu().numpy()
+    try:
+        d_org, d_dir, d_tn, d_tf = (hip.put(x) for x in (org, dirs, tnear, tfar))
+        # outputs start as garbage so that nothing unwritten can pass
+        junk_f = np.full(n, -7.0, np.float32)
+        junk_i = np.full(n, -7, np.int32)
+        d_t, d_u, d_v = (hip.put(junk_f) for _ in range(3))
+        d_p = hip.put(junk_i)
+        fp = lambda x: C.cast(x, C.POINTER(C.c_float))
+        ip = lambda x: C.cast(x, C.POINTER(C.c_int))
+        closest = mode == 0
+        rc = ctx.lib.spray_trace_rays_device(ctx.h, fp(d_org), fp(d_dir), fp(d_tn), fp(d_tf), n,
+                                             fp(d_t) if closest else None, fp(d_u) if closest else None,
+                                             fp(d_v) if closest else None, ip(d_p), mode, dom)
+        assert rc == 0, ctx.lib.spray_last_error(ctx.h)
+        # the entry point has waited for its stream (it reads the error word)
+        assert hip.rt.hipDeviceSynchronize() == 0
+        return (hip.get(d_t, junk_f), hip.get(d_u, junk_f), hip.get(d_v, junk_f),
+                hip.get(d_p, junk_i))
+    finally:
+        hip.free()
 
 
 def test_no_alloc_matches_trace_rays(tmp_path):
-    import torch
     import spray_amd as sa
     d = str(tmp_path)
     sa.scenegen_write(d, "s", 2, 2, 2, 24, rgb=0xFFFFFF)
@@ -61,7 +91,7 @@ def test_no_alloc_matches_trace_rays(tmp_path):
         for n, dom in ((200_003, 3), (1_537, 5)):
             org, dirs, tnear, tfar = _rays(rng, n)
             for mode in (0, 1):
-                t, u, v, p = _device_query(ctx, torch, org, dirs, tnear, tfar, mode, dom)
+                t, u, v, p = _device_query(ctx, org, dirs, tnear, tfar, mode, dom)
                 if mode == 0:
                     rt, ru, rv, rp = ctx.trace_rays(org, dirs, dom, tnear=tnear, tfar=tfar)
                     print(f"n {n} domain {dom}: hits {int((rp >= 0).sum())}")
