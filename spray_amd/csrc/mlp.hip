@@ -173,16 +173,24 @@ __device__ __forceinline__ uint32_t orderedF(float f) {
 // (logit > kHitLogit) bounds phase A: the ray's entries entering no later
 // than it.  A ray with no such prediction keeps every entry in phase A.
 //
+// kCount appends one ask record per asking ray, so the kernel streams a
+// dense list: every MFMA column is a real ask, and each wave keeps
+// kAskStages tiles of 16 records in flight (independent, coalesced loads).
+// The domain embeddings sit in LDS as bf16 layer-1 slots; second asks wait
+// in a per-wave LDS ring until 16 of them fill a tile.
+//
 // The weights stay in registers for the whole persistent launch, laid out
-// as v_mfma_f32_16x16x32_bf16 A operands (hidden units as rows); the rays
+// as v_mfma_f32_16x16x32_bf16 A operands (hidden units as rows); the asks
 // are the B operand's 16 columns.  Layer 1's accumulators hold hidden
-// units 16 mt + 4 (l / 16) + i of ray l % 16 in lane l, and layer 2 takes
+// units 16 mt + 4 (l / 16) + i of column l % 16 in lane l, and layer 2 takes
 // them as its B operand in place: its K-block kb reads slot j of lane group
 // g from unit 16 (2 kb + j / 4) + 4 g + j % 4, and the layer-2 weight
 // fragments are packed in that unit order (kPackFrags).  No LDS traffic
-// per MFMA; LDS holds only the biases and the output weights.
+// per MFMA.
 constexpr float kHitLogit = 0.0f;
 constexpr int kInferWaves = 4;  // waves per block
+constexpr int kAskStages = 4;   // ask tiles in flight per wave
+constexpr int kEmbLdsMax = 1024;  // domains whose embedding slots fit the LDS copy (48 B each)
 
 __device__ __forceinline__ int l2Unit(int kb, int g, int j) {
   return 16 * (2 * kb + (j >> 2)) + 4 * g + (j & 3);
@@ -203,38 +211,55 @@ __global__ void kPackFrags(const float* p, __bf16* w1f, __bf16* w2f) {
   }
 }
 
-// Layer-1 input slots 8 g .. 8 g + 7 of one (ray, domain) sample: the
-// normalised origin and the direction (slots 0..5), the domain embedding
-// (6..21), zeros (22..31).  Gathered as floats ahead of their use.
-__device__ __forceinline__ void loadSlots(const MlpDev& M, int g, float4 op, float4 ds, int dom,
-                                          float x[8]) {
-  const float* e = M.p + oEmb + (size_t)dom * kEmb;
+// Embedding part of the layer-1 input slots of domain d for lane groups
+// g = 0..2 (group 3 holds only padding): slots 6, 7 of group 0 are e0, e1;
+// group 1 holds e2..e9; group 2 e10..e15 and two zeros.
+__device__ __forceinline__ bf16x8 embSlotsOf(const float* e, int g) {
+  bf16x8 b;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) b[j] = tobf(0.f);
   if (g == 0) {
-    x[0] = (op.x - M.center[0]) * M.inv_half[0];
-    x[1] = (op.y - M.center[1]) * M.inv_half[1];
-    x[2] = (op.z - M.center[2]) * M.inv_half[2];
-    x[3] = ds.x;
-    x[4] = ds.y;
-    x[5] = ds.z;
-    x[6] = e[0];
-    x[7] = e[1];
+    b[6] = tobf(e[0]);
+    b[7] = tobf(e[1]);
   } else if (g == 1) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = e[2 + j];
+    for (int j = 0; j < 8; ++j) b[j] = tobf(e[2 + j]);
   } else if (g == 2) {
 #pragma unroll
-    for (int j = 0; j < 6; ++j) x[j] = e[10 + j];
-    x[6] = x[7] = 0.f;
-  } else {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = 0.f;
+    for (int j = 0; j < 6; ++j) b[j] = tobf(e[10 + j]);
+  }
+  return b;
+}
+
+// The LDS table of embedding slots, [domain][group 0..2].
+__device__ __forceinline__ void fillEmbLds(const MlpDev& M, int nd, bf16x8* embl) {
+  for (int i = threadIdx.x; i < nd * 3; i += blockDim.x) {
+    const int d = i / 3, g = i - 3 * d;
+    embl[i] = embSlotsOf(M.p + oEmb + (size_t)d * kEmb, g);
   }
 }
 
-__device__ __forceinline__ bf16x8 toSlots(const float x[8]) {
+// Layer-1 input slots 8 g .. 8 g + 7 of one (ray, domain) sample in bf16:
+// the normalised origin and the direction (slots 0..5), the domain
+// embedding (6..21), zeros (22..31).  The embedding comes from the LDS
+// table when there is one, else from the fp32 master weights.
+__device__ __forceinline__ bf16x8 askSlots(const MlpDev& M, const bf16x8* embl, int g, float ox,
+                                           float oy, float oz, float dx, float dy, float dz, int dom) {
   bf16x8 b;
+  if (g == 3) {
 #pragma unroll
-  for (int j = 0; j < 8; ++j) b[j] = tobf(x[j]);
+    for (int j = 0; j < 8; ++j) b[j] = tobf(0.f);
+    return b;
+  }
+  b = embl ? embl[dom * 3 + g] : embSlotsOf(M.p + oEmb + (size_t)dom * kEmb, g);
+  if (g == 0) {
+    b[0] = tobf((ox - M.center[0]) * M.inv_half[0]);
+    b[1] = tobf((oy - M.center[1]) * M.inv_half[1]);
+    b[2] = tobf((oz - M.center[2]) * M.inv_half[2]);
+    b[3] = tobf(dx);
+    b[4] = tobf(dy);
+    b[5] = tobf(dz);
+  }
   return b;
 }
 
@@ -276,18 +301,13 @@ __device__ __forceinline__ float inferColumns(const bf16x8 (&w1f)[8], const bf16
   return z + b3;
 }
 
-// One group of 16 rays as it moves through the software pipeline: the
-// candidate count and nearest candidates first, then the feature slots.
-struct InferGroup {
-  uint32_t r;
-  int rc;
-  uint4 nr;
-  float x0[8], x1[8];
-};
+__device__ __forceinline__ uint32_t boundKey(float t) { return orderedF(t); }
 
 __global__ void __launch_bounds__(64 * kInferWaves, 1) kPredictRays(TraceParams P, MlpDev M,
-                                                                   unsigned long long* rkey) {
+                                                                   int emb_lds) {
+  extern __shared__ bf16x8 embs[];  // [nd][3] embedding slots (when emb_lds)
   __shared__ __attribute__((aligned(16))) float b1s[kHid], b2s[kHid], w3s[kHid];
+  __shared__ float4 ring[kInferWaves][32 * 3];  // second asks of each wave
   const DevCounters* c = P.ctr;
   if (!c->active) return;
   for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
@@ -295,8 +315,10 @@ __global__ void __launch_bounds__(64 * kInferWaves, 1) kPredictRays(TraceParams 
     b2s[i] = M.p[oB2 + i];
     w3s[i] = M.p[oW3 + i];
   }
+  const bf16x8* embl = emb_lds ? embs : nullptr;
+  if (emb_lds) fillEmbLds(M, P.ndomains, embs);
   const float b3 = M.p[oB3];
-  const int l = __lane_id(), g = l >> 4, w = threadIdx.x >> 6;
+  const int l = __lane_id(), g = l >> 4, col = l & 15, w = threadIdx.x >> 6;
   bf16x8 w1f[8], w2f[8][4];
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
@@ -305,121 +327,205 @@ __global__ void __launch_bounds__(64 * kInferWaves, 1) kPredictRays(TraceParams 
     for (int kb = 0; kb < 4; ++kb) w2f[mt][kb] = *(const bf16x8*)(M.w2f + (((size_t)mt * 4 + kb) * 64 + l) * 8);
   }
   __syncthreads();
-  const uint32_t re = c->r_end;
-  const uint32_t stride = gridDim.x * kInferWaves * 16u;
-  // pipeline: the group two iterations ahead loads its counts and nearest
-  // candidates, the next one its feature slots, while this one is inferred
-  auto stageA = [&](uint32_t base, InferGroup& G) {
-    G.r = base + (uint32_t)(l & 15);
-    G.rc = 0;
-    G.nr = make_uint4(0u, 0u, 0u, 0u);
-    if (G.r < re) {
-      G.rc = P.rcand[G.r];
-      if (G.rc >= 2) G.nr = P.rnear[G.r];
-    }
+  const uint32_t reg = P.ask_region;
+  if (reg == 0u) return;
+  const uint32_t nwv = gridDim.x * kInferWaves;
+  const uint32_t first = blockIdx.x * kInferWaves + (uint32_t)w;
+  float4* q = ring[w];
+  uint32_t qh = 0, qn = 0;  // the ring's head and fill (wave-uniform)
+  // A tile of second asks from the ring: column col takes slot qh + col;
+  // columns at or beyond cnt are padding.
+  auto secondTile = [&](uint32_t cnt) {
+    waveSync();
+    const bool valid = (uint32_t)col < cnt;
+    const uint32_t sl = ((qh + (uint32_t)col) & 31u) * 3u;
+    const float4 b0 = q[sl], b1 = q[sl + 1], b2 = q[sl + 2];
+    waveSync();
+    const bf16x8 xb = askSlots(M, embl, g, b0.x, b0.y, b0.z, b0.w, b1.x, b1.y,
+                               valid ? __float_as_int(b2.y) : 0);
+    const float z1 = inferColumns(w1f, w2f, b1s, b2s, w3s, b3, xb);
+    if (g == 0 && valid) P.rkey[__float_as_uint(b2.x)] = z1 > kHitLogit ? boundKey(b1.z) : kNoBound;
   };
-  auto stageB = [&](InferGroup& G) {
-    float4 op = make_float4(0.f, 0.f, 0.f, 0.f), ds = op;
-    if (G.rc >= 2) {
-      op = P.R.op[G.r];
-      ds = P.R.ds[G.r];
-    }
-    loadSlots(M, g, op, ds, (int)G.nr.x, G.x0);
-    loadSlots(M, g, op, ds, (int)G.nr.z, G.x1);
-  };
-  uint32_t base = c->r_beg + (blockIdx.x * kInferWaves + w) * 16u;
-  InferGroup cur, nxt, nx2;
-  stageA(base, cur);
-  stageB(cur);
-  stageA(base + stride, nxt);
-  for (; base < re; base += stride) {
-    stageA(base + 2 * stride, nx2);
-    stageB(nxt);
-    const bool ask = cur.rc >= 2;
-    unsigned long long key = 0ull;
-    if (__ballot(ask) != 0ull) {
-      const float z0 = inferColumns(w1f, w2f, b1s, b2s, w3s, b3, toSlots(cur.x0));
-      const bool hit0 = ask && z0 > kHitLogit;
-      const bool ask1 = ask && !hit0 && cur.rc >= 3;
-      bool hit1 = false;
-      if (__ballot(ask1) != 0ull) {
-        const float z1 = inferColumns(w1f, w2f, b1s, b2s, w3s, b3, toSlots(cur.x1));
-        hit1 = ask1 && z1 > kHitLogit;
+  // one kCount block region per wave at a time
+  for (uint32_t rb = first; rb < P.ask_regions; rb += nwv) {
+    const uint32_t n = min(P.ask_cnt[rb], reg);
+    if (n == 0u) continue;
+    const float4* recs = P.ask + (size_t)rb * reg * 3;
+    // unconditional loads (columns past the end read the last record and are
+    // masked), so that no load result is merged with an older value and the
+    // loads stay in flight until their tile is consumed
+    float4 A0[kAskStages], A1[kAskStages], A2[kAskStages];
+    auto fetch = [&](int k, uint32_t tile) {
+      const float4* rec = recs + (size_t)min(tile * 16u + (uint32_t)col, n - 1u) * 3;
+      A0[k] = rec[0];
+      A1[k] = rec[1];
+      A2[k] = rec[2];
+    };
+#pragma unroll
+    for (int k = 0; k < kAskStages; ++k) fetch(k, (uint32_t)k);
+    for (uint32_t t0 = 0; t0 * 16u < n; t0 += kAskStages) {
+#pragma unroll
+      for (int k = 0; k < kAskStages; ++k) {
+        const uint32_t tile = t0 + (uint32_t)k;
+        if (tile * 16u >= n) break;  // wave-uniform
+        const float4 a0 = A0[k], a1 = A1[k], a2 = A2[k];
+        fetch(k, tile + kAskStages);  // the slot's next tile
+        const bool valid = tile * 16u + (uint32_t)col < n;
+        const bf16x8 xb = askSlots(M, embl, g, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
+                                   valid ? __float_as_int(a2.y) : 0);
+        const float z0 = inferColumns(w1f, w2f, b1s, b2s, w3s, b3, xb);
+        const bool hit0 = valid && z0 > kHitLogit;
+        const bool need2 = valid && !hit0 && __float_as_int(a2.w) >= 3;
+        if (g == 0 && valid && !need2) P.rkey[__float_as_uint(a2.x)] = hit0 ? boundKey(a1.z) : kNoBound;
+        // second asks: the ray with its second candidate's entry t and domain
+        const unsigned long long m = __ballot(g == 0 && need2);
+        if (m != 0ull) {
+          if (g == 0 && need2) {
+            const uint32_t sl =
+                ((qh + qn + (uint32_t)__popcll(m & ((1ull << l) - 1ull))) & 31u) * 3u;
+            q[sl] = a0;
+            q[sl + 1] = make_float4(a1.x, a1.y, a1.w, 0.f);
+            q[sl + 2] = make_float4(a2.x, a2.z, 0.f, 0.f);
+          }
+          qn += (uint32_t)__popcll(m);
+          if (qn >= 16u) {  // at most 31 wait: 15 before this tile's 16
+            secondTile(16u);
+            qh = (qh + 16u) & 31u;
+            qn -= 16u;
+          }
+        }
       }
-      // bit 32: a bound exists; below it the bound's entry t (ordered bits,
-      // complemented)
-      if (hit0 || hit1)
-        key = (1ull << 32) | (uint32_t)(~orderedF(__uint_as_float(hit0 ? cur.nr.y : cur.nr.w)));
     }
-    if (cur.r < re && l < 16) rkey[cur.r] = key;
-    cur = nxt;
-    nxt = nx2;
+  }
+  if (qn > 0u) secondTile(qn);
+}
+
+// Splits the radiance queue into the phase-A list and the deferred list,
+// in queue order (so that phase A keeps the schedule's domain order).
+// Entries of rays without a bound (kNoBound: one candidate domain, or no
+// candidate predicted to hold the hit) stay in phase A.  Each of
+// kAssignWaves waves owns a contiguous span of the queue (a multiple of 64
+// entries): kAssignCount writes one deferral bit per entry and the span's
+// two counts, kAssignScan turns the counts into list bases, kAssignWrite
+// places the entries.  No atomics.
+constexpr int kAssignUnroll = 4;
+
+__device__ __forceinline__ uint32_t assignSpan(uint32_t n) {
+  return ((n + kAssignWaves - 1) / kAssignWaves + 63u) & ~63u;
+}
+
+__global__ void __launch_bounds__(256) kAssignCount(TraceParams P) {
+  const DevCounters* c = P.ctr;
+  if (!c->active) return;
+  const uint32_t n = c->q_total[0], span = assignSpan(n);
+  const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int l = __lane_id();
+  const uint32_t beg = min(wv * span, n), end = min(beg + span, n);
+  uint32_t ndef = 0;
+  for (uint32_t q0 = beg; q0 < end; q0 += 64 * kAssignUnroll) {  // wave-uniform
+    bool d[kAssignUnroll];
+#pragma unroll
+    for (int k = 0; k < kAssignUnroll; ++k) {
+      const uint32_t q = q0 + 64u * k + (uint32_t)l;
+      d[k] = false;
+      if (q < end) {
+        const QEntry e = P.q[0][q];
+        d[k] = orderedF(e.tdom) > P.rkey[e.ray];  // beyond the chosen domain
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kAssignUnroll; ++k) {
+      const unsigned long long m = __ballot(d[k]);
+      const uint32_t w0 = q0 + 64u * k;
+      if (w0 < end && l == 0) P.dflag[w0 >> 6] = m;
+      ndef += (uint32_t)__popcll(m);
+    }
+  }
+  if (l == 0) {
+    P.acnt[wv] = (end - beg) - ndef;  // phase A
+    P.acnt[kAssignWaves + wv] = ndef;  // deferred
   }
 }
 
-// Splits the radiance queue into the phase-A list and the deferred list.
-// Entries of rays without a key (one candidate domain) stay in phase A.
-// Each block compacts 8 x 256 entries per step with one atomic per list.
-constexpr int kAssignItems = 8;
-__global__ void __launch_bounds__(256) kAssignPhase(TraceParams P, const unsigned long long* rkey) {
+// One block: exclusive scans of both count arrays (list bases in P.acnt's
+// place), the list lengths and the deferred-entry statistic.
+__global__ void __launch_bounds__(1024) kAssignScan(TraceParams P) {
   DevCounters* c = P.ctr;
   if (!c->active) return;
-  __shared__ uint32_t wcnt[2][kAssignItems][4];
-  __shared__ uint32_t bases[2];
-  const uint32_t n = c->q_total[0];
-  const int w = threadIdx.x >> 6, l = __lane_id();
-  const unsigned long long below = (1ull << l) - 1ull;
-  unsigned long long deferred = 0;
-  const uint32_t step = gridDim.x * 256u * kAssignItems;
-  for (uint32_t base = blockIdx.x * 256u * kAssignItems; base < n; base += step) {
-    bool dflag[kAssignItems], vflag[kAssignItems];
-    uint32_t rank[kAssignItems];
+  constexpr int kPer = kAssignWaves / 1024;
+  __shared__ uint32_t wsum[16];
+  const int t = threadIdx.x, l = __lane_id(), w = t >> 6;
+  for (int list = 0; list < 2; ++list) {
+    uint32_t* a = P.acnt + list * kAssignWaves;
+    uint32_t v[kPer], sum = 0;
 #pragma unroll
-    for (int k = 0; k < kAssignItems; ++k) {
-      uint32_t q = base + k * 256u + threadIdx.x;
-      bool valid = q < n, d = false;
-      if (valid) {
-        QEntry e = P.q[0][q];
-        unsigned long long key = rkey[e.ray];
-        d = key != 0ull && orderedF(e.tdom) > ~(uint32_t)key;  // beyond the chosen domain
-      }
-      vflag[k] = valid;
-      dflag[k] = d;
-      unsigned long long ba = __ballot(valid && !d), bb = __ballot(valid && d);
-      rank[k] = (uint32_t)__popcll((d ? bb : ba) & below);
-      if (l == 0) {
-        wcnt[0][k][w] = (uint32_t)__popcll(ba);
-        wcnt[1][k][w] = (uint32_t)__popcll(bb);
-      }
-      deferred += (valid && d) ? 1 : 0;
+    for (int k = 0; k < kPer; ++k) {
+      v[k] = a[t * kPer + k];
+      sum += v[k];
     }
-    __syncthreads();
-    if (threadIdx.x < 2) {  // exclusive prefix over (item, wave) in place, one atomic per list
-      uint32_t acc = 0;
-      for (int k = 0; k < kAssignItems; ++k)
-        for (int ww = 0; ww < 4; ++ww) {
-          uint32_t t = wcnt[threadIdx.x][k][ww];
-          wcnt[threadIdx.x][k][ww] = acc;
-          acc += t;
-        }
-      bases[threadIdx.x] = acc ? atomicAdd(&c->n_phase[threadIdx.x], acc) : 0u;
+    uint32_t x = sum;  // inclusive wave scan
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (l >= o) x += y;
     }
+    if (l == 63) wsum[w] = x;
     __syncthreads();
+    uint32_t pre = x - sum, total = 0;
+    for (int j = 0; j < 16; ++j) {
+      pre += j < w ? wsum[j] : 0u;
+      total += wsum[j];
+    }
 #pragma unroll
-    for (int k = 0; k < kAssignItems; ++k) {
-      if (!vflag[k]) continue;
-      const int list = dflag[k] ? 1 : 0;
-      uint32_t idx = bases[list] + wcnt[list][k][w] + rank[k];
-      if (idx < P.q_cap[0]) {
-        (list ? P.qb : P.qa)[idx] = base + k * 256u + threadIdx.x;
-      } else {
-        atomicOr(&c->err, 1u);
-      }
+    for (int k = 0; k < kPer; ++k) {
+      a[t * kPer + k] = pre;
+      pre += v[k];
+    }
+    if (t == 0) {
+      c->n_phase[list] = total;
+      if (list == 1) c->deferred += total;
     }
     __syncthreads();
   }
-  deferred = waveSumULL(deferred);
-  if (l == 0 && deferred) atomicAdd(&c->deferred, deferred);
+}
+
+__global__ void __launch_bounds__(256) kAssignWrite(TraceParams P) {
+  DevCounters* c = P.ctr;
+  if (!c->active) return;
+  const uint32_t n = c->q_total[0], span = assignSpan(n);
+  const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int l = __lane_id();
+  const unsigned long long below = (1ull << l) - 1ull;
+  const uint32_t beg = min(wv * span, n), end = min(beg + span, n);
+  uint32_t ba = P.acnt[wv], bb = P.acnt[kAssignWaves + wv];  // wave-uniform
+  bool over = false;
+  for (uint32_t q0 = beg; q0 < end; q0 += 64 * kAssignUnroll) {
+    unsigned long long m[kAssignUnroll];
+#pragma unroll
+    for (int k = 0; k < kAssignUnroll; ++k) {
+      const uint32_t w0 = q0 + 64u * k;
+      m[k] = w0 < end ? P.dflag[w0 >> 6] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kAssignUnroll; ++k) {
+      const uint32_t w0 = q0 + 64u * k;
+      if (w0 >= end) break;  // wave-uniform
+      const uint32_t cnt = min(end - w0, 64u);
+      const unsigned long long valid = cnt == 64u ? ~0ull : ((1ull << cnt) - 1ull);
+      const unsigned long long da = valid & ~m[k], db = m[k];
+      const uint32_t q = w0 + (uint32_t)l;
+      if ((valid >> l) & 1ull) {
+        const bool d = (db >> l) & 1ull;
+        const uint32_t idx = d ? bb + (uint32_t)__popcll(db & below) : ba + (uint32_t)__popcll(da & below);
+        if (idx < P.q_cap[0])
+          (d ? P.qb : P.qa)[idx] = q;
+        else
+          over = true;
+      }
+      ba += (uint32_t)__popcll(da);
+      bb += (uint32_t)__popcll(db);
+    }
+  }
+  if (over) atomicOr(&c->err, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,13 +760,14 @@ __global__ void __launch_bounds__(kMlpBlock) kLogits(MlpDev M, const float* samp
 struct Predictor {
   int nd = 0;
   int nparams = 0;
-  DevBuf p, m, v, g, w1, w2, w2t, w1et, w1f, w2f, loss, rkey;
+  DevBuf p, m, v, g, w1, w2, w2t, w1et, w1f, w2f, loss;
   DevBuf xt, h1t, dh2t, dh1t, index;
   int step = 0;
   bool trained = false;
   float center[3], inv_half[3];
   int tbatch = 0;
   int pred_grid = 0;  // persistent inference grid (blocks)
+  int emb_lds = 0;    // inference keeps the embedding slots in LDS
   MlpDev dev() const {
     MlpDev M;
     M.w1f = w1f.as<__bf16>();
@@ -682,7 +789,7 @@ void predictorRelease(Ctx* c) {
   if (!c->pred) return;
   Predictor* R = c->pred;
   for (DevBuf* b : {&R->p, &R->m, &R->v, &R->g, &R->w1, &R->w2, &R->w2t, &R->w1et, &R->w1f, &R->w2f, &R->loss,
-                    &R->rkey, &R->xt, &R->h1t, &R->dh2t, &R->dh1t, &R->index})
+                    &R->xt, &R->h1t, &R->dh2t, &R->dh1t, &R->index})
     b->release();
   delete R;
   c->pred = nullptr;
@@ -785,23 +892,34 @@ __global__ void kBatchIndex(uint32_t* idx, int batch, unsigned long long start,
   if (i < batch) idx[i] = (uint32_t)(((start + (unsigned long long)i) * stride) % n);
 }
 
-void predictorPhase(Ctx* c, const TraceParams& P) {
-  Predictor* R = c->pred;  // rkey was sized by predictorPrepare (no allocation here: graph capture)
+static size_t embLdsBytes(int nd) { return nd <= kEmbLdsMax ? (size_t)nd * 3 * sizeof(bf16x8) : 0; }
+
+void predictorInfer(Ctx* c, const TraceParams& P, hipStream_t s) {
+  Predictor* R = c->pred;  // grid set by predictorPrepare (no allocation here: graph capture)
+  kPredictRays<<<R->pred_grid, 64 * kInferWaves, R->emb_lds ? embLdsBytes(P.ndomains) : 0, s>>>(
+      P, R->dev(), R->emb_lds);
+}
+
+void predictorAssign(Ctx* c, const TraceParams& P) {
   hipStream_t s = c->stream;
-  kPredictRays<<<R->pred_grid, 64 * kInferWaves, 0, s>>>(P, R->dev(), R->rkey.as<unsigned long long>());
-  kAssignPhase<<<1024, 256, 0, s>>>(P, R->rkey.as<unsigned long long>());
+  kAssignCount<<<kAssignBlocks, 256, 0, s>>>(P);
+  kAssignScan<<<1, 1024, 0, s>>>(P);
+  kAssignWrite<<<kAssignBlocks, 256, 0, s>>>(P);
 }
 
 bool predictorReady(Ctx* c) { return c->pred && c->pred->trained; }
 
-void predictorPrepare(Ctx* c, size_t r_cap) {
+void predictorPrepare(Ctx* c) {
   if (!c->pred) throw StatusError(SPRAY_ERR_STATE, "learned schedule needs a trained predictor");
-  c->pred->rkey.reserve(r_cap * 8);
-  if (!c->pred->pred_grid) {
-    hipDeviceProp_t prop;
-    SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
-    c->pred->pred_grid = prop.multiProcessorCount * 2;  // two 4-wave blocks per CU
-  }
+  Predictor* R = c->pred;
+  const int nd = c->pool.ndomains;
+  R->emb_lds = nd <= kEmbLdsMax ? 1 : 0;
+  const size_t lds = embLdsBytes(nd);
+  hipDeviceProp_t prop;
+  SPRAY_HIP(hipGetDeviceProperties(&prop, c->device));
+  int per_cu = 0;
+  SPRAY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kPredictRays, 64 * kInferWaves, lds));
+  R->pred_grid = prop.multiProcessorCount * std::max(1, per_cu);
 }
 
 }  // namespace spray
@@ -1002,16 +1120,21 @@ int spray_mlp_grad(spray_ctx* ctx, const float* samples, int n, const float* par
 }
 
 // Logits of host samples through the inference kernel's arithmetic (the
-// register-resident fragments and the chained layers of kPredictRays), so
-// that the logits the learned schedule acts on are checked against the
-// float64 oracle.
-__global__ void __launch_bounds__(64) kLogitsInfer(MlpDev M, const float* samples, int n, float* out) {
+// register-resident fragments, the chained layers and the bf16 input slots
+// of kPredictRays), so that the logits the learned schedule acts on are
+// checked against the float64 oracle.  Even 16-sample tiles take the
+// embedding slots from the LDS table, odd tiles from the fp32 master (the
+// path of scenes with more than kEmbLdsMax domains), so both are checked.
+__global__ void __launch_bounds__(64) kLogitsInfer(MlpDev M, int nd, int emb_lds, const float* samples,
+                                                   int n, float* out) {
+  extern __shared__ bf16x8 embs[];
   __shared__ __attribute__((aligned(16))) float b1s[kHid], b2s[kHid], w3s[kHid];
   for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
     b1s[i] = M.p[oB1 + i];
     b2s[i] = M.p[oB2 + i];
     w3s[i] = M.p[oW3 + i];
   }
+  if (emb_lds) fillEmbLds(M, nd, embs);
   __syncthreads();
   const int l = __lane_id(), g = l >> 4;
   bf16x8 w1f[8], w2f[8][4];
@@ -1023,17 +1146,16 @@ __global__ void __launch_bounds__(64) kLogitsInfer(MlpDev M, const float* sample
   }
   for (int s0 = blockIdx.x * 16; s0 < n; s0 += gridDim.x * 16) {
     const int sidx = s0 + (l & 15);
-    float4 op = make_float4(0.f, 0.f, 0.f, 0.f), ds = op;
+    float x[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int dom = 0;
     if (sidx < n) {
       const float* sm = samples + (size_t)sidx * 8;
-      op = make_float4(sm[0], sm[1], sm[2], 0.f);
-      ds = make_float4(sm[3], sm[4], sm[5], 0.f);
+      for (int k = 0; k < 6; ++k) x[k] = sm[k];
       dom = (int)sm[6];
     }
-    float x[8];
-    loadSlots(M, g, op, ds, dom, x);
-    const float z = inferColumns(w1f, w2f, b1s, b2s, w3s, M.p[oB3], toSlots(x));
+    const bf16x8* embl = (emb_lds && ((s0 >> 4) & 1) == 0) ? embs : nullptr;
+    const bf16x8 xb = askSlots(M, embl, g, x[0], x[1], x[2], x[3], x[4], x[5], dom);
+    const float z = inferColumns(w1f, w2f, b1s, b2s, w3s, M.p[oB3], xb);
     if (l < 16 && sidx < n) out[sidx] = z;
   }
 }
@@ -1054,8 +1176,9 @@ int spray_mlp_logits(spray_ctx* ctx, const float* samples, int n, const float* p
     ds.reserve((size_t)n * 32);
     out.reserve((size_t)n * 4);
     SPRAY_HIP(hipMemcpy(ds.p, samples, (size_t)n * 32, hipMemcpyHostToDevice));
-    kLogitsInfer<<<std::min(1024, (n + 15) / 16), 64, 0, c->stream>>>(R->dev(), ds.as<float>(), n,
-                                                                     out.as<float>());
+    const int nd = c->pool.ndomains;
+    kLogitsInfer<<<std::min(1024, (n + 15) / 16), 64, embLdsBytes(nd), c->stream>>>(
+        R->dev(), nd, nd <= kEmbLdsMax ? 1 : 0, ds.as<float>(), n, out.as<float>());
     SPRAY_HIP(hipStreamSynchronize(c->stream));
     SPRAY_HIP(hipMemcpy(logits_out, out.p, (size_t)n * 4, hipMemcpyDeviceToHost));
   });

@@ -40,6 +40,12 @@ constexpr int kBlock = 256;
 // grid of kCount and kScatter (they must match: blocks own queue slots)
 constexpr int kQueueGrid = 1024;
 constexpr int kShadeRec = 5;  // float4s per shade record
+// Ask records per kCount block region: a block takes at most kBlock of
+// every kQueueGrid * kBlock consecutive rays of a pass.
+inline uint32_t askRegion(uint32_t r_cap) {
+  const uint32_t stride = (uint32_t)kQueueGrid * kBlock;
+  return (uint32_t)kBlock * ((r_cap + stride - 1) / stride);
+}
 // Domain grid of the domain-list pass: a uniform grid over the domain boxes
 // (at most kGridCells cells, kGridMaxRes per axis), each cell listing the
 // domains whose box
@@ -398,6 +404,8 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   uint32_t* lc = (uint32_t*)(smc + L.lc);                     // 2 * nd counts
   uint32_t* ls = (uint32_t*)(smc + L.ls);                     // nd score (32-bit per block)
   unsigned long long* lmask = (unsigned long long*)(smc + L.mask) + threadIdx.x;
+  __shared__ uint32_t ask_n[1];  // learned form: ask records of this block
+  if (threadIdx.x == 0) ask_n[0] = 0;
   for (int i = threadIdx.x; i < 2 * nd; i += blockDim.x) box[i] = P.dbox[i];
   DomGrid G;
   {
@@ -458,16 +466,33 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
         if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
-      if (kind == 0) {
-        P.rcand[r] = (uint8_t)min(hits, 255);
-        // the two nearest candidates, for the learned form's predictor
-        if (P.learned && kept >= 2)
-          P.rnear[r] = make_uint4((uint32_t)ti[0], __float_as_uint(tt[0]), (uint32_t)ti[1],
-                                  __float_as_uint(tt[1]));
+    }
+    // learned form: a radiance ray with two or more candidate domains writes
+    // an ask record for the predictor into this block's region (one LDS
+    // atomic per wave); the others have no phase-A bound
+    if (P.learned) {
+      const bool ask = valid && kind == 0 && kept >= 2;
+      const unsigned long long am = __ballot(ask);
+      uint32_t abase = 0;
+      if (am != 0ull) {
+        const int leader = __builtin_ctzll(am);
+        if (__lane_id() == leader) abase = atomicAdd(ask_n, (uint32_t)__popcll(am));
+        abase = __builtin_amdgcn_readlane(abase, leader);
+      }
+      const uint32_t a = abase + (uint32_t)__popcll(am & ((1ull << __lane_id()) - 1ull));
+      if (ask && a < P.ask_region) {
+        float4* rec = P.ask + ((size_t)blockIdx.x * P.ask_region + a) * 3;
+        rec[0] = make_float4(o[0], o[1], o[2], dir[0]);
+        rec[1] = make_float4(dir[1], dir[2], tt[0], tt[1]);
+        rec[2] = make_float4(__uint_as_float(r), __int_as_float(ti[0]), __int_as_float(ti[1]),
+                             __int_as_float(min(hits, 255)));
+      } else if (valid && kind == 0) {
+        P.rkey[r] = kNoBound;
       }
     }
   }
   __syncthreads();
+  if (P.learned && threadIdx.x == 0) P.ask_cnt[blockIdx.x] = min(*ask_n, P.ask_region);
   // per-block counts, [kind][domain][block]: kScatter's blocks take the same
   // rays and place their entries at the bases these counts give
   for (int j = threadIdx.x; j < 2 * nd; j += blockDim.x)
@@ -1298,8 +1323,8 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0;
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
-      wctr, score, order, q[2], qdom[2], rcand, rnear, dmask, dbox, ovf, vlog, log, sh_rec, qa,
-      qb;
+      wctr, score, order, q[2], qdom[2], ask, ask_cnt, rkey, dmask, dbox, ovf, vlog, log, sh_rec, qa,
+      qb, dflag, acnt;
   unsigned long long log_cap = 0;
   bool logging = false;
   DevBuf log_ctr;
@@ -1329,11 +1354,15 @@ struct TracerState {
     for (const DevBuf* x : {R, Pn, Sh})
       for (int k = 0; k < 4; ++k) b += x[k].bytes;
     for (const DevBuf* x : {&s_tmax, &s_occ, &vbuf, &image, &ctr, &tile_rec, &cnt, &off, &dbase, &bcnt,
-                            &bbase, &wctr, &score, &order, &q[0], &q[1], &qdom[0], &qdom[1], &rcand, &rnear,
-                            &dmask, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb})
+                            &bbase, &wctr, &score, &order, &q[0], &q[1], &qdom[0], &qdom[1], &ask, &ask_cnt, &rkey,
+                            &dmask, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb, &dflag, &acnt})
       b += x->bytes;
     return b;
   }
+  // learned form: the predictor's stream beside the schedule and scatter
+  // (forked from and joined back to the tracer's stream, graph capture included)
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // kernel timing of the trace launches (hipEvents on the tracer's stream)
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
@@ -1389,11 +1418,17 @@ void tracerRelease(Ctx* c) {
   TracerState* T = c->tracer;
   T->destroyGraphs();
   for (hipEvent_t e : T->ev) (void)hipEventDestroy(e);
+  if (T->aux) {
+    (void)hipStreamSynchronize(T->aux);
+    (void)hipStreamDestroy(T->aux);
+  }
+  for (hipEvent_t e : {T->fork_ev, T->join_ev})
+    if (e) (void)hipEventDestroy(e);
   for (auto* arr : {T->R, T->Pn, T->Sh})
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->rcand, &T->rnear, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb})
+                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->ask, &T->ask_cnt, &T->rkey, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb, &T->dflag, &T->acnt})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1508,8 +1543,11 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->s_tmax, (size_t)T->s_cap * 4);
   need(T, T->s_occ, (size_t)T->s_cap * 4);
   need(T, T->vbuf, (size_t)S * kHistory * 8);
-  need(T, T->rcand, (size_t)T->r_cap + 64);
-  if (cfg.schedule == SPRAY_SCHEDULE_LEARNED) need(T, T->rnear, (size_t)T->r_cap * 16 + 64);
+  if (cfg.schedule == SPRAY_SCHEDULE_LEARNED) {
+    need(T, T->ask, (size_t)kQueueGrid * askRegion(T->r_cap) * 48 + 64);
+    need(T, T->ask_cnt, (size_t)kQueueGrid * 4 + 64);
+    need(T, T->rkey, (size_t)T->r_cap * 4 + 64);
+  }
   need(T, T->dmask, ((size_t)T->r_cap + T->s_cap) * ((nd + 63) / 64) * 8 + 64);
   need(T, T->dbox, dboxBytes(nd));
   need(T, T->ctr, sizeof(DevCounters));
@@ -1522,6 +1560,10 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->wctr, (size_t)(nd + 3) * 4 + 64);
   need(T, T->qa, (size_t)T->q_cap[0] * 4);
   need(T, T->qb, (size_t)T->q_cap[0] * 4);
+  if (cfg.schedule == SPRAY_SCHEDULE_LEARNED) {
+    need(T, T->dflag, ((size_t)T->q_cap[0] / 64 + 2) * 8);
+    need(T, T->acnt, (size_t)2 * kAssignWaves * 4 + 64);
+  }
   // shade records: one per accepted hit of a pass, far fewer than queue
   // entries (an overflow grows and re-renders like any other pool)
   T->sh_cap = std::min(T->q_cap[0], T->r_cap);
@@ -1626,8 +1668,11 @@ static TraceParams makeParams(Ctx* c) {
   P.q[1] = T->q[1].as<QEntry>();
   P.qdom[0] = T->qdom[0].as<uint16_t>();
   P.qdom[1] = T->qdom[1].as<uint16_t>();
-  P.rcand = T->rcand.as<uint8_t>();
-  P.rnear = T->rnear.as<uint4>();
+  P.ask = T->ask.as<float4>();
+  P.ask_cnt = T->ask_cnt.as<uint32_t>();
+  P.ask_regions = kQueueGrid;
+  P.ask_region = T->ask.bytes >= (size_t)kQueueGrid * askRegion(T->r_cap) * 48 ? askRegion(T->r_cap) : 0u;
+  P.rkey = T->rkey.as<uint32_t>();
   P.dmask = T->dmask.as<unsigned long long>();
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
@@ -1636,6 +1681,8 @@ static TraceParams makeParams(Ctx* c) {
   P.sh_cap = T->sh_cap;
   P.qa = T->qa.as<uint32_t>();
   P.qb = T->qb.as<uint32_t>();
+  P.dflag = T->dflag.as<unsigned long long>();
+  P.acnt = T->acnt.as<uint32_t>();
   P.learned = (cfg.schedule == SPRAY_SCHEDULE_LEARNED) ? 1 : 0;
   P.log_on = T->logging ? 1 : 0;
   P.vlog = T->vlog.as<uint32_t>();
@@ -1753,13 +1800,26 @@ static size_t travShmem(int block) { return (size_t)kRing * block * 4; }
 
 static void passQueues(Ctx* c, const TraceParams& P) {
   hipStream_t s = c->stream;
+  TracerState* T = c->tracer;
   int nd = P.ndomains;
   kPassBegin<<<1, 1, 0, s>>>(P);
   kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
   kCount<<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+  // learned form: the predictor needs only kCount's ask records, so it runs
+  // on the second stream beside the schedule and the scatter
+  if (P.learned) {
+    SPRAY_HIP(hipEventRecord(T->fork_ev, s));
+    SPRAY_HIP(hipStreamWaitEvent(T->aux, T->fork_ev, 0));
+    predictorInfer(c, P, T->aux);
+    SPRAY_HIP(hipEventRecord(T->join_ev, T->aux));
+  }
   kSchedule<<<1, 1024, 0, s>>>(P);
   kBlockBases<<<2 * nd, kBlock, 0, s>>>(P);
   kScatter<<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);
+  if (P.learned) {
+    SPRAY_HIP(hipStreamWaitEvent(s, T->join_ev, 0));
+    predictorAssign(c, P);
+  }
 }
 
 // Shading of the pass's accepted hits (those not yet shaded), then the new
@@ -1931,7 +1991,6 @@ static void passBatched(Ctx* c, const TraceParams& P) {
   hipStream_t s = c->stream;
   int block = 64 * std::max(1, c->batch_tile);
   passQueues(c, P);
-  if (P.learned) predictorPhase(c, P);
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
   int ei = c->profile ? T->evBegin(s) : -1;
@@ -1995,7 +2054,12 @@ void tracerTraceFrame(Ctx* c, float* rgba_out, spray_stats* st) {
     if (cfg.schedule == SPRAY_SCHEDULE_LEARNED) {
       if (!predictorReady(c))
         throw StatusError(SPRAY_ERR_STATE, "the learned schedule needs a trained or loaded predictor");
-      predictorPrepare(c, T->r_cap);
+      predictorPrepare(c);
+      if (!T->aux) {
+        SPRAY_HIP(hipStreamCreateWithFlags(&T->aux, hipStreamNonBlocking));
+        SPRAY_HIP(hipEventCreateWithFlags(&T->fork_ev, hipEventDisableTiming));
+        SPRAY_HIP(hipEventCreateWithFlags(&T->join_ev, hipEventDisableTiming));
+      }
     }
     TraceParams P = makeParams(c);
     P.ovf = T->ovf.as<uint32_t>();

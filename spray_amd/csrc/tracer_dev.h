@@ -89,9 +89,19 @@ struct TraceParams {
   int* order;         // [nd] schedule
   QEntry* q[2];       // queues (radiance, shadow)
   uint16_t* qdom[2];  // domain of each queue entry
-  uint8_t* rcand;     // radiance rays: queue entries of this pass (saturating)
-  uint4* rnear;       // learned form, radiance rays with >= 2 candidates: the two nearest
-                      // candidate domains (id, entry t bits, id, entry t bits)
+  // learned form: one ask record (3 float4) per radiance ray of the pass with
+  // two or more candidate domains, written by kCount into its block's region
+  // (block b's records at [b * ask_region, b * ask_region + ask_cnt[b])):
+  // (origin, dir.x), (dir.y, dir.z, entry t of the nearest and of the second
+  // candidate), (ray, nearest domain, second domain, candidates)
+  float4* ask;
+  uint32_t* ask_cnt;    // [ask_regions]
+  uint32_t ask_regions;  // kCount's blocks
+  uint32_t ask_region;   // records per block region (0: no ask buffer)
+  // learned form, per radiance ray of the pass: the phase-A bound, the
+  // ordered bits of its entry t (entries entering later are deferred), or
+  // kNoBound
+  uint32_t* rkey;
   uint32_t rng[4];    // range launches: radiance base, count, shadow base, count
   unsigned long long* dmask;  // [pass ray][(nd + 63) / 64]: domains hit (kCount -> kScatter)
   uint32_t q_cap[2];
@@ -106,6 +116,8 @@ struct TraceParams {
   // learned form
   uint32_t* qa;       // learned form: radiance queue indices of phase A
   uint32_t* qb;       //   and of the deferred entries (traced last)
+  unsigned long long* dflag;  // learned form: deferral bit of each radiance queue entry
+  uint32_t* acnt;     // learned form: [2][kAssignWaves] span counts, then list bases
   int learned;
   // predictor logging: visit log of the pass (ray, domain, t) and samples
   int log_on;
@@ -116,12 +128,18 @@ struct TraceParams {
   unsigned long long* log_count;  // samples logged so far (kept across frames)
 };
 
+constexpr uint32_t kNoBound = 0xffffffffu;
+constexpr int kAssignBlocks = 1024;  // blocks of the phase split (4 waves each)
+constexpr int kAssignWaves = kAssignBlocks * 4;
+
 struct Ctx;
-// mlp.hip: assigns every radiance queue entry of the pass to phase A or the
-// deferred list (learned form)
-void predictorPhase(Ctx* c, const TraceParams& P);
+// mlp.hip, learned form: the phase-A bound of every asking ray of the pass
+// (after kCount; on stream s), then every radiance queue entry of the pass
+// into phase A or the deferred list (after kScatter and the bounds)
+void predictorInfer(Ctx* c, const TraceParams& P, hipStream_t s);
+void predictorAssign(Ctx* c, const TraceParams& P);
 bool predictorReady(Ctx* c);
-void predictorPrepare(Ctx* c, size_t r_cap);
+void predictorPrepare(Ctx* c);
 
 // ---- history buffer (VBuf, ooc/ooc_vbuf.cc:41-112) -------------------------
 // Each of the 4 depth slots of a sample is one 64-bit word: the accepted t in
