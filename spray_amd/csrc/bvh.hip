@@ -454,6 +454,13 @@ static long long rangeLevels(int n, RangeLevels* L, int* nlev) {
   return off;
 }
 
+// 4-wide collapse levels launched per host wait, and at most (deeper trees
+// do not fit the traversal stack anyway); a level counter block is the node
+// counter, the overflow flag and the item count of every level
+constexpr int kLevelBatch = 16;
+constexpr int kMaxLevels = 48;
+constexpr int kLevelWords = 2 + kMaxLevels + 1;
+
 // Collapse the binary tree into 4-wide nodes, one frontier level per launch.
 struct FrontierItem {
   int bnode;  // binary inner node
@@ -482,16 +489,17 @@ __device__ __forceinline__ int waveAlloc(int* ctr, int cnt) {
 }
 
 // One frontier level: *nin items in fin (0: nothing to do), next level's
-// items counted in *nout.  Launched level after level without host round
-// trips; the loop bound is uniform per block so whole waves allocate.
-__global__ void kCollapse(const FrontierItem* __restrict__ fin, const int* __restrict__ nin_ptr,
-                          FrontierItem* fout, int* nout, int* node_counter, int node_cap,
-                          const int* __restrict__ left, const int* __restrict__ right,
-                          const int* __restrict__ first, const int* __restrict__ last,
-                          const float* __restrict__ leafbox, const float* __restrict__ innerbox,
-                          BNode4* nodes, int* overflow) {
+// items counted in *nout, on blocks blk of nblk.  Launched level after level
+// without host round trips; the loop bound is uniform per block so whole
+// waves allocate.
+__device__ __forceinline__ void collapseLevel(
+    int blk, int nblk, const FrontierItem* __restrict__ fin, const int* __restrict__ nin_ptr,
+    FrontierItem* fout, int* nout, int* node_counter, int node_cap, const int* __restrict__ left,
+    const int* __restrict__ right, const int* __restrict__ first, const int* __restrict__ last,
+    const float* __restrict__ leafbox, const float* __restrict__ innerbox, BNode4* nodes,
+    int* overflow) {
   const int nin = *nin_ptr;
-  for (int base = blockIdx.x * blockDim.x; base < nin; base += gridDim.x * blockDim.x) {
+  for (int base = blk * blockDim.x; base < nin; base += nblk * blockDim.x) {
     const int it = base + threadIdx.x;
     const bool act = it < nin;
     FrontierItem f{0, 0};
@@ -570,12 +578,54 @@ __global__ void kCollapse(const FrontierItem* __restrict__ fin, const int* __res
   }
 }
 
-// Collapse state of one domain: node counter 1 (the root), no overflow,
-// one frontier item (the binary root) at level 0, empty later levels.
-__global__ void kCollapseInit(int* cc, int nwords, FrontierItem* fa, int root) {
-  for (int i = threadIdx.x; i < nwords; i += blockDim.x) cc[i] = (i == 0 || i == 2) ? 1 : 0;
-  if (threadIdx.x == 0) fa[0] = FrontierItem{root, 0};
+__global__ void kCollapse(const FrontierItem* __restrict__ fin, const int* __restrict__ nin_ptr,
+                          FrontierItem* fout, int* nout, int* node_counter, int node_cap,
+                          const int* __restrict__ left, const int* __restrict__ right,
+                          const int* __restrict__ first, const int* __restrict__ last,
+                          const float* __restrict__ leafbox, const float* __restrict__ innerbox,
+                          BNode4* nodes, int* overflow) {
+  collapseLevel(blockIdx.x, gridDim.x, fin, nin_ptr, fout, nout, node_counter, node_cap, left, right,
+                first, last, leafbox, innerbox, nodes, overflow);
 }
+
+// A group of domains collapsed together: one launch per level for all of
+// them, blocksPerDom blocks per domain.  Each domain's binary tree sits in
+// its own tree slot; cc is its level counter block ([0] node counter, [1]
+// overflow, [2 + l] items of level l).
+constexpr int kCollapseGroupMax = 16;
+struct CollapseDom {
+  const int *left, *right, *first, *last;
+  const float *leafbox, *innerbox;
+  FrontierItem *fa, *fb;
+  BNode4* nodes;
+  int* cc;
+  int node_cap;
+};
+struct CollapseGroup {
+  CollapseDom dom[kCollapseGroupMax];
+  int n;
+  int blocks_per_dom;
+};
+
+// Collapse state of each domain of the group: node counter 1 (the root),
+// no overflow, one frontier item (the binary root) at level 0, empty later
+// levels.
+__global__ void kCollapseGroupInit(CollapseGroup G) {
+  const CollapseDom& D = G.dom[blockIdx.x];
+  for (int i = threadIdx.x; i < kLevelWords; i += blockDim.x) D.cc[i] = (i == 0 || i == 2) ? 1 : 0;
+  if (threadIdx.x == 0) D.fa[0] = FrontierItem{0, 0};
+}
+
+__global__ void kCollapseGroupLevel(CollapseGroup G, int l) {
+  const int j = blockIdx.x / G.blocks_per_dom;  // uniform per block
+  const CollapseDom& D = G.dom[j];
+  const FrontierItem* fin = (l & 1) ? D.fb : D.fa;
+  FrontierItem* fout = (l & 1) ? D.fa : D.fb;
+  collapseLevel(blockIdx.x - j * G.blocks_per_dom, G.blocks_per_dom, fin, D.cc + 2 + l, fout,
+                D.cc + 3 + l, D.cc, D.node_cap, D.left, D.right, D.first, D.last, D.leafbox,
+                D.innerbox, D.nodes, D.cc + 1);
+}
+
 
 // Centroid bounds before their atomic min / max (as ordered bits).
 __global__ void kBoundsInit(uint32_t* b) {
@@ -762,26 +812,45 @@ static int gridFor(long long n, int block) {
 }
 
 // Scratch of one domain build, sized for the largest domain.
-// 4-wide collapse levels launched per host wait, and at most (deeper trees
-// do not fit the traversal stack anyway)
-constexpr int kLevelBatch = 16;
-constexpr int kMaxLevels = 48;
-constexpr int kLevelWords = 2 + kMaxLevels + 1;
 constexpr int kBuildStreams = 4;  // domain pipelines of a whole-scene LBVH build
+// tree slots of all pipelines together at most this many bytes
+constexpr size_t kTreeSlotBudget = (size_t)1 << 30;
 
-struct BuildScratch {
-  DevBuf keys, vals, keys2, vals2, hist, lb, ib, lft, rgt, fst, lst, pin, plf, fa, fb, cnts,
-      scal, adj, rpre, rsuf, rtab, lvl, sums;
-  DevBuf pc, pc2, pb, pb2, pnn, pcnt, pmf, pkf, pmp, pkp, pperm, pord;  // PLOC
-  void reserve(int maxf, long long maxv, bool ploc) {
+// One domain's binary tree (children, leaf ranges, leaf and inner boxes)
+// and its collapse frontiers: what the collapse reads after the build.
+struct TreeSlot {
+  DevBuf lb, ib, lft, rgt, fst, lst, fa, fb;
+  // a frontier level holds disjoint subtrees of more than kLeafMax triangles
+  static size_t frontierItems(int maxf) { return (size_t)maxf / (kLeafMax + 1) + 64; }
+  static size_t bytesFor(int maxf) {
+    return (size_t)maxf * (24 + 24 + 16) + 2 * frontierItems(maxf) * sizeof(FrontierItem);
+  }
+  void reserve(int maxf) {
     const size_t f4 = (size_t)maxf * 4 + 64;
-    for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &lft, &rgt, &fst, &lst, &pin, &plf})
-      b->reserve(f4);
-    hist.reserve((size_t)256 * ((maxf + kRsTile - 1) / kRsTile) * 4 + 64);
+    for (DevBuf* b : {&lft, &rgt, &fst, &lst}) b->reserve(f4);
     lb.reserve((size_t)maxf * 24 + 64);
     ib.reserve((size_t)maxf * 24 + 64);
-    fa.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
-    fb.reserve((size_t)maxf * sizeof(FrontierItem) + 64);
+    fa.reserve(frontierItems(maxf) * sizeof(FrontierItem));
+    fb.reserve(frontierItems(maxf) * sizeof(FrontierItem));
+  }
+  ~TreeSlot() {
+    for (DevBuf* b : {&lb, &ib, &lft, &rgt, &fst, &lst, &fa, &fb}) b->release();
+  }
+};
+
+struct BuildScratch {
+  DevBuf keys, vals, keys2, vals2, hist, pin, plf, cnts, scal, adj, rpre, rsuf, rtab, lvl, sums;
+  DevBuf pc, pc2, pb, pb2, pnn, pcnt, pmf, pkf, pmp, pkp, pperm, pord;  // PLOC
+  std::vector<TreeSlot> slot;  // trees of the domains a group collapse takes together
+  void reserve(int maxf, long long maxv, bool ploc, int nslots = 1) {
+    const size_t f4 = (size_t)maxf * 4 + 64;
+    for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &pin, &plf}) b->reserve(f4);
+    if (slot.size() != (size_t)std::max(1, nslots)) {  // fresh slots (no live buffer is copied)
+      slot.clear();
+      slot.resize(std::max(1, nslots));
+    }
+    for (TreeSlot& t : slot) t.reserve(maxf);
+    hist.reserve((size_t)256 * ((maxf + kRsTile - 1) / kRsTile) * 4 + 64);
     cnts.reserve((size_t)(maxv + 1) * 12 + 64);
     scal.reserve(64);
     lvl.reserve(kLevelWords * 4 + 64);
@@ -802,13 +871,21 @@ struct BuildScratch {
     }
   }
   ~BuildScratch() {
-    for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &lb, &ib, &lft, &rgt, &fst, &lst, &pin,
-                      &plf, &fa, &fb, &cnts, &scal, &adj, &rpre, &rsuf, &rtab, &lvl, &sums, &pc, &pc2,
-                      &pb, &pb2, &pnn, &pcnt,
-                      &pmf, &pkf, &pmp, &pkp, &pperm, &pord})
+    for (DevBuf* b : {&keys, &vals, &keys2, &vals2, &hist, &pin, &plf, &cnts, &scal, &adj, &rpre,
+                      &rsuf, &rtab, &lvl, &sums, &pc, &pc2, &pb, &pb2, &pnn, &pcnt, &pmf, &pkf, &pmp,
+                      &pkp, &pperm, &pord})
       b->release();
   }
 };
+
+struct BuildCache {
+  std::vector<BuildScratch> S;  // one scratch set per build pipeline
+};
+
+void buildCacheRelease(Ctx* c) {
+  delete c->build_cache;  // the scratch sets release their buffers
+  c->build_cache = nullptr;
+}
 
 void HostMeshSource::fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) {
   (void)c;
@@ -896,7 +973,17 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     P.fshade.reserve(tf * 48 + 64);
     P.nodes.reserve(tn * sizeof(BNode4));
   }
-  if (!paged) {
+  // A whole-scene LBVH build runs kBuildStreams domain pipelines side by side
+  // (domain d on stream d % kBuildStreams, each stream with its own scratch
+  // set), so the many small per-domain kernels of different domains overlap;
+  // each domain's collapse launches a fixed number of levels with no host
+  // wait, and the level counts of every domain are checked once after the
+  // last.  A source whose fills may run concurrently fills each domain in
+  // its pipeline.  PLOC and paged builds stay in one stream with a host
+  // check per domain.
+  const bool multi = !paged && c->bvh_builder == 0 && nd > 1;
+  const bool pipe_fill = multi && src.pipelined();
+  if (!paged && !pipe_fill) {
     for (int d = 0; d < nd; ++d) {
       const DomainDesc& D = P.desc[d];
       src.fill(c, d, P.vtx.as<float>() + D.vtx_off * 3, P.face.as<uint32_t>() + D.face_off * 3,
@@ -904,14 +991,6 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     }
     src.check(c);
   }
-  // A whole-scene LBVH build runs kBuildStreams domain pipelines side by side
-  // (domain d on stream d % kBuildStreams, each stream with its own scratch
-  // set), so the many small per-domain kernels of different domains overlap;
-  // each domain's collapse launches a fixed number of levels with no host
-  // wait, and the level counts of every domain are checked once after the
-  // last.  PLOC and paged builds stay in one stream with a host check per
-  // domain.
-  const bool multi = !paged && c->bvh_builder == 0 && nd > 1;
   DevBuf lvl_all;
   std::vector<int> launched(nd, 0);  // collapse levels launched per domain (multi)
   // a remesh of the same domain structure launches the previous depth + 4
@@ -922,10 +1001,10 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     DevBuf& lvl;
     ~ReleaseLvl() { lvl.release(); }
   } release_lvl{lvl_all};
-  // one domain's normals, shading records and BVH, issued on stream s
-  // levels > 0: collapse with that many level launches and no host wait
-  // (counts checked later); 0: host-checked collapse
-  auto buildOne = [&](BuildScratch& S, int d, hipStream_t s, int levels) {
+  // one domain's normals, shading records and BVH, issued on stream s, its
+  // binary tree in tree slot TS; grouped: the collapse is left to the
+  // domain's group (no host wait), else it is host-checked here
+  auto buildOne = [&](BuildScratch& S, TreeSlot& TS, int d, hipStream_t s, bool grouped) {
     DomainDesc& D = P.desc[d];
     const int nf = D.num_tris;
     const long long nv = src.numVerts(d);
@@ -938,6 +1017,9 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     }
     const float* vtx = P.vtx.as<float>() + D.vtx_off * 3;
     const uint32_t* face = P.face.as<uint32_t>() + D.face_off * 3;
+    if (pipe_fill)
+      src.fill(c, d, P.vtx.as<float>() + D.vtx_off * 3, P.face.as<uint32_t>() + D.face_off * 3,
+               P.rgb.as<uint32_t>() + D.vtx_off, s);
     // normals
     uint32_t* cnt = S.cnts.as<uint32_t>();
     uint32_t* cur = cnt + (nv + 1);  // cleared with cnt in one fill
@@ -973,7 +1055,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     }
     LTri* tris = P.tris.as<LTri>() + D.tri_off;
     BNode4* nodes = P.nodes.as<BNode4>() + D.node_off;
-    float* leafbox = S.lb.as<float>();
+    float* leafbox = TS.lb.as<float>();
     const int node_cap = nf / 2 + 16;
     // paged pool: the finished domain's image goes to pinned host memory
     // (and its normals to the host, for the normals query)
@@ -997,11 +1079,11 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       pageOut();
       return;
     }
-    int* L = S.lft.as<int>();
-    int* R = S.rgt.as<int>();
-    int* F = S.fst.as<int>();
-    int* Lst = S.lst.as<int>();
-    float* inner = S.ib.as<float>();
+    int* L = TS.lft.as<int>();
+    int* R = TS.rgt.as<int>();
+    int* F = TS.fst.as<int>();
+    int* Lst = TS.lst.as<int>();
+    float* inner = TS.ib.as<float>();
     // Collapse the binary tree under `root` into 4-wide nodes; returns the
     // number of 4-wide levels.
     // Levels are launched kLevelBatch at a time; each reads its item count
@@ -1015,14 +1097,14 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       init[2] = 1;
       SPRAY_HIP(hipMemcpyAsync(cc, init, sizeof(init), hipMemcpyHostToDevice, s));
       FrontierItem item{root, 0};
-      SPRAY_HIP(hipMemcpyAsync(S.fa.p, &item, sizeof(item), hipMemcpyHostToDevice, s));
+      SPRAY_HIP(hipMemcpyAsync(TS.fa.p, &item, sizeof(item), hipMemcpyHostToDevice, s));
       const int grid = std::min(gridFor(std::max(1, nf / 8), 256), 2048);
       int done = 0;
       for (;;) {
         const int upto = std::min(done + kLevelBatch, kMaxLevels);
         for (int l = done; l < upto; ++l) {
-          FrontierItem* fin = (l & 1) ? S.fb.as<FrontierItem>() : S.fa.as<FrontierItem>();
-          FrontierItem* fout = (l & 1) ? S.fa.as<FrontierItem>() : S.fb.as<FrontierItem>();
+          FrontierItem* fin = (l & 1) ? TS.fb.as<FrontierItem>() : TS.fa.as<FrontierItem>();
+          FrontierItem* fout = (l & 1) ? TS.fa.as<FrontierItem>() : TS.fb.as<FrontierItem>();
           kCollapse<<<grid, 256, 0, s>>>(fin, cc + 2 + l, fout, cc + 3 + l, cc, node_cap, L, R, F,
                                          Lst, leafbox, inner, nodes, cc + 1);
         }
@@ -1122,19 +1204,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       }
       kRangeBoxes<<<gridFor(nf - 1, 256), 256, 0, s>>>(nf - 1, F, Lst, leafbox, S.rpre.as<float>(),
                                                        S.rsuf.as<float>(), tab, RL, inner);
-      if (levels > 0) {
-        int* cc = lvl_all.as<int>() + (size_t)d * kLevelWords;
-        kCollapseInit<<<1, 64, 0, s>>>(cc, kLevelWords, S.fa.as<FrontierItem>(), 0);
-        const int grid = std::min(gridFor(std::max(1, nf / 8), 256), 2048);
-        for (int l = 0; l < levels; ++l) {
-          FrontierItem* fin = (l & 1) ? S.fb.as<FrontierItem>() : S.fa.as<FrontierItem>();
-          FrontierItem* fout = (l & 1) ? S.fa.as<FrontierItem>() : S.fb.as<FrontierItem>();
-          kCollapse<<<grid, 256, 0, s>>>(fin, cc + 2 + l, fout, cc + 3 + l, cc, node_cap, L, R, F, Lst,
-                                         leafbox, inner, nodes, cc + 1);
-        }
-        launched[d] = levels;
-        return;
-      }
+      if (grouped) return;  // collapsed with its group
       depth = collapse(0);
       if (!fitsStack(depth))
         throw StatusError(SPRAY_ERR_CAPACITY, "BVH too deep for the traversal stack");
@@ -1146,33 +1216,80 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
   if (!multi) {
     BuildScratch S;
     S.reserve(maxf, maxv, c->bvh_builder == 1);
-    for (int d = 0; d < nd; ++d) buildOne(S, d, s, 0);
+    for (int d = 0; d < nd; ++d) buildOne(S, S.slot[0], d, s, false);
   } else {
     const int K = std::min(kBuildStreams, nd);
-    std::vector<BuildScratch> S(K);
-    for (auto& x : S) x.reserve(maxf, maxv, false);
-    lvl_all.reserve((size_t)nd * kLevelWords * 4 + 64);
-    struct Pipes {  // the pipeline streams finish before the scratch sets go
-      std::vector<hipStream_t> st;
-      hipEvent_t ready = nullptr;
-      ~Pipes() {
-        for (hipStream_t x : st) {
-          (void)hipStreamSynchronize(x);
-          (void)hipStreamDestroy(x);
-        }
-        if (ready) (void)hipEventDestroy(ready);
-      }
-    } pipes;
-    pipes.st.assign(K, nullptr);
-    for (auto& x : pipes.st) SPRAY_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-    SPRAY_HIP(hipEventCreateWithFlags(&pipes.ready, hipEventDisableTiming));
-    SPRAY_HIP(hipEventRecord(pipes.ready, s));  // the meshes are in place
-    for (hipStream_t x : pipes.st) SPRAY_HIP(hipStreamWaitEvent(x, pipes.ready, 0));
-    for (int d = 0; d < nd; ++d) {
-      const int lv = prev_depth.empty() ? kMaxLevels : std::min(kMaxLevels, prev_depth[d] + 4);
-      buildOne(S[d % K], d, pipes.st[d % K], std::max(1, lv));
+    // each pipeline keeps the binary trees of up to G domains and collapses
+    // them together, one launch per level for the group
+    const size_t slot_b = std::max<size_t>(TreeSlot::bytesFor(maxf), 1);
+    const int G = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)kCollapseGroupMax, (size_t)((nd + K - 1) / K),
+                             kTreeSlotBudget / ((size_t)K * slot_b)}));
+    // a remesh keeps the pipelines' scratch sets for the next remesh (they
+    // only grow), so repeated remeshes allocate nothing here; other builds
+    // hold no scratch afterwards
+    std::vector<BuildScratch> local;
+    std::vector<BuildScratch>* Sv = &local;
+    if (src.rebuilds() && keep_buffers) {
+      if (!c->build_cache) c->build_cache = new BuildCache;
+      Sv = &c->build_cache->S;
+    } else {
+      buildCacheRelease(c);
     }
-    for (hipStream_t x : pipes.st) SPRAY_HIP(hipStreamSynchronize(x));
+    if ((int)Sv->size() != K) {  // fresh sets (no live buffer is copied)
+      Sv->clear();
+      Sv->resize(K);
+    }
+    std::vector<BuildScratch>& S = *Sv;
+    for (auto& x : S) x.reserve(maxf, maxv, false, G);
+    lvl_all.reserve((size_t)nd * kLevelWords * 4 + 64);
+    StreamSet pipes(K, s);  // declared after the scratch sets: joined before they go
+    std::vector<std::vector<int>> pending(K);  // domains of each pipeline's open group
+    auto flush = [&](int k) {
+      std::vector<int>& g = pending[k];
+      if (g.empty()) return;
+      CollapseGroup CG{};
+      CG.n = (int)g.size();
+      int levels = 1, bpd = 1;
+      for (int j = 0; j < CG.n; ++j) {
+        const int d = g[j];
+        const DomainDesc& D = P.desc[d];
+        TreeSlot& TS = S[k].slot[j];
+        CollapseDom& X = CG.dom[j];
+        X.left = TS.lft.as<int>();
+        X.right = TS.rgt.as<int>();
+        X.first = TS.fst.as<int>();
+        X.last = TS.lst.as<int>();
+        X.leafbox = TS.lb.as<float>();
+        X.innerbox = TS.ib.as<float>();
+        X.fa = TS.fa.as<FrontierItem>();
+        X.fb = TS.fb.as<FrontierItem>();
+        X.nodes = P.nodes.as<BNode4>() + D.node_off;
+        X.cc = lvl_all.as<int>() + (size_t)d * kLevelWords;
+        X.node_cap = D.num_tris / 2 + 16;
+        levels = std::max(levels, prev_depth.empty() ? kMaxLevels
+                                                     : std::min(kMaxLevels, prev_depth[d] + 4));
+        bpd = std::max(bpd, std::min(gridFor(std::max(1, D.num_tris / 8), 256), 2048));
+      }
+      CG.blocks_per_dom = bpd;
+      kCollapseGroupInit<<<CG.n, 64, 0, pipes[k]>>>(CG);
+      for (int l = 0; l < levels; ++l) kCollapseGroupLevel<<<CG.n * bpd, 256, 0, pipes[k]>>>(CG, l);
+      SPRAY_HIP(hipGetLastError());
+      for (int d : g) launched[d] = levels;
+      g.clear();
+    };
+    for (int d = 0; d < nd; ++d) {
+      const int k = d % K;
+      const bool grouped = P.desc[d].num_tris > kLeafMax;  // one-leaf roots are built directly
+      buildOne(S[k], S[k].slot[pending[k].size()], d, pipes[k], grouped);
+      if (grouped) {
+        pending[k].push_back(d);
+        if ((int)pending[k].size() == G) flush(k);
+      }
+    }
+    for (int k = 0; k < K; ++k) flush(k);
+    pipes.release();
+    if (pipe_fill) src.check(c);
     std::vector<int> h((size_t)nd * kLevelWords);
     SPRAY_HIP(hipMemcpy(h.data(), lvl_all.p, h.size() * 4, hipMemcpyDeviceToHost));
     std::vector<int> redo;
@@ -1193,7 +1310,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       P.desc[d].num_nodes = cc[0];
       P.bvh_depth[d] = depth;
     }
-    for (int d : redo) buildOne(S[0], d, s, 0);  // deeper than last time: host-checked levels
+    for (int d : redo) buildOne(S[0], S[0].slot[0], d, s, false);  // deeper than last time: host-checked
   }
   SPRAY_HIP(hipStreamSynchronize(s));
   long long used = 0;

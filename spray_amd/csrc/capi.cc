@@ -178,6 +178,7 @@ void spray_destroy(spray_ctx* ctx) {
   ctx->c.q_in.release();
   ctx->c.q_out.release();
   ctx->c.gen_store.release();
+  buildCacheRelease(&ctx->c);
   if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }

@@ -117,6 +117,57 @@ struct MeshSource {
   virtual void fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) = 0;
   // After the fills issued so far: throws if one of them found bad input.
   virtual void check(Ctx* c) { (void)c; }
+  // fill() of different domains may run concurrently on several streams,
+  // with no host wait inside.
+  virtual bool pipelined() const { return false; }
+  // The scene is rebuilt from this kind of source again and again
+  // (remeshing): the build keeps its scratch for the next build.
+  virtual bool rebuilds() const { return false; }
+};
+
+// Non-blocking streams that start after the work issued so far on `after`;
+// join() makes `after` wait for everything issued on them, and the
+// destructor waits for them and destroys them.
+struct StreamSet {
+  std::vector<hipStream_t> st;
+  std::vector<hipEvent_t> ev;
+  StreamSet(int k, hipStream_t after) {
+    st.assign(k, nullptr);
+    ev.assign(k, nullptr);
+    try {
+      for (int i = 0; i < k; ++i) {
+        SPRAY_HIP(hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking));
+        SPRAY_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+      }
+      SPRAY_HIP(hipEventRecord(ev[0], after));
+      for (int i = 0; i < k; ++i) SPRAY_HIP(hipStreamWaitEvent(st[i], ev[0], 0));
+    } catch (...) {
+      release();
+      throw;
+    }
+  }
+  StreamSet(const StreamSet&) = delete;
+  StreamSet& operator=(const StreamSet&) = delete;
+  int size() const { return (int)st.size(); }
+  hipStream_t operator[](int i) const { return st[i]; }
+  void join(hipStream_t after) {
+    for (size_t i = 0; i < st.size(); ++i) {
+      SPRAY_HIP(hipEventRecord(ev[i], st[i]));
+      SPRAY_HIP(hipStreamWaitEvent(after, ev[i], 0));
+    }
+  }
+  void release() {
+    for (hipStream_t x : st)
+      if (x) {
+        (void)hipStreamSynchronize(x);
+        (void)hipStreamDestroy(x);
+      }
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    st.clear();
+    ev.clear();
+  }
+  ~StreamSet() { release(); }
 };
 
 struct HostMeshSource : MeshSource {
@@ -133,6 +184,8 @@ struct Predictor;    // mlp.hip
 // Smallest sample budget of a batch of blocking tiles (the budget grows with
 // the HBM left after the scene is loaded).
 constexpr long long kBatchSamples = 8ll << 20;
+
+struct BuildCache;  // bvh.hip: build scratch kept between remeshes
 
 struct Ctx {
   spray_config cfg;
@@ -161,6 +214,7 @@ struct Ctx {
   // the generator's per-domain buffers stay allocated between remeshes
   bool generated = false;
   DevBuf gen_store;
+  BuildCache* build_cache = nullptr;
   spray_scenegen_params gen{};
   std::string gen_name, gen_extra;
 };
@@ -168,6 +222,7 @@ struct Ctx {
 // bvh.hip: builds the pool from `src` (the domains of c->scene); with
 // keep_buffers the pool's device buffers are reused where large enough
 void buildPool(Ctx* c, MeshSource& src, bool keep_buffers = false);
+void buildCacheRelease(Ctx* c);
 // mesh_gpu.hip: scene loading from PLY files (binary bodies decoded on the
 // GPU), the GPU marching-cubes generator, mesh readback and the scene writer
 void loadScenePly(Ctx* c, HostScene& scene);

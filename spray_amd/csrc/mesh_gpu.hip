@@ -724,6 +724,7 @@ __global__ void kMcTotals(const uint32_t* vtotal, const uint32_t* ftotal, uint32
 // free HBM), they are kept for none and the emission recomputes them per
 // domain into scratch.
 constexpr size_t kMcKeepBudget = (size_t)16 << 30;
+constexpr int kCountStreams = 4;  // streams of the count pass over the domains
 
 struct McSource : MeshSource {
   struct Parts {  // byte offsets of one domain's buffers
@@ -749,8 +750,11 @@ struct McSource : MeshSource {
                  al((size_t)(edgeBlocks(g) + 1) * 4), al((size_t)(cellBlocks(g) + 1) * 4)};
   }
   // sign bytes, cut bits and scanned block offsets of domain d; with bnd,
-  // also its vertex bounds, and with tot its vertex and face totals
-  void count(const McGrid& g, const Parts& at, float* bnd, uint32_t* tot, hipStream_t s) {
+  // also its vertex bounds (partials in bpart), and with tot its vertex and
+  // face totals.  sums: the scans' tile sums (scanSumsWords of the larger
+  // scan), or null for the calling thread's own buffer.
+  void count(const McGrid& g, const Parts& at, float* bnd, uint32_t* tot, hipStream_t s,
+             float* bpart = nullptr, uint32_t* sums = nullptr) {
     uint8_t* sgn = store.as<uint8_t>() + at.sgn;
     unsigned long long* bits = (unsigned long long*)(store.as<char>() + at.bits);
     uint32_t* eoff = (uint32_t*)(store.as<char>() + at.eoff);
@@ -759,15 +763,21 @@ struct McSource : MeshSource {
     const long long eb = edgeBlocks(g), cb = cellBlocks(g);
     kMcSign<<<gridOf(np, kBlk, kMcCellBlocks), kBlk, mcTabBytes(g), s>>>(g, sgn);
     kMcEdgeBits<<<gridOf(ne, kBlk), kBlk, 0, s>>>(g, sgn, bits, eoff);
-    exclusiveScan(eoff, eoff, eb + 1, s);
+    auto scan = [&](uint32_t* x, long long n) {
+      if (sums)
+        exclusiveScan(x, x, n, s, sums);
+      else
+        exclusiveScan(x, x, n, s);
+    };
+    scan(eoff, eb + 1);
     if (bnd) {
+      float* bp = bpart ? bpart : part.as<float>();
       const int vgrid = gridOf(eb * (kBlk / 64), kBlk, kBoundBlocks);
-      kMcVerts<<<vgrid, kBlk, mcTabBytes(g), s>>>(g, bits, eoff, 0u, nullptr, nullptr,
-                                                 part.as<float>());
-      kBoundsFinal<<<1, kBlk, 0, s>>>(part.as<float>(), vgrid, bnd);
+      kMcVerts<<<vgrid, kBlk, mcTabBytes(g), s>>>(g, bits, eoff, 0u, nullptr, nullptr, bp);
+      kBoundsFinal<<<1, kBlk, 0, s>>>(bp, vgrid, bnd);
     }
     kMcCellCount<<<gridOf(nc, kBlk), kBlk, 0, s>>>(g, sgn, ntri.as<uint8_t>(), coff);
-    exclusiveScan(coff, coff, cb + 1, s);
+    scan(coff, cb + 1);
     if (tot) kMcTotals<<<1, 1, 0, s>>>(eoff + eb, coff + cb, tot);
   }
   McSource(Ctx* c, const spray_scenegen_params& prm) : p(prm), store(c->gen_store) {
@@ -831,10 +841,36 @@ struct McSource : MeshSource {
     hipStream_t s = c->stream;
     SPRAY_HIP(hipMemcpyAsync(table.p, T, 256 * 25, hipMemcpyHostToDevice, s));
     SPRAY_HIP(hipMemcpyAsync(ntri.p, counts, 256, hipMemcpyHostToDevice, s));
-    // count pass: vertex and face totals and vertex bounds of every domain
+    // count pass: vertex and face totals and vertex bounds of every domain;
+    // with every domain's buffers kept, the domains run on kCountStreams
+    // streams, each with its own bound partials and scan tile sums
     uint32_t* tot = dev_out.as<uint32_t>();
     float* bnd = (float*)(tot + 2 * (size_t)nd);
-    for (int d = 0; d < nd; ++d) count(grid[d], parts[d], bnd + 6 * d, tot + 2 * d, s);
+    if (keep && nd > 1) {
+      const int K = std::min(kCountStreams, nd);
+      long long scan_max = 0;
+      for (int d = 0; d < nd; ++d)
+        scan_max = std::max(scan_max, std::max(edgeBlocks(grid[d]), cellBlocks(grid[d])) + 1);
+      const size_t part_b = align256((size_t)kBoundBlocks * 6 * 4);
+      const size_t sums_b = align256((size_t)scanSumsWords(scan_max) * 4);
+      struct Per {  // per stream: bound partials, tile sums
+        DevBuf b;
+        ~Per() { b.release(); }
+      } pb;
+      DevBuf& per = pb.b;
+      per.reserve((part_b + sums_b) * K + 64);
+      {
+        StreamSet st(K, s);
+        for (int d = 0; d < nd; ++d) {
+          char* base = per.as<char>() + (part_b + sums_b) * (d % K);
+          count(grid[d], parts[d], bnd + 6 * d, tot + 2 * d, st[d % K], (float*)base,
+                (uint32_t*)(base + part_b));
+        }
+        st.join(s);
+      }
+    } else {
+      for (int d = 0; d < nd; ++d) count(grid[d], parts[d], bnd + 6 * d, tot + 2 * d, s);
+    }
     SPRAY_HIP(hipGetLastError());
     std::vector<uint32_t> h((size_t)nd * 2);
     bounds.resize((size_t)nd * 6);
@@ -853,6 +889,9 @@ struct McSource : MeshSource {
   }
   long long numVerts(int d) const override { return nv[d]; }
   long long numFaces(int d) const override { return nf[d]; }
+  // with every domain's count buffers kept, a fill reads only its own
+  bool pipelined() const override { return keep; }
+  bool rebuilds() const override { return true; }
   void fill(Ctx* c, int d, float* v, uint32_t* f, uint32_t* rgb, hipStream_t s) override {
     (void)c;
     const McGrid& g = grid[d];
@@ -880,6 +919,7 @@ void loadScenePly(Ctx* c, HostScene& scene) {
   c->scene = std::move(scene);
   c->generated = false;
   c->gen_store.release();
+  buildCacheRelease(c);
   PlySource src(c);
   tracerRelease(c);
   buildPool(c, src);
