@@ -454,6 +454,16 @@ static long long rangeLevels(int n, RangeLevels* L, int* nlev) {
   return off;
 }
 
+// 4-wide nodes of a domain of nf triangles, at most.  In the collapsed tree
+// every inner node has two or more children and at most nf leaves; a node
+// with fewer than four children has only leaf children, so it covers more
+// than kLeafMax triangles, and those nodes are disjoint.  With N4 nodes of
+// four children and Ns of fewer: 3 N4 + Ns <= nf - 1 and Ns <= nf / (kLeafMax
+// + 1), so N4 + Ns <= (nf - 1) / 3 + 2 nf / (3 (kLeafMax + 1)).
+__host__ __device__ inline long long nodeCap(long long nf) {
+  return nf / 3 + (2 * nf) / (3 * (kLeafMax + 1)) + 16;
+}
+
 // 4-wide collapse levels launched per host wait, and at most (deeper trees
 // do not fit the traversal stack anyway); a level counter block is the node
 // counter, the overflow flag and the item count of every level
@@ -604,7 +614,6 @@ struct CollapseDom {
 struct CollapseGroup {
   CollapseDom dom[kCollapseGroupMax];
   int n;
-  int blocks_per_dom;
 };
 
 // Collapse state of each domain of the group: node counter 1 (the root),
@@ -616,12 +625,12 @@ __global__ void kCollapseGroupInit(CollapseGroup G) {
   if (threadIdx.x == 0) D.fa[0] = FrontierItem{0, 0};
 }
 
-__global__ void kCollapseGroupLevel(CollapseGroup G, int l) {
-  const int j = blockIdx.x / G.blocks_per_dom;  // uniform per block
+__global__ void kCollapseGroupLevel(CollapseGroup G, int l, int blocks_per_dom) {
+  const int j = blockIdx.x / blocks_per_dom;  // uniform per block
   const CollapseDom& D = G.dom[j];
   const FrontierItem* fin = (l & 1) ? D.fb : D.fa;
   FrontierItem* fout = (l & 1) ? D.fa : D.fb;
-  collapseLevel(blockIdx.x - j * G.blocks_per_dom, G.blocks_per_dom, fin, D.cc + 2 + l, fout,
+  collapseLevel(blockIdx.x - j * blocks_per_dom, blocks_per_dom, fin, D.cc + 2 + l, fout,
                 D.cc + 3 + l, D.cc, D.node_cap, D.left, D.right, D.first, D.last, D.leafbox,
                 D.innerbox, D.nodes, D.cc + 1);
 }
@@ -814,7 +823,7 @@ static int gridFor(long long n, int block) {
 // Scratch of one domain build, sized for the largest domain.
 constexpr int kBuildStreams = 4;  // domain pipelines of a whole-scene LBVH build
 // tree slots of all pipelines together at most this many bytes
-constexpr size_t kTreeSlotBudget = (size_t)1 << 30;
+constexpr size_t kTreeSlotBudget = (size_t)3 << 29;  // 1.5 GB
 
 // One domain's binary tree (children, leaf ranges, leaf and inner boxes)
 // and its collapse frontiers: what the collapse reads after the build.
@@ -935,10 +944,10 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     D.eta_b = hd.eta_b;
     tv += nv;
     tf += nf;
-    tn += nf / 2 + 16;  // 4-wide node bound for leaves of <= 4
+    tn += nodeCap(nf);
     maxf = std::max(maxf, (int)nf);
     maxv = std::max(maxv, nv);
-    max_nodes = std::max(max_nodes, nf / 2 + 16);
+    max_nodes = std::max(max_nodes, nodeCap(nf));
   }
   P.total_verts = tv;
   P.total_tris = tf;
@@ -959,7 +968,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     for (int d = 0; d < nd; ++d) {
       P.h_image_off[d] = (long long)bytes;
       const size_t nf = (size_t)P.desc[d].num_tris;
-      bytes += ((nf / 2 + 16) * sizeof(BNode4) + nf * (sizeof(LTri) + 48) + 255) & ~(size_t)255;
+      bytes += ((size_t)nodeCap((long long)nf) * sizeof(BNode4) + nf * (sizeof(LTri) + 48) + 255) & ~(size_t)255;
     }
     SPRAY_HIP(hipHostMalloc(&P.h_image, bytes, hipHostMallocDefault));
     P.h_nrm.resize(nd);
@@ -1056,7 +1065,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     LTri* tris = P.tris.as<LTri>() + D.tri_off;
     BNode4* nodes = P.nodes.as<BNode4>() + D.node_off;
     float* leafbox = TS.lb.as<float>();
-    const int node_cap = nf / 2 + 16;
+    const int node_cap = (int)nodeCap(nf);
     // paged pool: the finished domain's image goes to pinned host memory
     // (and its normals to the host, for the normals query)
     auto pageOut = [&]() {
@@ -1266,14 +1275,17 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
         X.fb = TS.fb.as<FrontierItem>();
         X.nodes = P.nodes.as<BNode4>() + D.node_off;
         X.cc = lvl_all.as<int>() + (size_t)d * kLevelWords;
-        X.node_cap = D.num_tris / 2 + 16;
+        X.node_cap = (int)nodeCap(D.num_tris);
         levels = std::max(levels, prev_depth.empty() ? kMaxLevels
                                                      : std::min(kMaxLevels, prev_depth[d] + 4));
         bpd = std::max(bpd, std::min(gridFor(std::max(1, D.num_tris / 8), 256), 2048));
       }
-      CG.blocks_per_dom = bpd;
       kCollapseGroupInit<<<CG.n, 64, 0, pipes[k]>>>(CG);
-      for (int l = 0; l < levels; ++l) kCollapseGroupLevel<<<CG.n * bpd, 256, 0, pipes[k]>>>(CG, l);
+      for (int l = 0; l < levels; ++l) {
+        // level l holds at most 4^l items (one root, at most four children per item)
+        const int b = l < 12 ? std::min(bpd, (int)(((1ll << (2 * l)) + 255) / 256)) : bpd;
+        kCollapseGroupLevel<<<CG.n * b, 256, 0, pipes[k]>>>(CG, l, b);
+      }
       SPRAY_HIP(hipGetLastError());
       for (int d : g) launched[d] = levels;
       g.clear();

@@ -13,7 +13,7 @@ constexpr int kHistory = 4;         // SPRAY_SPECU_HISTORY_SIZE (src/CMakeLists.
 constexpr int kListSize = 16;       // SPRAY_RAY_DOMAIN_LIST_SIZE (src/CMakeLists.txt:32)
 constexpr int kStatSlots = kListSize + 1;  // depths >= 16 share one zero-weight slot
 constexpr int kMaxDomains = 1024;  // kCount/kScatter stage 48 bytes per domain in LDS
-constexpr int kLeafMax = 4;         // triangles per BVH leaf
+constexpr int kLeafMax = 2;         // triangles per BVH leaf
 
 enum MaterialType { kDiffuse = 0, kMirror = 1, kGlass = 2, kTransmission = 3 };
 enum LightType { kPointLight = 0, kHemisphereLight = 1, kRectLight = 2 };

@@ -222,6 +222,8 @@ __global__ void kPassBegin(TraceParams P) {
   c->active = (c->r_end > c->r_beg || c->s_end > c->s_beg) ? 1u : 0u;
   if (c->active) c->passes++;
   c->vlog_count = 0;
+  // the visit log runs only while the sample log has room
+  c->log_live = (P.log_on && *P.log_count < P.log_cap) ? 1u : 0u;
   c->shade_count = 0;
   c->sh_mark = 0;
   c->s_local = c->s_mark;
@@ -1029,6 +1031,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   TravCount tc;
   TravState s;
   unsigned long long rvis = 0, svis = 0, pruned = 0;  // wave-uniform
+  const bool log_live = P.log_on && c->log_live;
   const int lane = __lane_id();
   bool active = false, exhausted = false;
   int kind = 0, dom = 0;
@@ -1109,7 +1112,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     }
     travNodePhase(s, st, tc, active, P.nodes);
     travLeafPhase(s, st, tc, active, P.tris);
-    bool upd = false;
+    bool upd = false, logged = false;
     float4 wd = make_float4(0.f, 0.f, 0.f, 0.f), hh = wd;
     if (active && travDone(s)) {
       active = false;
@@ -1121,17 +1124,27 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
         const int samid = __float_as_int(P.R.ds[ray].w);
         float hist[4] = {hh.x, hh.y, hh.z, hh.w};
         upd = s.prim >= 0 && vbufUpdate(P.vbuf, samid, hist, depth, s.tlim);
-        if (P.log_on) {  // visit log for the predictor: (ray, domain, t)
-          uint32_t i = atomicAdd(&c->vlog_count, 1u);
-          if (i < P.vlog_cap) {
-            P.vlog[(size_t)i * 3] = ray;
-            P.vlog[(size_t)i * 3 + 1] = (uint32_t)dom;
-            P.vlog[(size_t)i * 3 + 2] = __float_as_uint(s.prim >= 0 ? s.tlim : kInfF);
-          }
-        }
+        logged = log_live;
         if (upd) atomicMin((uint32_t*)&P.R.h[ray] + depth, __float_as_uint(s.tlim));
       } else if (s.prim > 0) {
         __hip_atomic_store(&P.Sx.occluded[ray], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    // visit log for the predictor, (ray, domain, t) of every finished
+    // radiance visit: one atomic per wave and round
+    if (log_live) {
+      const unsigned long long lm = __ballot(logged);
+      if (lm != 0ull) {
+        const int leader = __builtin_ctzll(lm);
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(&c->vlog_count, (uint32_t)__popcll(lm));
+        b = __builtin_amdgcn_readlane(b, leader);
+        const uint32_t i = b + (uint32_t)__popcll(lm & ((1ull << lane) - 1ull));
+        if (logged && i < P.vlog_cap) {
+          P.vlog[(size_t)i * 3] = ray;
+          P.vlog[(size_t)i * 3 + 1] = (uint32_t)dom;
+          P.vlog[(size_t)i * 3 + 2] = __float_as_uint(s.prim >= 0 ? s.tlim : kInfF);
+        }
       }
     }
     // shade records: slots come from the wave's chunk (one atomic per
@@ -1289,20 +1302,36 @@ __global__ void kWindowAdvance(TraceParams P) {
 // says whether this domain produced the ray's final closest hit.
 __global__ void kLogFinalize(TraceParams P) {
   DevCounters* c = P.ctr;
-  if (!P.log_on) return;
+  if (!P.log_on || !c->log_live) return;
   uint32_t n = min(c->vlog_count, P.vlog_cap);
   const uint32_t* vl = P.vlog;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    uint32_t r = vl[(size_t)i * 3];
-    int dom = (int)vl[(size_t)i * 3 + 1];
-    float t = __uint_as_float(vl[(size_t)i * 3 + 2]);
-    float4 wd = P.R.wd[r];
-    int depth = __float_as_int(wd.w) & 0xff;
-    float4 hh = P.R.h[r];
-    float best = depth == 0 ? hh.x : depth == 1 ? hh.y : depth == 2 ? hh.z : hh.w;
-    float label = (t != kInfF && t == best) ? 1.f : 0.f;
-    unsigned long long slot = atomicAdd(P.log_count, 1ull);
-    if (slot >= P.log_cap) continue;
+  const int lane = __lane_id();
+  // wave-uniform loop: one atomic per wave and step for the sample slots
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t r = 0;
+    int dom = 0;
+    float label = 0.f;
+    if (valid) {
+      r = vl[(size_t)i * 3];
+      dom = (int)vl[(size_t)i * 3 + 1];
+      const float t = __uint_as_float(vl[(size_t)i * 3 + 2]);
+      float4 wd = P.R.wd[r];
+      int depth = __float_as_int(wd.w) & 0xff;
+      float4 hh = P.R.h[r];
+      float best = depth == 0 ? hh.x : depth == 1 ? hh.y : depth == 2 ? hh.z : hh.w;
+      label = (t != kInfF && t == best) ? 1.f : 0.f;
+    }
+    const unsigned long long vm = __ballot(valid);
+    if (vm == 0ull) continue;
+    const int leader = __builtin_ctzll(vm);
+    unsigned long long sb = 0;
+    if (lane == leader) sb = atomicAdd(P.log_count, (unsigned long long)__popcll(vm));
+    sb = ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(sb >> 32), leader) << 32) |
+         (unsigned long long)__builtin_amdgcn_readlane((uint32_t)sb, leader);
+    const unsigned long long slot = sb + (unsigned long long)__popcll(vm & ((1ull << lane) - 1ull));
+    if (!valid || slot >= P.log_cap) continue;
     float4 op = P.R.op[r], ds = P.R.ds[r];
     float* o = P.log_buf + slot * 8;
     o[0] = op.x;

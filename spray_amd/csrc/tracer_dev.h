@@ -37,6 +37,7 @@ struct DevCounters {
   uint32_t r_mark, s_mark;              // end of the last enqueued ranges
   uint32_t q_total[2];                  // entries in this pass's queues
   uint32_t vlog_count;                  // visit-log entries of this pass
+  uint32_t log_live;                    // predictor logging on and its sample log not yet full
   uint32_t shade_count;                 // accepted hits of this pass (shade records)
   uint32_t sh_mark, s_local;            // shade records / spawned shadow rays already processed
                                         // (advanced per domain by the paged domain sweep)
