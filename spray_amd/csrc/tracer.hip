@@ -396,6 +396,9 @@ __device__ __forceinline__ int rayDomains(const float4* box, const DomGrid& G,
 // Isector::isectAll / isectWithoutCurrentDomain (ooc_isector.h:126-174) as
 // counts: entries per domain and the DomainStats score
 // (ooc_domain_stats.cc:60-88) weight max(0, 16 - position in the ray's list).
+// kLearned: the learned form's ask records are written (a separate
+// instantiation, so the other forms keep kCount's smaller register set).
+template <bool kLearned>
 __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
@@ -472,7 +475,7 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     // learned form: a radiance ray with two or more candidate domains writes
     // an ask record for the predictor into this block's region (one LDS
     // atomic per wave); the others have no phase-A bound
-    if (P.learned) {
+    if (kLearned) {
       const bool ask = valid && kind == 0 && kept >= 2;
       const unsigned long long am = __ballot(ask);
       uint32_t abase = 0;
@@ -494,7 +497,7 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
     }
   }
   __syncthreads();
-  if (P.learned && threadIdx.x == 0) P.ask_cnt[blockIdx.x] = min(*ask_n, P.ask_region);
+  if (kLearned && threadIdx.x == 0) P.ask_cnt[blockIdx.x] = min(*ask_n, P.ask_region);
   // per-block counts, [kind][domain][block]: kScatter's blocks take the same
   // rays and place their entries at the bases these counts give
   for (int j = threadIdx.x; j < 2 * nd; j += blockDim.x)
@@ -1545,8 +1548,8 @@ static void ensureState(Ctx* c, double grow) {
   int nd = c->pool.ndomains;
   if (nd > 1024) throw StatusError(SPRAY_ERR_CAPACITY, "the tracer supports at most 1024 domains");
   if (countShmem(nd) > 65536)  // many domains: kCount's LDS beyond the default limit
-    SPRAY_HIP(hipFuncSetAttribute((const void*)kCount, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)countShmem(nd)));
+    for (const void* k : {(const void*)kCount<false>, (const void*)kCount<true>})
+      SPRAY_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)countShmem(nd)));
   int depths = std::min(cfg.bounces, kHistory);
   double spec = 2.0 * grow;  // headroom for speculative re-shading
   auto cap = [](double v) { return (uint32_t)std::min<double>(3.9e9, v + 4096); };
@@ -1833,7 +1836,10 @@ static void passQueues(Ctx* c, const TraceParams& P) {
   int nd = P.ndomains;
   kPassBegin<<<1, 1, 0, s>>>(P);
   kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
-  kCount<<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+  if (P.learned)
+    kCount<true><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+  else
+    kCount<false><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
   // learned form: the predictor needs only kCount's ask records, so it runs
   // on the second stream beside the schedule and the scatter
   if (P.learned) {
