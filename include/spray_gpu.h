@@ -243,7 +243,9 @@ int spray_schedule_log(spray_ctx* ctx, int max_records);
 int spray_schedule_read(spray_ctx* ctx, int* nrec, int* meta, unsigned long long* scores, int* order);
 
 /* Learned form (new): domain-hit predictor.  Logging collects
- * (origin, direction, domain, hit) samples from plain-schedule frames;
+ * (origin, direction, domain, hit) samples from the frames traced while it
+ * is on (any schedule; the benchmark logs plain frames, and batched or
+ * learned frames for online fine-tuning) until max_samples are held;
  * training runs the MLP's own forward/backward/Adam kernels on the GPU. */
 int spray_predictor_log(spray_ctx* ctx, int enable, long long max_samples);
 long long spray_predictor_num_samples(const spray_ctx* ctx);
