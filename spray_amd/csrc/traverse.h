@@ -102,13 +102,14 @@ struct TravCount {
 };
 
 // Tests the 4 child boxes of a node against [tnear, tfar]; returns a 4-bit
-// hit mask and the entry distances.  safe: a ray parallel to a slab with
+// hit mask and the entry distances.  kSafe: a ray parallel to a slab with
 // its origin on one of the slab's planes gives 0 * inf = NaN there; that
 // axis then bounds nothing (only rays with an infinite 1/dir component
-// can produce it, so waves without such rays skip the check).
+// can produce it, so waves without such rays use the unchecked form).
+template <bool kSafe>
 __device__ __forceinline__ int nodeBoxes(const BNode4* __restrict__ nd, const float org[3],
                                          const float inv[3], float tnear, float tfar,
-                                         float tmin_o[4], int child_o[4], bool safe) {
+                                         float tmin_o[4], int child_o[4]) {
   const float4* q = (const float4*)nd;
   float4 lox = q[0], hix = q[1], loy = q[2], hiy = q[3], loz = q[4], hiz = q[5];
   int4 ch = ((const int4*)nd)[6];
@@ -128,7 +129,7 @@ __device__ __forceinline__ int nodeBoxes(const BNode4* __restrict__ nd, const fl
     float lx0 = fminf(ax, bx), lx1 = fmaxf(ax, bx);
     float ly0 = fminf(ay, by), ly1 = fmaxf(ay, by);
     float lz0 = fminf(az, bz), lz1 = fmaxf(az, bz);
-    if (safe) {  // wave-uniform
+    if (kSafe) {
       if (ax != ax || bx != bx) lx0 = -kInfF, lx1 = kInfF;
       if (ay != ay || by != by) ly0 = -kInfF, ly1 = kInfF;
       if (az != az || bz != bz) lz0 = -kInfF, lz1 = kInfF;
@@ -209,8 +210,9 @@ __device__ __forceinline__ void cswap(float& ka, int& ia, float& kb, int& ib) {
 
 // One inner node: the hit children (inner nodes and leaves) in order of
 // entry distance; the nearest becomes `cur`, the others are pushed.
+template <bool kSafe>
 __device__ __forceinline__ void travNode(TravState& s, LaneStack& st,
-                                         const BNode4* __restrict__ nodes, bool safe) {
+                                         const BNode4* __restrict__ nodes) {
   if (++s.guard > kMaxNodeVisits) {
     st.w |= 1u << 16;
     s.cur = s.pend = kStackEmpty;
@@ -219,7 +221,7 @@ __device__ __forceinline__ void travNode(TravState& s, LaneStack& st,
   float tmin[4];
   int child[4];
   const BNode4* nd = &nodes[s.node_base + s.cur];
-  int mask = nodeBoxes(nd, s.org, s.inv, s.tnear, s.tlim, tmin, child, safe);
+  int mask = nodeBoxes<kSafe>(nd, s.org, s.inv, s.tnear, s.tlim, tmin, child);
   float ks[4];
   int m = 0;
 #pragma unroll
@@ -277,15 +279,14 @@ __device__ __forceinline__ bool travTri(TravState& s, uint32_t tri, const LTri* 
 // Node phase over the wave: lanes with an inner `cur` descend until every
 // busy lane holds a postponed leaf, none can descend further, or fewer than
 // half of the wave's rays would take a step.
-__device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravCount& cnt,
-                                              bool active, const BNode4* __restrict__ nodes) {
-  // wave-uniform: NaN-checked box tests only when some ray is axis-parallel
-  const bool safe = __ballot(active && hasInfInv(s.inv)) != 0ull;
+template <bool kSafe>
+__device__ __forceinline__ void travNodeLoop(TravState& s, LaneStack& st, TravCount& cnt,
+                                             bool active, const BNode4* __restrict__ nodes) {
   while (true) {
     const bool step = active && isInnerRef(s.cur);
     cnt.nodes += (unsigned long long)__popcll(__ballot(step));
     if (step) {
-      travNode(s, st, nodes, safe);
+      travNode<kSafe>(s, st, nodes);
       if (isLeafRef(s.cur) && s.pend == kStackEmpty) {
         s.pend = s.cur;
         s.cur = st.pop();
@@ -300,6 +301,17 @@ __device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravC
     const int nmore = __popcll(__ballot(more));
     if (nmore == 0 || __ballot(hungry) == 0ull || nmore * 2 < __popcll(__ballot(active))) break;
   }
+}
+
+// NaN-checked box tests only in waves with an axis-parallel ray: two copies
+// of the loop, so that the common one carries no NaN handling (whose merges
+// would also cost a canonicalisation before every min/max).
+__device__ __forceinline__ void travNodePhase(TravState& s, LaneStack& st, TravCount& cnt,
+                                              bool active, const BNode4* __restrict__ nodes) {
+  if (__ballot(active && hasInfInv(s.inv)) != 0ull)
+    travNodeLoop<true>(s, st, cnt, active, nodes);
+  else
+    travNodeLoop<false>(s, st, cnt, active, nodes);
 }
 
 // Leaf phase over the wave, one triangle per lane and step: each lane
