@@ -593,7 +593,6 @@ __device__ __forceinline__ void scatterDomains(const float4* box, int nd, const 
       const uint32_t dst = lb_kind[dd] + atomicAdd(&lc_kind[dd], 1u);
       if (dst < P.q_cap[kind]) {
         P.q[kind][dst] = QEntry{r, tmin};
-        P.qdom[kind][dst] = (uint16_t)dd;
       }
     }
   }
@@ -1041,6 +1040,10 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   uint32_t ray = 0;
   // wave-uniform: entries left in this wave's range; shade slots left in its chunk
   uint32_t r_cur = 0, r_end = 0, sh_cur = 0, sh_end = 0;
+  // per lane: the last queue segment met, [cur_s, cur_e) of queue cur_k - 1
+  // (0: none), and its domain
+  uint32_t cur_s = 0, cur_e = 0;
+  int cur_k = 0, cur_d = 0;
   // small launches keep ranges at one refill's worth so every wave gets work
   const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
   const uint32_t range = (unsigned long long)total >= (unsigned long long)nwaves * 1024u ? kRange : 64u;
@@ -1075,7 +1078,31 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
                              : e >= nr + ns ? P.qb[e - nr - ns]
                                             : (rlist ? rlist[e] : rbase + e);
             qe = P.q[kind][q];
-            dom = plain_dom >= 0 ? plain_dom : (int)P.qdom[kind][q];
+            if (plain_dom >= 0) {
+              dom = plain_dom;
+            } else {
+              // The entry's domain: queues are laid out domain by domain in
+              // schedule order, so it is the segment of P.off that holds q.
+              // A lane's positions rise within a queue, so it keeps the last
+              // segment it met (cur_k: kind + 1, 0 none) and searches only
+              // for an entry beyond it.
+              if (cur_k != kind + 1 || q < cur_s || q >= cur_e) {
+                const uint32_t* of = P.off + kind * (nd + 1);
+                int rk = 0, hi = nd - 1;
+                while (rk < hi) {  // the last segment starting at or before q
+                  const int mid = (rk + hi + 1) >> 1;
+                  if (of[mid] <= q)
+                    rk = mid;
+                  else
+                    hi = mid - 1;
+                }
+                cur_k = kind + 1;
+                cur_s = of[rk];
+                cur_e = of[rk + 1];
+                cur_d = P.order[rk];
+              }
+              dom = cur_d;
+            }
           }
           ray = qe.ray;
           const RayPool& pool = kind ? P.S : P.R;
@@ -1355,7 +1382,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0;
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
-      wctr, score, order, q[2], qdom[2], ask, ask_cnt, rkey, dmask, dbox, ovf, vlog, log, sh_rec, qa,
+      wctr, score, order, q[2], ask, ask_cnt, rkey, dmask, dbox, ovf, vlog, log, sh_rec, qa,
       qb, dflag, acnt;
   unsigned long long log_cap = 0;
   bool logging = false;
@@ -1386,7 +1413,7 @@ struct TracerState {
     for (const DevBuf* x : {R, Pn, Sh})
       for (int k = 0; k < 4; ++k) b += x[k].bytes;
     for (const DevBuf* x : {&s_tmax, &s_occ, &vbuf, &image, &ctr, &tile_rec, &cnt, &off, &dbase, &bcnt,
-                            &bbase, &wctr, &score, &order, &q[0], &q[1], &qdom[0], &qdom[1], &ask, &ask_cnt, &rkey,
+                            &bbase, &wctr, &score, &order, &q[0], &q[1], &ask, &ask_cnt, &rkey,
                             &dmask, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb, &dflag, &acnt})
       b += x->bytes;
     return b;
@@ -1460,7 +1487,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->qdom[0], &T->qdom[1], &T->ask, &T->ask_cnt, &T->rkey, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb, &T->dflag, &T->acnt})
+                    &T->q[1], &T->ask, &T->ask_cnt, &T->rkey, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb, &T->dflag, &T->acnt})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1518,7 +1545,8 @@ static double stateBytesPerSample(const Ctx* c) {
   const double per_ray = std::min(nd, 12), words = (nd + 63) / 64;
   const double r = 1.0 + spec * depths, s = spv * spec * depths, p = spec;
   const double q0 = spec * per_ray, q1 = spv * spec * per_ray * 0.5;
-  return r * 64 + s * 72 + p * 64 + q0 * (8 + 2 + 8) + std::min(q0, r) * kShadeRec * 16 + q1 * 10 +
+  // radiance queue: entry + phase-list index; shadow queue: entry
+  return r * 64 + s * 72 + p * 64 + q0 * (8 + 8) + std::min(q0, r) * kShadeRec * 16 + q1 * 8 +
          (r + s) * 8 * words + 32 + r;
 }
 
@@ -1604,8 +1632,6 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->order, (size_t)nd * 4 + 64);
   need(T, T->q[0], (size_t)T->q_cap[0] * sizeof(QEntry));
   need(T, T->q[1], (size_t)T->q_cap[1] * sizeof(QEntry));
-  need(T, T->qdom[0], (size_t)T->q_cap[0] * 2 + 64);
-  need(T, T->qdom[1], (size_t)T->q_cap[1] * 2 + 64);
   T->h_cnt.resize(2 * nd);
   T->h_order.resize(nd);
   need(T, T->image, (size_t)cfg.image_w * cfg.image_h * 16);
@@ -1698,8 +1724,6 @@ static TraceParams makeParams(Ctx* c) {
   P.order = T->order.as<int>();
   P.q[0] = T->q[0].as<QEntry>();
   P.q[1] = T->q[1].as<QEntry>();
-  P.qdom[0] = T->qdom[0].as<uint16_t>();
-  P.qdom[1] = T->qdom[1].as<uint16_t>();
   P.ask = T->ask.as<float4>();
   P.ask_cnt = T->ask_cnt.as<uint32_t>();
   P.ask_regions = kQueueGrid;

@@ -89,7 +89,6 @@ struct TraceParams {
   unsigned long long* score;  // [nd]
   int* order;         // [nd] schedule
   QEntry* q[2];       // queues (radiance, shadow)
-  uint16_t* qdom[2];  // domain of each queue entry
   // learned form: one ask record (3 float4) per radiance ray of the pass with
   // two or more candidate domains, written by kCount into its block's region
   // (block b's records at [b * ask_region, b * ask_region + ask_cnt[b])):
