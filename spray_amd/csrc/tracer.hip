@@ -94,6 +94,14 @@ constexpr int kShadeBlock = 512;  // threads per shading block (one pool atomic 
 // atomic per range, not per refill); the ranges still follow queue order,
 // so the waves of a launch work through the domains together.
 constexpr uint32_t kRange = 256;
+// The persistent launches hand out ranges through kWorkCounters counters
+// (a wave starts at its block's, block index mod kWorkCounters, which is its
+// XCD under round-robin dispatch), kWorkStride words apart: device-scope
+// atomics on one address are served one at a time, and one counter for a
+// launch's 250k ranges kept that address busy much of the launch.
+constexpr int kWorkCounters = 8;
+constexpr int kWorkStride = 1024;
+static_assert((kWorkCounters & (kWorkCounters - 1)) == 0, "a wave steps through the counters mod K");
 // Shade slots are taken kShadeChunk at a time per wave.
 constexpr uint32_t kShadeChunk = 64;
 static_assert(kQueueGrid % kBlock == 0, "kBlockBases splits the grid evenly over its threads");
@@ -241,6 +249,8 @@ __global__ void kClearQ(TraceParams P) {
   }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd + 3; i += gridDim.x * blockDim.x)
     P.wctr[i] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * kWorkCounters; i += gridDim.x * blockDim.x)
+    P.wq[i * kWorkStride] = 0;
 }
 
 // Ray of the current pass: index into the radiance range, then the shadow range.
@@ -983,6 +993,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   uint32_t nr = 0, ns = 0, nb = 0, rbase = 0, sbase = 0, local_base = 0;
   const uint32_t* rlist = nullptr;
   uint32_t* wc;
+  uint32_t* wq = nullptr;  // persistent launches: their kWorkCounters counters
   const bool local = plain_dom == -2;
   if (plain_dom >= 0) {
     rbase = P.dbase[plain_dom];
@@ -996,15 +1007,18 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     sbase = P.rng[2];
     ns = P.rng[3];
     wc = &P.wctr[nd];
+    wq = P.wq;
   } else if (local) {
     local_base = c->s_local;
     uint32_t end = min(c->s_count, P.s_cap);
     ns = end > local_base ? end - local_base : 0u;
     wc = &P.wctr[nd + 2];
+    wq = P.wq + 2 * kWorkCounters * kWorkStride;
   } else if (!P.learned) {
     nr = c->q_total[0];
     ns = c->q_total[1];
     wc = &P.wctr[nd];
+    wq = P.wq;
   } else {
     // learned: the phase-A radiance list, the shadow queues, then the
     // deferred list, filtered against the histories phase A updated
@@ -1021,6 +1035,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
       rlist = P.qb;
     }
     wc = &P.wctr[nd + phase];
+    wq = P.wq + phase * kWorkCounters * kWorkStride;
   }
   // bounded by the queue capacities even after an overflow
   if (!local) {
@@ -1045,8 +1060,21 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
   uint32_t cur_s = 0, cur_e = 0;
   int cur_k = 0, cur_d = 0;
   // small launches keep ranges at one refill's worth so every wave gets work
-  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t wpb = blockDim.x >> 6, nwaves = gridDim.x * wpb;
   const uint32_t range = (unsigned long long)total >= (unsigned long long)nwaves * 1024u ? kRange : 64u;
+  // Each wave of the blocks the GPU holds at once takes its first range at a
+  // fixed place (wave w: [w * range, (w + 1) * range)), so a launch starts
+  // without a burst of atomics on one counter; the counter hands out the
+  // entries after those ranges, from dyn0 on.
+  const uint32_t sblocks = min(gridDim.x, (uint32_t)max(P.trav_static, 0));
+  const uint32_t dyn0 = sblocks * wpb * range;
+  bool first = blockIdx.x < sblocks;
+  // split counters: counter k hands out the ranges k, k + K, k + 2K, ... of
+  // the entries from dyn0 on, so all of them move through the queue together;
+  // a wave takes from kc, its block's at first, and moves to the next once
+  // kc is spent (a spent counter stays spent), until all K are
+  const uint32_t nranges = total > dyn0 ? (total - dyn0 + range - 1) / range : 0u;
+  int kc = (int)(blockIdx.x % kWorkCounters), kspent = 0;
   // one launch per domain (plain) takes exactly the slots it fills
   const uint32_t sh_chunk = plain_dom >= 0 ? 0u : kShadeChunk;
   while (true) {
@@ -1054,12 +1082,32 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
     int nidle = __popcll(idle);
     if (!exhausted && (nidle >= P.refill || nidle == 64)) {
       if (r_cur >= r_end) {
-        const int leader = __builtin_ctzll(idle);
-        uint32_t b = 0;
-        if (lane == leader) b = atomicAdd(wc, range);
-        b = __builtin_amdgcn_readlane(b, leader);
+        uint32_t b;
+        if (first) {
+          first = false;
+          b = (blockIdx.x * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * range;
+        } else if (wq) {
+          const int leader = __builtin_ctzll(idle);
+          uint32_t j = nranges;
+          while (kspent < kWorkCounters) {
+            uint32_t i = 0;
+            if (lane == leader) i = atomicAdd(wq + kc * kWorkStride, 1u);
+            j = (uint32_t)kc + kWorkCounters * __builtin_amdgcn_readfirstlane(
+                                                   __builtin_amdgcn_readlane(i, leader));
+            if (j < nranges) break;
+            kc = (kc + 1) & (kWorkCounters - 1);
+            ++kspent;
+          }
+          b = j < nranges ? dyn0 + j * range : total;
+        } else {
+          const int leader = __builtin_ctzll(idle);
+          uint32_t a = 0;
+          if (lane == leader) a = atomicAdd(wc, range);
+          a = __builtin_amdgcn_readlane(a, leader);
+          b = a < total - min(total, dyn0) ? dyn0 + a : total;
+        }
         r_cur = min(b, total);
-        r_end = min(b + range, total);
+        r_end = total - r_cur > range ? r_cur + range : total;
         if (r_cur >= total) exhausted = true;
       }
       const uint32_t base = r_cur, limit = r_end;
@@ -1268,6 +1316,10 @@ __global__ void kDomainDone(TraceParams P) {
   c->s_local = min(c->s_count, P.s_cap);
   P.wctr[P.ndomains] = 0;
   P.wctr[P.ndomains + 2] = 0;
+  for (int k = 0; k < kWorkCounters; ++k) {
+    P.wq[k * kWorkStride] = 0;
+    P.wq[(2 * kWorkCounters + k) * kWorkStride] = 0;
+  }
 }
 
 // Retire (ooc_tcontext.inl:123-135): shadow rays tested in this pass.
@@ -1382,12 +1434,14 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0;
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
-      wctr, score, order, q[2], ask, ask_cnt, rkey, dmask, dbox, ovf, vlog, log, sh_rec, qa,
+      wctr, wq, score, order, q[2], ask, ask_cnt, rkey, dmask, dbox, ovf, vlog, log, sh_rec, qa,
       qb, dflag, acnt;
   unsigned long long log_cap = 0;
   bool logging = false;
   DevBuf log_ctr;
   int grid_persistent = 0;
+  int num_cu = 0;
+  int trav_res_block = 0, trav_res_blocks = 0;  // block size -> resident kTraverse blocks
   // captured tile graphs by key (schedule, windows, batch parameters,
   // logging, configuration version).  A graph whose buffers were reallocated
   // is retired, never launched again, and destroyed only when the tracer is
@@ -1413,7 +1467,7 @@ struct TracerState {
     for (const DevBuf* x : {R, Pn, Sh})
       for (int k = 0; k < 4; ++k) b += x[k].bytes;
     for (const DevBuf* x : {&s_tmax, &s_occ, &vbuf, &image, &ctr, &tile_rec, &cnt, &off, &dbase, &bcnt,
-                            &bbase, &wctr, &score, &order, &q[0], &q[1], &ask, &ask_cnt, &rkey,
+                            &bbase, &wctr, &wq, &score, &order, &q[0], &q[1], &ask, &ask_cnt, &rkey,
                             &dmask, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb, &dflag, &acnt})
       b += x->bytes;
     return b;
@@ -1486,7 +1540,7 @@ void tracerRelease(Ctx* c) {
   for (auto* arr : {T->R, T->Pn, T->Sh})
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
-                    &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->score, &T->order, &T->q[0],
+                    &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->wq, &T->score, &T->order, &T->q[0],
                     &T->q[1], &T->ask, &T->ask_cnt, &T->rkey, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb, &T->dflag, &T->acnt})
     b->release();
   delete T;
@@ -1618,6 +1672,7 @@ static void ensureState(Ctx* c, double grow) {
   need(T, T->bcnt, (size_t)2 * nd * kQueueGrid * 4 + 64);
   need(T, T->bbase, (size_t)2 * nd * kQueueGrid * 4 + 64);
   need(T, T->wctr, (size_t)(nd + 3) * 4 + 64);
+  need(T, T->wq, (size_t)3 * kWorkCounters * kWorkStride * 4);
   need(T, T->qa, (size_t)T->q_cap[0] * 4);
   need(T, T->qb, (size_t)T->q_cap[0] * 4);
   if (cfg.schedule == SPRAY_SCHEDULE_LEARNED) {
@@ -1641,6 +1696,7 @@ static void ensureState(Ctx* c, double grow) {
   int block = 64 * std::max(1, c->batch_tile);
   int per_cu = std::max(1, 1536 / block);
   T->grid_persistent = prop.multiProcessorCount * per_cu;
+  T->num_cu = prop.multiProcessorCount;
 }
 
 static TraceParams makeParams(Ctx* c) {
@@ -1720,6 +1776,7 @@ static TraceParams makeParams(Ctx* c) {
   P.bcnt = T->bcnt.as<uint32_t>();
   P.bbase = T->bbase.as<uint32_t>();
   P.wctr = T->wctr.as<uint32_t>();
+  P.wq = T->wq.as<uint32_t>();
   P.score = T->score.as<unsigned long long>();
   P.order = T->order.as<int>();
   P.q[0] = T->q[0].as<QEntry>();
@@ -1854,6 +1911,18 @@ static void ensureOverflow(TracerState* T, size_t lanes) { need(T, T->ovf, lanes
 
 static size_t travShmem(int block) { return (size_t)kRing * block * 4; }
 
+// Blocks of kTraverse with `block` threads that the GPU holds at once.
+static int travResidentBlocks(Ctx* c, int block) {
+  TracerState* T = c->tracer;
+  if (T->trav_res_block != block) {
+    int nb = 0;
+    SPRAY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kTraverse, block, travShmem(block)));
+    T->trav_res_blocks = std::max(1, nb) * std::max(1, T->num_cu);
+    T->trav_res_block = block;
+  }
+  return T->trav_res_blocks;
+}
+
 static void passQueues(Ctx* c, const TraceParams& P) {
   hipStream_t s = c->stream;
   TracerState* T = c->tracer;
@@ -1890,6 +1959,7 @@ static void passShade(Ctx* c, const TraceParams& P) {
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
   int block = 64 * std::max(1, c->batch_tile);
+  Q.trav_static = travResidentBlocks(c, block);
   int ei = c->profile ? T->evBegin(s) : -1;
   kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -2, 0);
   if (ei >= 0) T->evEnd(ei, s);
@@ -1935,6 +2005,7 @@ static bool passPlain(Ctx* c, const TraceParams& P) {
   ensureOverflow(T, (size_t)max_grid * kTravBlock);
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
+  Q.trav_static = travResidentBlocks(c, kTravBlock);
   std::vector<int> doms;  // domains with entries, in schedule order
   for (int k = 0; k < nd; ++k) {
     int d = T->h_order[k];
@@ -2014,6 +2085,7 @@ static bool passPagedBatched(Ctx* c, const TraceParams& P) {
   const int block = 64 * std::max(1, c->batch_tile);
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
+  Q.trav_static = travResidentBlocks(c, block);
   for (size_t j = 0; j < chunks.size(); ++j) {
     const std::vector<int> cur = chunkDomains(chunks[j]);
     for (int d : cur) {
@@ -2052,6 +2124,7 @@ static void passBatched(Ctx* c, const TraceParams& P) {
   passQueues(c, P);
   TraceParams Q = P;
   Q.ovf = T->ovf.as<uint32_t>();
+  Q.trav_static = travResidentBlocks(c, block);
   int ei = c->profile ? T->evBegin(s) : -1;
   kTraverse<<<T->grid_persistent, block, travShmem(block), s>>>(Q, -1, 0);
   if (ei >= 0) T->evEnd(ei, s);

@@ -86,6 +86,7 @@ struct TraceParams {
   uint32_t* bcnt;     // [2][nd][queue grid] entries per block (kCount)
   uint32_t* bbase;    // [2][nd][queue grid] queue base per block (kScatter)
   uint32_t* wctr;     // [nd + 3] work counters: per domain (plain), phase A / B, local shadows
+  uint32_t* wq;       // [3][kWorkCounters] split counters of the persistent launches, kWorkStride apart
   unsigned long long* score;  // [nd]
   int* order;         // [nd] schedule
   QEntry* q[2];       // queues (radiance, shadow)
@@ -106,6 +107,7 @@ struct TraceParams {
   unsigned long long* dmask;  // [pass ray][(nd + 63) / 64]: domains hit (kCount -> kScatter)
   uint32_t q_cap[2];
   int refill;         // a wave refetches work once this many lanes are idle
+  int trav_static;    // blocks of the trace kernel resident at once: their waves take a first range
   uint32_t* ovf;      // traversal stack overflow areas
   // shade records of the pass (accepted hits), SoA
   // shade records, kShadeRec float4 each: the ray's op, ds, wd (w: depth |
