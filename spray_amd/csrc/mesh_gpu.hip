@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "ctx.h"
@@ -46,6 +47,16 @@ namespace {
 int gridOf(long long n, int block, long long cap = 65535) {
   long long g = (n + block - 1) / block;
   return (int)std::max<long long>(1, std::min<long long>(g, cap));
+}
+
+// Upper bound on kMcVerts' grid.  The environment variable
+// SPRAY_MC_VERTS_BLOCKS (a positive block count) lowers it, so a test can run
+// the path where each wave takes several groups of words; unset, only the
+// launch's own cap applies.
+long long mcVertsCap(long long cap) {
+  const char* e = std::getenv("SPRAY_MC_VERTS_BLOCKS");
+  const long long v = e ? std::atoll(e) : 0;
+  return v >= 1 ? std::min(v, cap) : cap;
 }
 
 // ---------------------------------------------------------------------------
@@ -772,7 +783,7 @@ struct McSource : MeshSource {
     scan(eoff, eb + 1);
     if (bnd) {
       float* bp = bpart ? bpart : part.as<float>();
-      const int vgrid = gridOf(eb * (kBlk / 64), kBlk, kBoundBlocks);
+      const int vgrid = gridOf(eb * (kBlk / 64), kBlk, mcVertsCap(kBoundBlocks));
       kMcVerts<<<vgrid, kBlk, mcTabBytes(g), s>>>(g, bits, eoff, 0u, nullptr, nullptr, bp);
       kBoundsFinal<<<1, kBlk, 0, s>>>(bp, vgrid, bnd);
     }
@@ -902,7 +913,7 @@ struct McSource : MeshSource {
     const uint32_t* eoff = (const uint32_t*)(store.as<char>() + at.eoff);
     const uint32_t* coff = (const uint32_t*)(store.as<char>() + at.coff);
     const long long ne = 3 * g.E, nc = (long long)g.n * g.n * g.n;
-    kMcVerts<<<gridOf(edgeBlocks(g) * (kBlk / 64), kBlk), kBlk, mcTabBytes(g), s>>>(
+    kMcVerts<<<gridOf(edgeBlocks(g) * (kBlk / 64), kBlk, mcVertsCap(65535)), kBlk, mcTabBytes(g), s>>>(
         g, bits, eoff, p.rgb, v, rgb, nullptr);
     kMcCells<<<gridOf(nc, kBlk, kMcCellBlocks), kBlk, 0, s>>>(g, sgn, table.as<int8_t>(),
                                                              ntri.as<uint8_t>(), ec, bits, eoff,

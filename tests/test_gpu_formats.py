@@ -265,6 +265,25 @@ def test_gpu_generator_matches_cpu_generator(tmp_path, gen):
     assert float(np.sqrt((err ** 2).mean())) <= 1e-6 and float(err.max()) <= 1e-4
 
 
+@pytest.mark.parametrize("gi,blocks", [(0, 1), (0, 3), (2, 2)], ids=["lattice-1", "lattice-3", "fine-2"])
+def test_mc_verts_small_grid(tmp_path, monkeypatch, gi, blocks):
+    """kMcVerts on a grid of a few blocks (SPRAY_MC_VERTS_BLOCKS): each wave
+    walks several groups of cut-mask words, reusing its LDS slots, in the
+    bounds pass and the vertex pass, and the meshes, bounds and counts are
+    still the CPU generator's, before and after a remesh."""
+    gen = GEN[gi]
+    p00 = _cpu_scene(str(tmp_path), "p00", gen, phase=0.0)
+    p05 = _cpu_scene(str(tmp_path), "p05", gen, phase=0.05)
+    monkeypatch.setenv("SPRAY_MC_VERTS_BLOCKS", str(blocks))
+    with _ctx(schedule=1) as a, _ctx(schedule=1) as b, _ctx(schedule=1) as c:
+        _gen(a, "s", gen)
+        b.load_scene(p00, str(tmp_path))
+        _same_scene(a, b)
+        a.remesh(0.05)
+        c.load_scene(p05, str(tmp_path))
+        _same_scene(a, c)
+
+
 def test_remesh_in_place(tmp_path):
     """Remeshing at a new phase gives the scene generated at that phase
     (meshes, bounds, normals, frame), after a frame whose pass graphs were
