@@ -280,6 +280,63 @@ def test_axis_parallel_rays(ctx_c1, ref_c1):
         assert np.array_equal(occ.astype(bool), rp >= 0), dom
 
 
+def test_far_origins_at_vertices_and_edges(ctx_c1, ref_c1):
+    """Rays from origins 2 to 4000 units away (so |org / dir| is large and
+    the fused box test's pad carries the bound), aimed exactly at mesh
+    vertices, at edge midpoints and at points one float step off them, and
+    rays skimming the domain's box faces: closest hits and any hits equal the
+    oracle's, bit for bit."""
+    rng = np.random.default_rng(77)
+    for dom in (0, 21, 63):
+        v, f, _, _ = ref_c1.mesh(dom)
+        v = v.reshape(-1, 3).astype(np.float64)
+        f = f.reshape(-1, 3)
+        nv, nf = 4000, 4000
+        tv = v[rng.integers(0, len(v), nv)]
+        fe = f[rng.integers(0, len(f), nf)]
+        k = rng.integers(0, 3, nf)
+        te = 0.5 * (v[fe[np.arange(nf), k]] + v[fe[np.arange(nf), (k + 1) % 3]])
+        tgt = np.concatenate([tv, te]).astype(np.float32)
+        off = np.nextafter(tgt, np.float32(np.inf))  # one float step off
+        tgt = np.concatenate([tgt, off])
+        n = len(tgt)
+        u = rng.normal(size=(n, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        dist = np.array([2.0, 60.0, 4000.0])[rng.integers(0, 3, n)]
+        org = (tgt.astype(np.float64) + u * dist[:, None]).astype(np.float32)
+        d = tgt.astype(np.float64) - org.astype(np.float64)
+        dirs = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        # skimming rays: along a face of the domain box, just inside it
+        _, _, lo, hi, _ = ref_c1.domain(dom)
+        lo, hi = lo.astype(np.float64), hi.astype(np.float64)
+        m = 4000
+        so = rng.uniform(lo, hi, (m, 3))
+        ax = rng.integers(0, 3, m)
+        so[np.arange(m), ax] = np.where(rng.random(m) < 0.5, lo[ax], hi[ax])
+        sd = rng.normal(size=(m, 3))
+        sd[np.arange(m), ax] = rng.normal(size=m) * 1e-4
+        sd /= np.linalg.norm(sd, axis=1, keepdims=True)
+        so = so - sd * 50.0
+        org = np.concatenate([org, so.astype(np.float32)])
+        dirs = np.concatenate([dirs, sd.astype(np.float32)])
+        t, uu, vv, p = ctx_c1.trace_rays(org, dirs, dom)
+        rt, ru, rv, rp = ref_c1.closest(dom, org, dirs)
+        # where the two traversals differ, and on a sample, brute force over
+        # every triangle of the domain decides (it depends on no box test)
+        check = np.flatnonzero((p != rp) | (t != rt))
+        check = np.union1d(check, rng.choice(len(p), 300, replace=False))
+        bt, bu, bv, bp = ref_c1.closest(dom, org[check], dirs[check], brute=True)
+        print(f"domain {dom}: {(p >= 0).sum()} hits of {len(p)}, {int(((p != rp) | (t != rt)).sum())} "
+              f"differ from the oracle's traversal")
+        assert (p >= 0).sum() > len(p) // 4
+        assert np.array_equal(p[check], bp) and np.array_equal(t[check], bt), dom
+        assert np.array_equal(uu[check], bu) and np.array_equal(vv[check], bv), dom
+        assert (p == rp).mean() >= 0.9999
+        occ = ctx_c1.trace_rays(org, dirs, dom, mode=1)
+        assert np.array_equal(occ[check], bp >= 0), dom
+        assert (occ == (rp >= 0)).mean() >= 0.9999
+
+
 @pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "batched"])
 def test_domain_grid_frame_matches_oracle(tmp_path, schedule):
     """512 domains (8x8x8), the largest scene among the parity tests: the
