@@ -78,9 +78,12 @@ __global__ void kVertNormal(const float* __restrict__ vtx, const uint32_t* __res
 // Shading record per face: the three vertex normals and colours of the
 // face in one 48-byte record, so shading reads one record per hit instead
 // of the face and then six vertex attributes.
+// It also sets the centroid bounds (ordered bits, see kCentroidBounds) to
+// their starting values for the Morton pass that follows it on the stream.
 __global__ void kFaceShade(const uint32_t* __restrict__ face, long long nf,
                            const float* __restrict__ nrm, const uint32_t* __restrict__ rgb,
-                           float4* out) {
+                           float4* out, uint32_t* cbounds) {
+  if (blockIdx.x == 0 && threadIdx.x < 6) cbounds[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nf;
        i += (long long)gridDim.x * blockDim.x) {
     for (int k = 0; k < 3; ++k) {
@@ -636,11 +639,6 @@ __global__ void kCollapseGroupLevel(CollapseGroup G, int l, int blocks_per_dom) 
 }
 
 
-// Centroid bounds before their atomic min / max (as ordered bits).
-__global__ void kBoundsInit(uint32_t* b) {
-  if (threadIdx.x < 6) b[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
-}
-
 // Root for tiny domains (<= kLeafMax triangles): one leaf child.
 __global__ void kTinyRoot(int nf, const float* __restrict__ leafbox, BNode4* nodes) {
   BNode4 nd;
@@ -982,9 +980,9 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     P.fshade.reserve(tf * 48 + 64);
     P.nodes.reserve(tn * sizeof(BNode4));
   }
-  // A whole-scene LBVH build runs kBuildStreams domain pipelines side by side
-  // (domain d on stream d % kBuildStreams, each stream with its own scratch
-  // set), so the many small per-domain kernels of different domains overlap;
+  // A whole-scene LBVH build runs up to kBuildStreams domain pipelines side
+  // by side, one per free hardware queue (domain d on stream d % K, each
+  // stream with its own scratch set), so the many small per-domain kernels of different domains overlap;
   // each domain's collapse launches a fixed number of levels with no host
   // wait, and the level counts of every domain are checked once after the
   // last.  A source whose fills may run concurrently fills each domain in
@@ -1040,12 +1038,11 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
                                                               S.adj.as<uint32_t>());
     kVertNormal<<<gridFor(nv, 256), 256, 0, s>>>(vtx, face, nv, off, cnt, S.adj.as<uint32_t>(),
                                                   P.nrm.as<float>() + D.vtx_off * 3);
+    // Morton codes (kFaceShade sets the centroid bounds' starting values)
+    uint32_t* bmm = S.scal.as<uint32_t>();
     kFaceShade<<<gridFor(nf, 256), 256, 0, s>>>(face, nf, P.nrm.as<float>() + D.vtx_off * 3,
                                                  P.rgb.as<uint32_t>() + D.vtx_off,
-                                                 P.fshade.as<float4>() + D.shade_off * 3);
-    // Morton codes
-    uint32_t* bmm = S.scal.as<uint32_t>();
-    kBoundsInit<<<1, 64, 0, s>>>(bmm);
+                                                 P.fshade.as<float4>() + D.shade_off * 3, bmm);
     kCentroidBounds<<<std::min(gridFor(nf, 256), kBoundsGrid), 256, 0, s>>>(vtx, face, nf, bmm,
                                                                             bmm + 3);
     kMorton<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, bmm, bmm + 3, S.keys.as<uint32_t>(),
@@ -1227,7 +1224,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     S.reserve(maxf, maxv, c->bvh_builder == 1);
     for (int d = 0; d < nd; ++d) buildOne(S, S.slot[0], d, s, false);
   } else {
-    const int K = std::min(kBuildStreams, nd);
+    const int K = std::min(pipelineCount(kBuildStreams), nd);
     // each pipeline keeps the binary trees of up to G domains and collapses
     // them together, one launch per level for the group
     const size_t slot_b = std::max<size_t>(TreeSlot::bytesFor(maxf), 1);
@@ -1252,7 +1249,10 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     std::vector<BuildScratch>& S = *Sv;
     for (auto& x : S) x.reserve(maxf, maxv, false, G);
     lvl_all.reserve((size_t)nd * kLevelWords * 4 + 64);
-    StreamSet pipes(K, s);  // declared after the scratch sets: joined before they go
+    // pipeline 0 on s, the others on their own streams (declared after the
+    // scratch sets: joined before they go)
+    StreamSet side(K - 1, s);
+    auto pipe = [&](int k) { return k == 0 ? s : side[k - 1]; };
     std::vector<std::vector<int>> pending(K);  // domains of each pipeline's open group
     auto flush = [&](int k) {
       std::vector<int>& g = pending[k];
@@ -1280,11 +1280,11 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
                                                      : std::min(kMaxLevels, prev_depth[d] + 4));
         bpd = std::max(bpd, std::min(gridFor(std::max(1, D.num_tris / 8), 256), 2048));
       }
-      kCollapseGroupInit<<<CG.n, 64, 0, pipes[k]>>>(CG);
+      kCollapseGroupInit<<<CG.n, 64, 0, pipe(k)>>>(CG);
       for (int l = 0; l < levels; ++l) {
         // level l holds at most 4^l items (one root, at most four children per item)
         const int b = l < 12 ? std::min(bpd, (int)(((1ll << (2 * l)) + 255) / 256)) : bpd;
-        kCollapseGroupLevel<<<CG.n * b, 256, 0, pipes[k]>>>(CG, l, b);
+        kCollapseGroupLevel<<<CG.n * b, 256, 0, pipe(k)>>>(CG, l, b);
       }
       SPRAY_HIP(hipGetLastError());
       for (int d : g) launched[d] = levels;
@@ -1293,14 +1293,15 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
     for (int d = 0; d < nd; ++d) {
       const int k = d % K;
       const bool grouped = P.desc[d].num_tris > kLeafMax;  // one-leaf roots are built directly
-      buildOne(S[k], S[k].slot[pending[k].size()], d, pipes[k], grouped);
+      buildOne(S[k], S[k].slot[pending[k].size()], d, pipe(k), grouped);
       if (grouped) {
         pending[k].push_back(d);
         if ((int)pending[k].size() == G) flush(k);
       }
     }
     for (int k = 0; k < K; ++k) flush(k);
-    pipes.release();
+    side.release();
+    SPRAY_HIP(hipStreamSynchronize(s));  // pipeline 0
     if (pipe_fill) src.check(c);
     std::vector<int> h((size_t)nd * kLevelWords);
     SPRAY_HIP(hipMemcpy(h.data(), lvl_all.p, h.size() * 4, hipMemcpyDeviceToHost));

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -125,6 +127,22 @@ struct MeshSource {
   virtual bool rebuilds() const { return false; }
 };
 
+// Pipelines that side-by-side work should use: at most `want`, and no more
+// than the hardware queues HIP gives a process (GPU_MAX_HW_QUEUES, 4 by
+// default).  Pipeline 0 runs on the context's stream, the others on a
+// StreamSet of pipelines - 1 streams.  A stream beyond the queues shares a
+// queue with another and runs after it: four pipelines on side streams
+// alone left the context's queue idle and put two pipelines on one queue,
+// which set the length of the whole build.
+inline int pipelineCount(int want) {
+  int hw = 4;
+  if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) {
+    const int v = std::atoi(e);
+    if (v > 0) hw = v;
+  }
+  return std::max(1, std::min(want, hw));
+}
+
 // Non-blocking streams that start after the work issued so far on `after`;
 // join() makes `after` wait for everything issued on them, and the
 // destructor waits for them and destroys them.
@@ -132,6 +150,7 @@ struct StreamSet {
   std::vector<hipStream_t> st;
   std::vector<hipEvent_t> ev;
   StreamSet(int k, hipStream_t after) {
+    if (k <= 0) return;
     st.assign(k, nullptr);
     ev.assign(k, nullptr);
     try {

@@ -853,12 +853,13 @@ struct McSource : MeshSource {
     SPRAY_HIP(hipMemcpyAsync(table.p, T, 256 * 25, hipMemcpyHostToDevice, s));
     SPRAY_HIP(hipMemcpyAsync(ntri.p, counts, 256, hipMemcpyHostToDevice, s));
     // count pass: vertex and face totals and vertex bounds of every domain;
-    // with every domain's buffers kept, the domains run on kCountStreams
-    // streams, each with its own bound partials and scan tile sums
+    // with every domain's buffers kept, the domains run on up to
+    // kCountStreams streams (the context's among them), each with its own
+    // bound partials and scan tile sums
     uint32_t* tot = dev_out.as<uint32_t>();
     float* bnd = (float*)(tot + 2 * (size_t)nd);
     if (keep && nd > 1) {
-      const int K = std::min(kCountStreams, nd);
+      const int K = std::min(pipelineCount(kCountStreams), nd);
       long long scan_max = 0;
       for (int d = 0; d < nd; ++d)
         scan_max = std::max(scan_max, std::max(edgeBlocks(grid[d]), cellBlocks(grid[d])) + 1);
@@ -871,11 +872,11 @@ struct McSource : MeshSource {
       DevBuf& per = pb.b;
       per.reserve((part_b + sums_b) * K + 64);
       {
-        StreamSet st(K, s);
+        StreamSet st(K - 1, s);  // stream 0 is s itself
         for (int d = 0; d < nd; ++d) {
           char* base = per.as<char>() + (part_b + sums_b) * (d % K);
-          count(grid[d], parts[d], bnd + 6 * d, tot + 2 * d, st[d % K], (float*)base,
-                (uint32_t*)(base + part_b));
+          count(grid[d], parts[d], bnd + 6 * d, tot + 2 * d, d % K ? st[d % K - 1] : s,
+                (float*)base, (uint32_t*)(base + part_b));
         }
         st.join(s);
       }
