@@ -102,6 +102,8 @@ constexpr uint32_t kRange = 256;
 constexpr int kWorkCounters = 8;
 constexpr int kWorkStride = 1024;
 static_assert((kWorkCounters & (kWorkCounters - 1)) == 0, "a wave steps through the counters mod K");
+// kCount's lanes take new rays once this many of a wave's lanes wait.
+constexpr int kCountRefill = 16;
 // Shade slots are taken kShadeChunk at a time per wave.
 constexpr uint32_t kShadeChunk = 64;
 static_assert(kQueueGrid % kBlock == 0, "kBlockBases splits the grid evenly over its threads");
@@ -272,10 +274,11 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
   return false;
 }
 
-// One ray's domain list: per-domain counts (lc), the 16 nearest (t, id)
-// entries sorted (registers), the domains hit as a bit mask (one word per 64
-// domains, this lane's words at mask[w * stride]) so that the scatter
-// re-tests only those, and the number of domains hit.  Candidates come from
+// One ray's domain list, a walk that a lane resumes one cell step at a
+// time: per-domain counts (lc), the 16 nearest (t, id) entries sorted
+// (registers), the domains hit as a bit mask (one word per 64 domains, this
+// lane's words at mask[w * stride]) so that the scatter re-tests only those,
+// and the number of domains hit.  Candidates come from
 // a 3D-DDA walk of the domain grid over the ray's t range inside the grid
 // (from max(0, entry)).  The walk is a
 // supercover where it matters: when the ray crosses two or three cell faces
@@ -287,45 +290,47 @@ __device__ __forceinline__ bool passRay(const TraceParams& P, long long i, int* 
 // already marked gets the exact slab test, and the list is kept sorted by
 // (t, id), so the result does not depend on the grid or on the visiting
 // order.
-__device__ __forceinline__ int rayDomains(const float4* box, const DomGrid& G,
-                                              const uint32_t* __restrict__ coff,
-                                              const uint16_t* __restrict__ cids, const float o[3],
-                                              const float d[3], const float inv[3], bool safe,
-                                              int spawn, uint32_t* lc_kind, float tt[kTopDepths],
-                                              int ti[kTopDepths], int* kept_o,
-                                              unsigned long long* mask, int stride) {
-  int kept = 0, hits = 0;
-  // the ray's interval inside the grid box; an axis with an infinite 1/dir
-  // does not step and only needs the origin inside the slab
-  float t0 = 0.f, t1 = kInfF;
-  bool in = true;
-  bool moves[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    moves[k] = fabsf(inv[k]) < kInfF;
-    if (moves[k]) {
-      const float a = (G.lo[k] - o[k]) * inv[k], b = (G.hi[k] - o[k]) * inv[k];
-      t0 = fmaxf(t0, fminf(a, b));
-      t1 = fminf(t1, fmaxf(a, b));
-    } else if (o[k] < G.lo[k] || o[k] > G.hi[k]) {
-      in = false;
-    }
-  }
+// What every lane of a block shares: domain boxes, grid, cell lists, the
+// block's counts (LDS) and the lanes' hit-mask words (LDS).
+struct DomEnv {
+  const float4* box;
+  DomGrid G;
+  const uint32_t* __restrict__ coff;
+  const uint16_t* __restrict__ cids;
+  uint32_t* lc;
+  unsigned long long* mask;  // this lane's words at mask[w * stride]
+  int stride, nd;
+};
+
+struct DomWalk {
+  // the ray and its list
+  float o[3], d[3], inv[3];
+  int spawn, kind;
+  float tt[kTopDepths];
+  int ti[kTopDepths];
+  int kept, hits;
+  // the walk: current cell (8 bits per axis, kGridMaxRes <= 256), step per
+  // axis (2 bits each, -1..1 stored + 1), exit t, steps left (0: done)
+  uint32_t cell, steps;
+  float t1;
+  int left;
+
   // candidates of one cell (cells outside the grid have none)
-  auto visit = [&](int x, int y, int z) {
+  __device__ __forceinline__ void visit(const DomEnv& E, int x, int y, int z, bool safe) {
+    const DomGrid& G = E.G;
     if (x < 0 || y < 0 || z < 0 || x >= G.r[0] || y >= G.r[1] || z >= G.r[2]) return;
-    const int cell = (z * G.r[1] + y) * G.r[0] + x;
-    const uint32_t q1 = coff[cell + 1];
-    for (uint32_t q = coff[cell]; q < q1; ++q) {
-      const int dd = cids[q];
-      unsigned long long* mw = &mask[(dd >> 6) * stride];
+    const int cl = (z * G.r[1] + y) * G.r[0] + x;
+    const uint32_t q1 = E.coff[cl + 1];
+    for (uint32_t q = E.coff[cl]; q < q1; ++q) {
+      const int dd = E.cids[q];
+      unsigned long long* mw = &E.mask[(dd >> 6) * E.stride];
       if ((*mw >> (dd & 63)) & 1ull) continue;  // already a hit
       float tmin;
       // isectWithoutCurrentDomain (ooc_isector.h:147-174): a shadow ray's
       // spawning domain was tested right after shading
-      if (!slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
+      if (!slabBox4(E.box[2 * dd], E.box[2 * dd + 1], o, inv, safe, &tmin) || dd == spawn) continue;
       ++hits;
-      atomicAdd(&lc_kind[dd], 1u);
+      atomicAdd(&E.lc[kind * E.nd + dd], 1u);
       *mw |= 1ull << (dd & 63);
       // insertion into the (t, id)-sorted top list (unrolled: stays in
       // registers); steps past every active lane's list end are skipped
@@ -352,9 +357,32 @@ __device__ __forceinline__ int rayDomains(const float4* box, const DomGrid& G,
         if (kept < kTopDepths) ++kept;
       }
     }
-  };
-  if (in && t0 <= t1) {
-    int c[3], st[3], nb[3];
+  }
+
+  // Starts the walk: the ray's interval inside the grid box (an axis with an
+  // infinite 1/dir does not step and only needs the origin inside the slab),
+  // the first cell and the neighbours across the faces its point lies on.
+  __device__ __forceinline__ void start(const DomEnv& E, bool safe) {
+    const DomGrid& G = E.G;
+    kept = 0;
+    hits = 0;
+    left = 0;
+    float t0 = 0.f;
+    t1 = kInfF;
+    bool in = true, moves[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      moves[k] = fabsf(inv[k]) < kInfF;
+      if (moves[k]) {
+        const float a = (G.lo[k] - o[k]) * inv[k], b = (G.hi[k] - o[k]) * inv[k];
+        t0 = fmaxf(t0, fminf(a, b));
+        t1 = fminf(t1, fmaxf(a, b));
+      } else if (o[k] < G.lo[k] || o[k] > G.hi[k]) {
+        in = false;
+      }
+    }
+    if (!(in && t0 <= t1)) return;
+    int nb[3], c[3], st[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float p = o[k] + t0 * d[k];
@@ -364,44 +392,52 @@ __device__ __forceinline__ int rayDomains(const float4* box, const DomGrid& G,
       const float fr = u - (float)c[k];
       nb[k] = (fr < kGridTol && c[k] > 0) ? -1 : (fr > 1.f - kGridTol && c[k] < G.r[k] - 1) ? 1 : 0;
     }
-    // the neighbours across faces the first point lies on
     const int nbm = (nb[0] != 0 ? 1 : 0) | (nb[1] != 0 ? 2 : 0) | (nb[2] != 0 ? 4 : 0);
     for (int sub = 1; sub < 8; ++sub)
       if ((sub & ~nbm) == 0)
-        visit(c[0] + ((sub & 1) ? nb[0] : 0), c[1] + ((sub & 2) ? nb[1] : 0),
-              c[2] + ((sub & 4) ? nb[2] : 0));
-    const int max_steps = G.r[0] + G.r[1] + G.r[2] + 3;
-    for (int step = 0; step < max_steps; ++step) {
-      visit(c[0], c[1], c[2]);
-      // next cell: the axes whose cell faces the ray crosses first
-      float tn[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        tn[k] = st[k] == 0 ? kInfF
-                           : (G.lo[k] + (float)(c[k] + (st[k] > 0 ? 1 : 0)) * G.h[k] - o[k]) * inv[k];
-      const float m = fminf(fminf(tn[0], tn[1]), tn[2]);
-      if (!(m <= t1)) break;
-      const float lim = m + kGridTol * fmaxf(1.f, fabsf(m));
-      const int tm = (st[0] != 0 && tn[0] <= lim ? 1 : 0) | (st[1] != 0 && tn[1] <= lim ? 2 : 0) |
-                     (st[2] != 0 && tn[2] <= lim ? 4 : 0);
-      // crossings at (nearly) the same t: the cells of the partial steps too
-      for (int sub = 1; sub < 7; ++sub)
-        if ((sub & ~tm) == 0 && sub != tm)
-          visit(c[0] + ((sub & 1) ? st[0] : 0), c[1] + ((sub & 2) ? st[1] : 0),
-                c[2] + ((sub & 4) ? st[2] : 0));
-      bool out = false;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (tm & (1 << k)) {
-          c[k] += st[k];
-          out = out || c[k] < 0 || c[k] >= G.r[k];
-        }
-      if (out) break;
-    }
+        visit(E, c[0] + ((sub & 1) ? nb[0] : 0), c[1] + ((sub & 2) ? nb[1] : 0),
+              c[2] + ((sub & 4) ? nb[2] : 0), safe);
+    cell = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16);
+    steps = (uint32_t)(st[0] + 1) | ((uint32_t)(st[1] + 1) << 2) | ((uint32_t)(st[2] + 1) << 4);
+    left = G.r[0] + G.r[1] + G.r[2] + 3;
   }
-  *kept_o = kept;
-  return hits;
-}
+
+  // One step: the current cell, then the next cell(s) the ray enters.
+  __device__ __forceinline__ void step(const DomEnv& E, bool safe) {
+    const DomGrid& G = E.G;
+    int c[3] = {(int)(cell & 255u), (int)((cell >> 8) & 255u), (int)((cell >> 16) & 255u)};
+    const int st[3] = {(int)(steps & 3u) - 1, (int)((steps >> 2) & 3u) - 1,
+                       (int)((steps >> 4) & 3u) - 1};
+    visit(E, c[0], c[1], c[2], safe);
+    --left;
+    // next cell: the axes whose cell faces the ray crosses first
+    float tn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      tn[k] = st[k] == 0 ? kInfF
+                         : (G.lo[k] + (float)(c[k] + (st[k] > 0 ? 1 : 0)) * G.h[k] - o[k]) * inv[k];
+    const float m = fminf(fminf(tn[0], tn[1]), tn[2]);
+    if (!(m <= t1)) {
+      left = 0;
+      return;
+    }
+    const float lim = m + kGridTol * fmaxf(1.f, fabsf(m));
+    const int tm = (st[0] != 0 && tn[0] <= lim ? 1 : 0) | (st[1] != 0 && tn[1] <= lim ? 2 : 0) |
+                   (st[2] != 0 && tn[2] <= lim ? 4 : 0);
+    // crossings at (nearly) the same t: the cells of the partial steps too
+    for (int sub = 1; sub < 7; ++sub)
+      if ((sub & ~tm) == 0 && sub != tm)
+        visit(E, c[0] + ((sub & 1) ? st[0] : 0), c[1] + ((sub & 2) ? st[1] : 0),
+              c[2] + ((sub & 4) ? st[2] : 0), safe);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (tm & (1 << k)) {
+        c[k] += st[k];
+        if (c[k] < 0 || c[k] >= G.r[k]) left = 0;
+      }
+    cell = (uint32_t)(c[0] & 255) | ((uint32_t)(c[1] & 255) << 8) | ((uint32_t)(c[2] & 255) << 16);
+  }
+};
 
 // Isector::isectAll / isectWithoutCurrentDomain (ooc_isector.h:126-174) as
 // counts: entries per domain and the DomainStats score
@@ -409,7 +445,7 @@ __device__ __forceinline__ int rayDomains(const float4* box, const DomGrid& G,
 // kLearned: the learned form's ask records are written (a separate
 // instantiation, so the other forms keep kCount's smaller register set).
 template <bool kLearned>
-__global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
+__global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
   const int nd = P.ndomains, nw = (nd + 63) >> 6;
@@ -443,68 +479,106 @@ __global__ void __launch_bounds__(kBlock) kCount(TraceParams P) {
   const DevCounters* c = P.ctr;
   const long long total = (long long)(c->r_end - c->r_beg) + (c->s_end - c->s_beg);
   const long long stride = (long long)gridDim.x * blockDim.x;
-  // wave-uniform loop (the slab-test variant is chosen per wave)
-  for (long long base = blockIdx.x * (long long)blockDim.x; base < total; base += stride) {
-    const long long i = base + threadIdx.x;
-    int kind = 0;
-    uint32_t r = 0;
-    bool valid = passRay(P, i, &kind, &r);
-    // shadow rays occluded in their spawning domain are done (procShads2)
-    if (valid && kind == 1 && P.Sx.occluded[r]) valid = false;
-    // the 16 nearest (t, id) entries carry score weight
-    float tt[kTopDepths];
-    int ti[kTopDepths];
-    int kept = 0, hits = 0;
-    float o[3] = {0.f, 0.f, 0.f}, inv[3] = {1.f, 1.f, 1.f}, dir[3] = {1.f, 1.f, 1.f};
-    int spawn = -1;
-    if (valid) {
-      const RayPool& pool = kind ? P.S : P.R;
-      float4 op = pool.op[r], ds = pool.ds[r];
-      o[0] = op.x;
-      o[1] = op.y;
-      o[2] = op.z;
-      dir[0] = ds.x;
-      dir[1] = ds.y;
-      dir[2] = ds.z;
-      inv[0] = 1.0f / ds.x;
-      inv[1] = 1.0f / ds.y;
-      inv[2] = 1.0f / ds.z;
-      spawn = kind ? (__float_as_int(pool.wd[r].w) >> 8) : -1;
+  // The wave's rays are i = wbase + k * stride + j (j < 64, k = 0, 1, ...),
+  // the rays a grid-stride loop would give it, so every block counts the
+  // rays kScatter's block places.  Walks differ in length, so a lane takes
+  // the wave's next ray as soon as its own is done (once kCountRefill lanes
+  // wait), instead of the wave waiting for its longest walk.
+  const long long wbase = blockIdx.x * (long long)blockDim.x + (threadIdx.x & ~63);
+  const int lane = __lane_id();
+  uint32_t seq = 0;  // wave-uniform: rays handed out
+  bool more = wbase < total;
+  bool busy = false;
+  int kind = 0;
+  uint32_t r = 0, i = 0;  // the lane's ray: pool index, pass index (< 2^32)
+  DomEnv E;
+  E.box = box;
+  E.G = G;
+  E.coff = coff;
+  E.cids = cids;
+  E.lc = lc;
+  E.mask = lmask;
+  E.stride = blockDim.x;
+  E.nd = nd;
+  DomWalk W;
+  W.kind = 0;
+  W.left = 0;
+  W.kept = W.hits = 0;
+  W.spawn = -1;
+  W.o[0] = W.o[1] = W.o[2] = 0.f;
+  W.d[0] = W.d[1] = W.d[2] = 1.f;
+  W.inv[0] = W.inv[1] = W.inv[2] = 1.f;
+  while (true) {
+    const unsigned long long idle = __ballot(!busy);
+    const int nidle = __popcll(idle);
+    bool fresh = false;
+    if (more && (nidle >= kCountRefill || nidle == 64)) {
+      if (!busy) {
+        const uint32_t q = seq + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        const long long ii = wbase + (long long)(q >> 6) * stride + (q & 63u);
+        i = (uint32_t)ii;
+        bool valid = ii < total && passRay(P, ii, &kind, &r);
+        // shadow rays occluded in their spawning domain are done (procShads2)
+        if (valid && kind == 1 && P.Sx.occluded[r]) valid = false;
+        if (valid) {
+          const RayPool& pool = kind ? P.S : P.R;
+          const float4 op = pool.op[r], ds = pool.ds[r];
+          W.o[0] = op.x;
+          W.o[1] = op.y;
+          W.o[2] = op.z;
+          W.d[0] = ds.x;
+          W.d[1] = ds.y;
+          W.d[2] = ds.z;
+          W.inv[0] = 1.0f / ds.x;
+          W.inv[1] = 1.0f / ds.y;
+          W.inv[2] = 1.0f / ds.z;
+          W.spawn = kind ? (__float_as_int(pool.wd[r].w) >> 8) : -1;
+          W.kind = kind;
+          for (int w = 0; w < nw; ++w) lmask[w * blockDim.x] = 0ull;
+          // NaN-checked slab tests only in waves with an axis-parallel ray
+          W.start(E, __ballot(hasInfInv(W.inv)) != 0ull);
+          busy = fresh = true;
+        }
+      }
+      seq += (uint32_t)nidle;
+      more = wbase + (long long)(seq >> 6) * stride + (seq & 63u) < total;
     }
-    // NaN-checked slab tests only in waves with an axis-parallel ray
-    const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
-    if (valid) {
-      for (int w = 0; w < nw; ++w) lmask[w * blockDim.x] = 0ull;
-      hits = rayDomains(box, G, coff, cids, o, dir, inv, safe, spawn, lc + kind * nd, tt, ti, &kept,
-                        lmask, blockDim.x);
+    const bool safe = __ballot(busy && hasInfInv(W.inv)) != 0ull;
+    if (busy && W.left > 0 && !fresh) W.step(E, safe);
+    // a ray whose walk is done: its mask words, its score, (learned) its ask
+    const bool fin = busy && W.left <= 0;
+    if (fin) {
       for (int w = 0; w < nw; ++w) P.dmask[(size_t)i * nw + w] = lmask[w * blockDim.x];
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
-        if (k < kept) atomicAdd(&ls[ti[k]], (uint32_t)(kTopDepths - k));
+        if (k < W.kept) atomicAdd(&ls[W.ti[k]], (uint32_t)(kTopDepths - k));
     }
     // learned form: a radiance ray with two or more candidate domains writes
     // an ask record for the predictor into this block's region (one LDS
     // atomic per wave); the others have no phase-A bound
     if (kLearned) {
-      const bool ask = valid && kind == 0 && kept >= 2;
+      const bool ask = fin && kind == 0 && W.kept >= 2;
       const unsigned long long am = __ballot(ask);
       uint32_t abase = 0;
       if (am != 0ull) {
         const int leader = __builtin_ctzll(am);
-        if (__lane_id() == leader) abase = atomicAdd(ask_n, (uint32_t)__popcll(am));
+        if (lane == leader) abase = atomicAdd(ask_n, (uint32_t)__popcll(am));
         abase = __builtin_amdgcn_readlane(abase, leader);
       }
-      const uint32_t a = abase + (uint32_t)__popcll(am & ((1ull << __lane_id()) - 1ull));
+      const uint32_t a = abase + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
       if (ask && a < P.ask_region) {
         float4* rec = P.ask + ((size_t)blockIdx.x * P.ask_region + a) * 3;
-        rec[0] = make_float4(o[0], o[1], o[2], dir[0]);
-        rec[1] = make_float4(dir[1], dir[2], tt[0], tt[1]);
-        rec[2] = make_float4(__uint_as_float(r), __int_as_float(ti[0]), __int_as_float(ti[1]),
-                             __int_as_float(min(hits, 255)));
-      } else if (valid && kind == 0) {
+        const float4 ds = P.R.ds[r];
+        rec[0] = make_float4(W.o[0], W.o[1], W.o[2], ds.x);
+        rec[1] = make_float4(ds.y, ds.z, W.tt[0], W.tt[1]);
+        rec[2] = make_float4(__uint_as_float(r), __int_as_float(W.ti[0]), __int_as_float(W.ti[1]),
+                             __int_as_float(min(W.hits, 255)));
+      } else if (fin && kind == 0) {
         P.rkey[r] = kNoBound;
       }
     }
+    if (fin) busy = false;
+    if (!more && __ballot(busy) == 0ull) break;
   }
   __syncthreads();
   if (kLearned && threadIdx.x == 0) P.ask_cnt[blockIdx.x] = min(*ask_n, P.ask_region);
