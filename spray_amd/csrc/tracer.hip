@@ -489,8 +489,7 @@ __global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
   uint32_t seq = 0;  // wave-uniform: rays handed out
   bool more = wbase < total;
   bool busy = false;
-  int kind = 0;
-  uint32_t r = 0, i = 0;  // the lane's ray: pool index, pass index (< 2^32)
+  uint32_t i = 0;  // the lane's ray: pass index (< 2^32; its pool index follows from it)
   DomEnv E;
   E.box = box;
   E.G = G;
@@ -517,6 +516,8 @@ __global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
         const uint32_t q = seq + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         const long long ii = wbase + (long long)(q >> 6) * stride + (q & 63u);
         i = (uint32_t)ii;
+        int kind = 0;
+        uint32_t r = 0;
         bool valid = ii < total && passRay(P, ii, &kind, &r);
         // shadow rays occluded in their spawning domain are done (procShads2)
         if (valid && kind == 1 && P.Sx.occluded[r]) valid = false;
@@ -557,7 +558,9 @@ __global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
     // an ask record for the predictor into this block's region (one LDS
     // atomic per wave); the others have no phase-A bound
     if (kLearned) {
-      const bool ask = fin && kind == 0 && W.kept >= 2;
+      const bool rad = fin && W.kind == 0;
+      const uint32_t r = c->r_beg + i;  // a radiance ray's pool index
+      const bool ask = rad && W.kept >= 2;
       const unsigned long long am = __ballot(ask);
       uint32_t abase = 0;
       if (am != 0ull) {
@@ -573,7 +576,7 @@ __global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
         rec[1] = make_float4(ds.y, ds.z, W.tt[0], W.tt[1]);
         rec[2] = make_float4(__uint_as_float(r), __int_as_float(W.ti[0]), __int_as_float(W.ti[1]),
                              __int_as_float(min(W.hits, 255)));
-      } else if (fin && kind == 0) {
+      } else if (rad) {
         P.rkey[r] = kNoBound;
       }
     }
