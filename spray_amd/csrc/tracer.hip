@@ -776,10 +776,10 @@ __device__ __forceinline__ void childHistory(const float ph[4], int depth, float
 // buffers alternate, so a call never overwrites words the previous call's
 // readers may still need.
 struct BlockAppend {
-  uint32_t* sm;  // 2 x 17 words of LDS
+  uint32_t* sm;  // 2 x 34 words of LDS
   uint32_t flip;
   __device__ uint32_t operator()(bool pred, uint32_t* counter, uint32_t cap, uint32_t* err) {
-    uint32_t* b = sm + 17 * flip;
+    uint32_t* b = sm + 34 * flip;
     flip ^= 1u;
     const unsigned long long m = __ballot(pred);
     const int lane = __lane_id(), w = threadIdx.x >> 6;
@@ -800,6 +800,45 @@ struct BlockAppend {
     const uint32_t idx = b[16] + b[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     return (pred && idx < cap) ? idx : 0xffffffffu;
   }
+  // Up to two items into each of two pools in one call (the same two
+  // barriers): pool A takes every wave's a1 items, then its a2 items; pool B
+  // its b1 then b2 items.  idx gets the a1, a2, b1, b2 slots (~0u: none).
+  __device__ void pair(bool a1, bool a2, uint32_t* ca, uint32_t capa, bool b1, bool b2,
+                       uint32_t* cb, uint32_t capb, uint32_t* err, uint32_t idx[4]) {
+    uint32_t* b = sm + 34 * flip;
+    flip ^= 1u;
+    const unsigned long long ma1 = __ballot(a1), ma2 = __ballot(a2), mb1 = __ballot(b1),
+                             mb2 = __ballot(b2);
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t na1 = (uint32_t)__popcll(ma1), nb1 = (uint32_t)__popcll(mb1);
+    if (lane == 0) {
+      b[w] = na1 + (uint32_t)__popcll(ma2);
+      b[17 + w] = nb1 + (uint32_t)__popcll(mb2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {  // lane 0: pool A, lane 1: pool B
+      uint32_t* bb = b + 17 * threadIdx.x;
+      uint32_t acc = 0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        const uint32_t x = bb[i];
+        bb[i] = acc;
+        acc += x;
+      }
+      const uint32_t cap = threadIdx.x ? capb : capa;
+      const uint32_t base = acc ? atomicAdd(threadIdx.x ? cb : ca, acc) : 0u;
+      bb[16] = base;
+      if (acc && base + acc > cap) atomicOr(err, 1u);
+    }
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const uint32_t ba = b[16] + b[w], bb = b[33] + b[17 + w];
+    const uint32_t i0 = ba + (uint32_t)__popcll(ma1 & lt), i1 = ba + na1 + (uint32_t)__popcll(ma2 & lt);
+    const uint32_t i2 = bb + (uint32_t)__popcll(mb1 & lt), i3 = bb + nb1 + (uint32_t)__popcll(mb2 & lt);
+    idx[0] = (a1 && i0 < capa) ? i0 : 0xffffffffu;
+    idx[1] = (a2 && i1 < capa) ? i1 : 0xffffffffu;
+    idx[2] = (b1 && i2 < capb) ? i2 : 0xffffffffu;
+    idx[3] = (b2 && i3 < capb) ? i3 : 0xffffffffu;
+  }
 };
 
 __device__ __forceinline__ void spawnShadow(const TraceParams& P, BlockAppend& app, bool pred,
@@ -814,17 +853,23 @@ __device__ __forceinline__ void spawnShadow(const TraceParams& P, BlockAppend& a
   P.Sx.occluded[idx] = 0u;
 }
 
-__device__ __forceinline__ void spawnBounce(const TraceParams& P, BlockAppend& app, bool pred,
-                                            bool pending, f3 pos, int pixid, f3 wi, int samid,
-                                            f3 Lr, int depth, const float hist[4], float t) {
+// The hit's bounce rays (reflection b1 along d1, transmission b2 along d2),
+// appended together.
+__device__ __forceinline__ void spawnBounces(const TraceParams& P, BlockAppend& app, bool b1,
+                                             f3 d1, f3 L1, bool b2, f3 d2, f3 L2, bool pending,
+                                             f3 pos, int pixid, int samid, int depth,
+                                             const float hist[4], float t) {
   // rays whose next virtual depth is 4 go to the pending pool (genR2,
   // ooc_shader_pt.h:73-90) with their parent's full history
-  uint32_t ir = app(pred && !pending, &P.ctr->r_count, P.r_cap, &P.ctr->err);
-  uint32_t ip = app(pred && pending, &P.ctr->p_count, P.p_cap, &P.ctr->err);
+  uint32_t ix[4];
+  app.pair(b1 && !pending, b2 && !pending, &P.ctr->r_count, P.r_cap, b1 && pending, b2 && pending,
+           &P.ctr->p_count, P.p_cap, &P.ctr->err, ix);
   float h[4];
   childHistory(hist, depth, t, h);
-  if (ir != 0xffffffffu) writeRay(P.R, ir, pos, pixid, wi, samid, Lr, depth + 1, h);
-  if (ip != 0xffffffffu) writeRay(P.Pn, ip, pos, pixid, wi, samid, Lr, 0, h);
+  if (ix[0] != 0xffffffffu) writeRay(P.R, ix[0], pos, pixid, d1, samid, L1, depth + 1, h);
+  if (ix[1] != 0xffffffffu) writeRay(P.R, ix[1], pos, pixid, d2, samid, L2, depth + 1, h);
+  if (ix[2] != 0xffffffffu) writeRay(P.Pn, ix[2], pos, pixid, d1, samid, L1, 0, h);
+  if (ix[3] != 0xffffffffu) writeRay(P.Pn, ix[3], pos, pixid, d2, samid, L2, 0, h);
 }
 
 // Light sampling (render/light.h:41-89 and the rect light)
@@ -1023,8 +1068,7 @@ __device__ __forceinline__ void shadeSpawn(const TraceParams& P, BlockAppend& ap
     d1 = wi;
     L1 = Lr;
   }
-  spawnBounce(P, app, b1, pending, pos, pixid, d1, samid, L1, depth, hist, t);
-  spawnBounce(P, app, b2, pending, pos, pixid, d2, samid, L2, depth, hist, t);
+  spawnBounces(P, app, b1, d1, L1, b2, d2, L2, pending, pos, pixid, samid, depth, hist, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1346,7 +1390,7 @@ __global__ void __launch_bounds__(kTravBlock) kTraverse(TraceParams P, int plain
 // buffer's final entry for this pass is shaded once (stale speculative hits
 // would only spawn rays that can never retire, so they are skipped).
 __global__ void __launch_bounds__(kShadeBlock, 4) kShade(TraceParams P) {
-  __shared__ uint32_t app_sm[2 * 17];
+  __shared__ uint32_t app_sm[2 * 34];
   DevCounters* c = P.ctr;
   if (!c->active) return;
   BlockAppend app{app_sm, 0u};
