@@ -293,9 +293,13 @@ __global__ void kKarras(const uint32_t* __restrict__ keys, int n, int* left, int
 }
 
 // Leaf boxes in sorted order + the leaf-ordered triangle array of the pool.
+// parent_init (may be null): set to -1 for every leaf, the Karras
+// hierarchy's parent array before kKarras fills it (the root keeps -1).
 __global__ void kLeaves(const float* __restrict__ vtx, const uint32_t* __restrict__ face, int nf,
-                        const uint32_t* __restrict__ order, float* leafbox, LTri* out) {
+                        const uint32_t* __restrict__ order, float* leafbox, LTri* out,
+                        int* parent_init = nullptr) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+    if (parent_init) parent_init[i] = -1;
     uint32_t p = order[i];
     const float* a = vtx + (size_t)face[p * 3] * 3;
     const float* b = vtx + (size_t)face[p * 3 + 1] * 3;
@@ -1187,8 +1191,7 @@ void buildPool(Ctx* c, MeshSource& src, bool keep_buffers) {
       // Karras LBVH: leaves in Morton order, bottom-up refit
       int* PI = S.pin.as<int>();
       int* PL = S.plf.as<int>();
-      kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris);
-      SPRAY_HIP(hipMemsetAsync(PI, 0xff, (size_t)nf * 4, s));  // root parent = -1
+      kLeaves<<<gridFor(nf, 256), 256, 0, s>>>(vtx, face, nf, va, leafbox, tris, PI);
       kKarras<<<gridFor(nf, 256), 256, 0, s>>>(ka, nf, L, R, F, Lst, PI, PL);
       // inner boxes as unions over leaf ranges (replaces a bottom-up refit)
       RangeLevels RL;
