@@ -108,11 +108,6 @@ constexpr int kCountRefill = 16;
 constexpr uint32_t kShadeChunk = 64;
 static_assert(kQueueGrid % kBlock == 0, "kBlockBases splits the grid evenly over its threads");
 
-__device__ __forceinline__ void flushCount(unsigned long long* dst, unsigned long long v) {
-  v = waveSumULL(v);
-  if (__lane_id() == 0 && v) atomicAdd(dst, v);
-}
-
 // Block-wide flush of N wave-uniform counts: one atomic per counter and
 // block instead of per wave (waves that end together would otherwise queue
 // on the same device-scope atomics).  Every thread of the block calls it.
@@ -130,6 +125,77 @@ __device__ __forceinline__ void blockFlush(unsigned long long* const (&dst)[N],
     if (sum) atomicAdd(dst[threadIdx.x], sum);
   }
 }
+
+// Appends to a ray pool with one device-scope atomic per block and call
+// (same-address atomics from every wave queue up at the device level).
+// Every thread of the block makes the same sequence of calls; the two LDS
+// buffers alternate, so a call never overwrites words the previous call's
+// readers may still need.
+struct BlockAppend {
+  uint32_t* sm;  // 2 x 34 words of LDS
+  uint32_t flip;
+  __device__ uint32_t operator()(bool pred, uint32_t* counter, uint32_t cap, uint32_t* err) {
+    uint32_t* b = sm + 34 * flip;
+    flip ^= 1u;
+    const unsigned long long m = __ballot(pred);
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    if (lane == 0) b[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t acc = 0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        const uint32_t x = b[i];
+        b[i] = acc;
+        acc += x;
+      }
+      const uint32_t base = acc ? atomicAdd(counter, acc) : 0u;
+      b[16] = base;
+      if (acc && base + acc > cap) atomicOr(err, 1u);
+    }
+    __syncthreads();
+    const uint32_t idx = b[16] + b[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    return (pred && idx < cap) ? idx : 0xffffffffu;
+  }
+  // Up to two items into each of two pools in one call (the same two
+  // barriers): pool A takes every wave's a1 items, then its a2 items; pool B
+  // its b1 then b2 items.  idx gets the a1, a2, b1, b2 slots (~0u: none).
+  __device__ void pair(bool a1, bool a2, uint32_t* ca, uint32_t capa, bool b1, bool b2,
+                       uint32_t* cb, uint32_t capb, uint32_t* err, uint32_t idx[4]) {
+    uint32_t* b = sm + 34 * flip;
+    flip ^= 1u;
+    const unsigned long long ma1 = __ballot(a1), ma2 = __ballot(a2), mb1 = __ballot(b1),
+                             mb2 = __ballot(b2);
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t na1 = (uint32_t)__popcll(ma1), nb1 = (uint32_t)__popcll(mb1);
+    if (lane == 0) {
+      b[w] = na1 + (uint32_t)__popcll(ma2);
+      b[17 + w] = nb1 + (uint32_t)__popcll(mb2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {  // lane 0: pool A, lane 1: pool B
+      uint32_t* bb = b + 17 * threadIdx.x;
+      uint32_t acc = 0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        const uint32_t x = bb[i];
+        bb[i] = acc;
+        acc += x;
+      }
+      const uint32_t cap = threadIdx.x ? capb : capa;
+      const uint32_t base = acc ? atomicAdd(threadIdx.x ? cb : ca, acc) : 0u;
+      bb[16] = base;
+      if (acc && base + acc > cap) atomicOr(err, 1u);
+    }
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const uint32_t ba = b[16] + b[w], bb = b[33] + b[17 + w];
+    const uint32_t i0 = ba + (uint32_t)__popcll(ma1 & lt), i1 = ba + na1 + (uint32_t)__popcll(ma2 & lt);
+    const uint32_t i2 = bb + (uint32_t)__popcll(mb1 & lt), i3 = bb + nb1 + (uint32_t)__popcll(mb2 & lt);
+    idx[0] = (a1 && i0 < capa) ? i0 : 0xffffffffu;
+    idx[1] = (a2 && i1 < capa) ? i1 : 0xffffffffu;
+    idx[2] = (b1 && i2 < capb) ? i2 : 0xffffffffu;
+    idx[3] = (b2 && i3 < capb) ? i3 : 0xffffffffu;
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Tile / window bookkeeping (single-thread kernels keep the whole loop on
@@ -183,7 +249,9 @@ __global__ void kWindowStart(TraceParams P) {
   c->s_mark = 0;
 }
 __global__ void kRestartPending(TraceParams P) {
+  __shared__ uint32_t app_sm[2 * 34];
   DevCounters* c = P.ctr;
+  BlockAppend app{app_sm, 0u};
   const uint32_t n = min(c->p_count, P.p_cap);
   unsigned long long useful = 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -193,7 +261,7 @@ __global__ void kRestartPending(TraceParams P) {
     float h[4] = {hh.x, hh.y, hh.z, hh.w};
     int samid = valid ? __float_as_int(P.Pn.ds[i].w) : 0;
     bool ok = valid && vbufCorrect(P.vbuf, samid, h, kHistory);
-    uint32_t idx = waveAppend(ok, &c->r_count, P.r_cap, &c->err);
+    const uint32_t idx = app(ok, &c->r_count, P.r_cap, &c->err);
     if (ok && idx != 0xffffffffu) {
       P.R.op[idx] = P.Pn.op[i];
       P.R.ds[idx] = P.Pn.ds[i];
@@ -203,7 +271,7 @@ __global__ void kRestartPending(TraceParams P) {
       ++useful;
     }
   }
-  flushCount(&c->useful, useful);
+  blockFlush<1>({&c->useful}, {waveSumULL(useful)});
 }
 __global__ void kRestartDone(TraceParams P) {
   DevCounters* c = P.ctr;
@@ -770,76 +838,6 @@ __device__ __forceinline__ void childHistory(const float ph[4], int depth, float
   for (int k = 0; k < kHistory; ++k) h[k] = (k < depth) ? ph[k] : (k == depth ? t : kInfF);
 }
 
-// Appends to a ray pool with one device-scope atomic per block and call
-// (same-address atomics from every wave queue up at the device level).
-// Every thread of the block makes the same sequence of calls; the two LDS
-// buffers alternate, so a call never overwrites words the previous call's
-// readers may still need.
-struct BlockAppend {
-  uint32_t* sm;  // 2 x 34 words of LDS
-  uint32_t flip;
-  __device__ uint32_t operator()(bool pred, uint32_t* counter, uint32_t cap, uint32_t* err) {
-    uint32_t* b = sm + 34 * flip;
-    flip ^= 1u;
-    const unsigned long long m = __ballot(pred);
-    const int lane = __lane_id(), w = threadIdx.x >> 6;
-    if (lane == 0) b[w] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t acc = 0;
-      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
-        const uint32_t x = b[i];
-        b[i] = acc;
-        acc += x;
-      }
-      const uint32_t base = acc ? atomicAdd(counter, acc) : 0u;
-      b[16] = base;
-      if (acc && base + acc > cap) atomicOr(err, 1u);
-    }
-    __syncthreads();
-    const uint32_t idx = b[16] + b[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    return (pred && idx < cap) ? idx : 0xffffffffu;
-  }
-  // Up to two items into each of two pools in one call (the same two
-  // barriers): pool A takes every wave's a1 items, then its a2 items; pool B
-  // its b1 then b2 items.  idx gets the a1, a2, b1, b2 slots (~0u: none).
-  __device__ void pair(bool a1, bool a2, uint32_t* ca, uint32_t capa, bool b1, bool b2,
-                       uint32_t* cb, uint32_t capb, uint32_t* err, uint32_t idx[4]) {
-    uint32_t* b = sm + 34 * flip;
-    flip ^= 1u;
-    const unsigned long long ma1 = __ballot(a1), ma2 = __ballot(a2), mb1 = __ballot(b1),
-                             mb2 = __ballot(b2);
-    const int lane = __lane_id(), w = threadIdx.x >> 6;
-    const uint32_t na1 = (uint32_t)__popcll(ma1), nb1 = (uint32_t)__popcll(mb1);
-    if (lane == 0) {
-      b[w] = na1 + (uint32_t)__popcll(ma2);
-      b[17 + w] = nb1 + (uint32_t)__popcll(mb2);
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {  // lane 0: pool A, lane 1: pool B
-      uint32_t* bb = b + 17 * threadIdx.x;
-      uint32_t acc = 0;
-      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
-        const uint32_t x = bb[i];
-        bb[i] = acc;
-        acc += x;
-      }
-      const uint32_t cap = threadIdx.x ? capb : capa;
-      const uint32_t base = acc ? atomicAdd(threadIdx.x ? cb : ca, acc) : 0u;
-      bb[16] = base;
-      if (acc && base + acc > cap) atomicOr(err, 1u);
-    }
-    __syncthreads();
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    const uint32_t ba = b[16] + b[w], bb = b[33] + b[17 + w];
-    const uint32_t i0 = ba + (uint32_t)__popcll(ma1 & lt), i1 = ba + na1 + (uint32_t)__popcll(ma2 & lt);
-    const uint32_t i2 = bb + (uint32_t)__popcll(mb1 & lt), i3 = bb + nb1 + (uint32_t)__popcll(mb2 & lt);
-    idx[0] = (a1 && i0 < capa) ? i0 : 0xffffffffu;
-    idx[1] = (a2 && i1 < capa) ? i1 : 0xffffffffu;
-    idx[2] = (b1 && i2 < capb) ? i2 : 0xffffffffu;
-    idx[3] = (b2 && i3 < capb) ? i3 : 0xffffffffu;
-  }
-};
 
 __device__ __forceinline__ void spawnShadow(const TraceParams& P, BlockAppend& app, bool pred,
                                             f3 pos, int pixid, f3 wi, int samid, f3 Lr, int depth,
@@ -1493,8 +1491,7 @@ __global__ void kWindowEnd(TraceParams P) {
       prev = t;
     }
   }
-  flushCount(&c->useful, useful);
-  flushCount(&c->final_hits, hits);
+  blockFlush<2>({&c->useful, &c->final_hits}, {waveSumULL(useful), waveSumULL(hits)});
 }
 __global__ void kWindowAdvance(TraceParams P) {
   int* rec = (int*)P.tile_rec;

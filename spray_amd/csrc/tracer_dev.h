@@ -218,23 +218,4 @@ __device__ __forceinline__ bool vbufUpdate(unsigned long long* vb, int samid, co
   }
 }
 
-// Wave-aggregated append: one atomic per wave, lane order preserved.
-__device__ __forceinline__ uint32_t waveAppend(bool pred, uint32_t* counter, uint32_t cap,
-                                               uint32_t* err) {
-  unsigned long long m = __ballot(pred);
-  if (m == 0ull) return 0xffffffffu;
-  int lane = __lane_id();
-  int leader = __builtin_ctzll(m);
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader, 64);
-  uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-  if (!pred) return 0xffffffffu;
-  if (idx >= cap) {
-    atomicOr(err, 1u);
-    return 0xffffffffu;
-  }
-  return idx;
-}
-
 }  // namespace spray
