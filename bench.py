@@ -250,7 +250,7 @@ def main():
             useful_rays=st["useful_rays"], radiance_visits=st["radiance_visits"],
             shadow_visits=st["shadow_visits"], shades=st["shades"], final_hits=st["final_hits"],
             queue_entries=st["queue_entries"], pruned=st["pruned_entries"],
-            deferred=st["deferred_entries"], passes=st["passes"],
+            deferred=st["deferred_entries"], passes=st["passes"], regrows=st["regrows"],
             node_visits=st["node_visits"], tri_tests=st["tri_tests"],
             trace_kernel_launches=launches, trace_kernel_ms=kern_ms,
             trace_kernel_bytes=algorithmic_bytes(st_prof),

@@ -297,6 +297,16 @@ __global__ void kPassBegin(TraceParams P) {
   c->s_beg = c->s_mark;
   c->s_end = min(c->s_count, P.s_cap);
   c->s_mark = c->s_end;
+  // the pass's domain hit masks hold at most mask_cap rays: a larger pass is
+  // cut to that size and reported as a pool overflow (grow and render again)
+  const uint32_t nr = c->r_end - c->r_beg, ns = c->s_end - c->s_beg;
+  if ((unsigned long long)nr + ns > P.mask_cap) {
+    c->err |= 1u;
+    c->r_end = c->r_beg + min(nr, P.mask_cap);
+    c->s_end = c->s_beg + min(ns, P.mask_cap - (c->r_end - c->r_beg));
+    c->r_mark = c->r_end;
+    c->s_mark = c->s_end;
+  }
   c->active = (c->r_end > c->r_beg || c->s_end > c->s_beg) ? 1u : 0u;
   if (c->active) c->passes++;
   c->vlog_count = 0;
@@ -1549,7 +1559,7 @@ __global__ void kLogFinalize(TraceParams P) {
 // Host side
 struct TracerState {
   long long S = 0;  // samples of the largest tile
-  uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0;
+  uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0, mask_cap = 0;
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
       wctr, wq, score, order, q[2], ask, ask_cnt, rkey, dmask, dbox, ovf, vlog, log, sh_rec, qa,
@@ -1716,10 +1726,10 @@ static double stateBytesPerSample(const Ctx* c) {
   const double spec = 2.0, depths = std::min(cfg.bounces, kHistory), spv = spawnPerVertex(c);
   const double per_ray = std::min(nd, 12), words = (nd + 63) / 64;
   const double r = 1.0 + spec * depths, s = spv * spec * depths, p = spec;
-  const double q0 = spec * per_ray, q1 = spv * spec * per_ray * 0.5;
+  const double q0 = spec * per_ray, q1 = spv * spec * per_ray * 0.5, m = spec * (1.0 + spv);
   // radiance queue: entry + phase-list index; shadow queue: entry
   return r * 64 + s * 72 + p * 64 + q0 * (8 + 8) + std::min(q0, r) * kShadeRec * 16 + q1 * 8 +
-         (r + s) * 8 * words + 32 + r;
+         m * 8 * words + 32 + r;
 }
 
 static void ensureState(Ctx* c, double grow) {
@@ -1759,6 +1769,9 @@ static void ensureState(Ctx* c, double grow) {
   uint32_t per_ray = (uint32_t)std::min(nd, 12);  // domains a ray crosses, typical bound
   uint32_t q0 = cap((double)S * spec * per_ray);
   uint32_t q1 = cap((double)S * spawnPerVertex(c) * spec * per_ray * 0.5);
+  // rays of one pass: at most one radiance ray and its shadow rays per
+  // sample when each pass shades once, with the same speculative headroom
+  uint32_t mask_cap = std::min(r_cap + (uint64_t)s_cap, (uint64_t)cap((double)S * (1.0 + spawnPerVertex(c)) * spec));
   if (T->S != S || T->ndomains != nd) T->releaseGraph();
   T->S = S;
   T->ndomains = nd;
@@ -1767,6 +1780,7 @@ static void ensureState(Ctx* c, double grow) {
   T->p_cap = std::max(T->p_cap, p_cap);
   T->q_cap[0] = std::max(T->q_cap[0], q0);
   T->q_cap[1] = std::max(T->q_cap[1], q1);
+  T->mask_cap = std::max(T->mask_cap, mask_cap);
   for (int k = 0; k < 4; ++k) {
     need(T, T->R[k], (size_t)T->r_cap * 16);
     need(T, T->Pn[k], (size_t)T->p_cap * 16);
@@ -1780,7 +1794,7 @@ static void ensureState(Ctx* c, double grow) {
     need(T, T->ask_cnt, (size_t)kQueueGrid * 4 + 64);
     need(T, T->rkey, (size_t)T->r_cap * 4 + 64);
   }
-  need(T, T->dmask, ((size_t)T->r_cap + T->s_cap) * ((nd + 63) / 64) * 8 + 64);
+  need(T, T->dmask, (size_t)T->mask_cap * ((nd + 63) / 64) * 8 + 64);
   need(T, T->dbox, dboxBytes(nd));
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
@@ -1905,6 +1919,7 @@ static TraceParams makeParams(Ctx* c) {
   P.ask_region = T->ask.bytes >= (size_t)kQueueGrid * askRegion(T->r_cap) * 48 ? askRegion(T->r_cap) : 0u;
   P.rkey = T->rkey.as<uint32_t>();
   P.dmask = T->dmask.as<unsigned long long>();
+  P.mask_cap = T->mask_cap;
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
   P.refill = std::min(64, std::max(1, c->batch_chunk));

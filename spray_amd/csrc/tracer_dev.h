@@ -105,6 +105,7 @@ struct TraceParams {
   uint32_t* rkey;
   uint32_t rng[4];    // range launches: radiance base, count, shadow base, count
   unsigned long long* dmask;  // [pass ray][(nd + 63) / 64]: domains hit (kCount -> kScatter)
+  uint32_t mask_cap;          // pass rays the masks hold
   uint32_t q_cap[2];
   int refill;         // a wave refetches work once this many lanes are idle
   int trav_static;    // blocks of the trace kernel resident at once: their waves take a first range
