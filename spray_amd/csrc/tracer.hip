@@ -522,7 +522,10 @@ struct DomWalk {
 // (ooc_domain_stats.cc:60-88) weight max(0, 16 - position in the ray's list).
 // kLearned: the learned form's ask records are written (a separate
 // instantiation, so the other forms keep kCount's smaller register set).
-template <bool kLearned>
+// kList: more than 256 domains, rays with at most kTopDepths hit domains
+// write the packed list instead of the mask (an instantiation of its own, so
+// scenes with fewer domains keep the mask-only register set).
+template <bool kLearned, bool kList>
 __global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
@@ -627,7 +630,22 @@ __global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
     // a ray whose walk is done: its mask words, its score, (learned) its ask
     const bool fin = busy && W.left <= 0;
     if (fin) {
-      for (int w = 0; w < nw; ++w) P.dmask[(size_t)i * nw + w] = lmask[w * blockDim.x];
+      unsigned long long* row = P.dmask + (size_t)i * nw;
+      if (kList && W.hits <= kTopDepths) {
+        // the top list holds every hit domain: 32 bytes instead of the mask
+        uint32_t pk[kTopDepths / 2];
+#pragma unroll
+        for (int j = 0; j < kTopDepths / 2; ++j)
+          pk[j] = (2 * j < W.hits ? (uint32_t)W.ti[2 * j] : 0xffffu) |
+                  ((2 * j + 1 < W.hits ? (uint32_t)W.ti[2 * j + 1] : 0xffffu) << 16);
+        uint4* r4 = (uint4*)row;
+        r4[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+        r4[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+        P.dfmt[i] = 1;
+      } else {
+        for (int w = 0; w < nw; ++w) row[w] = lmask[w * blockDim.x];
+        if (kList) P.dfmt[i] = 0;
+      }
 #pragma unroll
       for (int k = 0; k < kTopDepths; ++k)
         if (k < W.kept) atomicAdd(&ls[W.ti[k]], (uint32_t)(kTopDepths - k));
@@ -743,6 +761,17 @@ __global__ void __launch_bounds__(1024) kSchedule(TraceParams P) {
 // The queue entries of one ray, at its block's base plus the block-local
 // slot of each domain; only the domains kCount marked are tested again
 // (for their entry distance).
+// One queue entry of ray r for domain dd, at the domain's entry distance.
+__device__ __forceinline__ void scatterOne(const float4* box, const float o[3], const float inv[3],
+                                           bool safe, int dd, uint32_t* lc_kind,
+                                           const uint32_t* lb_kind, const TraceParams& P, int kind,
+                                           uint32_t r) {
+  float tmin;
+  slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin);
+  const uint32_t dst = lb_kind[dd] + atomicAdd(&lc_kind[dd], 1u);
+  if (dst < P.q_cap[kind]) P.q[kind][dst] = QEntry{r, tmin};
+}
+
 __device__ __forceinline__ void scatterDomains(const float4* box, int nd, const float o[3],
                                                const float inv[3], bool safe,
                                                const unsigned long long* mask, uint32_t* lc_kind,
@@ -753,12 +782,7 @@ __device__ __forceinline__ void scatterDomains(const float4* box, int nd, const 
     while (m) {
       const int dd = (w << 6) + __builtin_ctzll(m);
       m &= m - 1ull;
-      float tmin;
-      slabBox4(box[2 * dd], box[2 * dd + 1], o, inv, safe, &tmin);
-      const uint32_t dst = lb_kind[dd] + atomicAdd(&lc_kind[dd], 1u);
-      if (dst < P.q_cap[kind]) {
-        P.q[kind][dst] = QEntry{r, tmin};
-      }
+      scatterOne(box, o, inv, safe, dd, lc_kind, lb_kind, P, kind, r);
     }
   }
 }
@@ -790,6 +814,8 @@ __global__ void __launch_bounds__(kBlock) kBlockBases(TraceParams P) {
 
 // Scatter the queue entries: the same grid and ray-to-block mapping as
 // kCount, so each block owns the slots kBlockBases reserved for it.
+// kList: the rays kCount gave a packed hit list (as kCount's kList).
+template <bool kList>
 __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
   extern __shared__ uint32_t sm[];
   if (!P.ctr->active) return;
@@ -825,9 +851,22 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
     }
     // the same wave-uniform slab variant as kCount (same rays per wave)
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
-    if (valid)
-      scatterDomains(box, nd, o, inv, safe, P.dmask + (size_t)i * ((nd + 63) >> 6),
-                     lc + kind * nd, lb + kind * nd, P, kind, r);
+    const int nw = (nd + 63) >> 6;
+    if (kList && valid && P.dfmt[i]) {
+      // packed ids of the ray's hit domains (kCount's top list)
+      const uint4* r4 = (const uint4*)(P.dmask + (size_t)i * nw);
+      const uint4 a = r4[0], b = r4[1];
+      const uint32_t pk[kTopDepths / 2] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int j = 0; j < kTopDepths; ++j) {
+        const int dd = (int)((pk[j >> 1] >> (16 * (j & 1))) & 0xffffu);
+        if (dd == 0xffff) break;
+        scatterOne(box, o, inv, safe, dd, lc + kind * nd, lb + kind * nd, P, kind, r);
+      }
+    } else if (valid) {
+      scatterDomains(box, nd, o, inv, safe, P.dmask + (size_t)i * nw, lc + kind * nd,
+                     lb + kind * nd, P, kind, r);
+    }
   }
 }
 
@@ -1562,7 +1601,7 @@ struct TracerState {
   uint32_t r_cap = 0, s_cap = 0, p_cap = 0, q_cap[2] = {0, 0}, sh_cap = 0, mask_cap = 0;
   int ndomains = 0;
   DevBuf R[4], Pn[4], Sh[4], s_tmax, s_occ, vbuf, image, ctr, tile_rec, cnt, off, dbase, bcnt, bbase,
-      wctr, wq, score, order, q[2], ask, ask_cnt, rkey, dmask, dbox, ovf, vlog, log, sh_rec, qa,
+      wctr, wq, score, order, q[2], ask, ask_cnt, rkey, dmask, dfmt, dbox, ovf, vlog, log, sh_rec, qa,
       qb, dflag, acnt;
   unsigned long long log_cap = 0;
   bool logging = false;
@@ -1596,7 +1635,7 @@ struct TracerState {
       for (int k = 0; k < 4; ++k) b += x[k].bytes;
     for (const DevBuf* x : {&s_tmax, &s_occ, &vbuf, &image, &ctr, &tile_rec, &cnt, &off, &dbase, &bcnt,
                             &bbase, &wctr, &wq, &score, &order, &q[0], &q[1], &ask, &ask_cnt, &rkey,
-                            &dmask, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb, &dflag, &acnt})
+                            &dmask, &dfmt, &dbox, &ovf, &vlog, &log, &sh_rec, &qa, &qb, &dflag, &acnt})
       b += x->bytes;
     return b;
   }
@@ -1669,7 +1708,7 @@ void tracerRelease(Ctx* c) {
     for (int k = 0; k < 4; ++k) arr[k].release();
   for (DevBuf* b : {&T->s_tmax, &T->s_occ, &T->vbuf, &T->image, &T->ctr, &T->tile_rec, &T->cnt,
                     &T->off, &T->dbase, &T->bcnt, &T->bbase, &T->wctr, &T->wq, &T->score, &T->order, &T->q[0],
-                    &T->q[1], &T->ask, &T->ask_cnt, &T->rkey, &T->dmask, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb, &T->dflag, &T->acnt})
+                    &T->q[1], &T->ask, &T->ask_cnt, &T->rkey, &T->dmask, &T->dfmt, &T->dbox, &T->ovf, &T->vlog, &T->log, &T->sh_rec, &T->log_ctr, &T->qa, &T->qb, &T->dflag, &T->acnt})
     b->release();
   delete T;
   c->tracer = nullptr;
@@ -1729,7 +1768,7 @@ static double stateBytesPerSample(const Ctx* c) {
   const double q0 = spec * per_ray, q1 = spv * spec * per_ray * 0.5, m = spec * (1.0 + spv);
   // radiance queue: entry + phase-list index; shadow queue: entry
   return r * 64 + s * 72 + p * 64 + q0 * (8 + 8) + std::min(q0, r) * kShadeRec * 16 + q1 * 8 +
-         m * 8 * words + 32 + r;
+         m * (8 * words + 1) + 32 + r;
 }
 
 static void ensureState(Ctx* c, double grow) {
@@ -1758,7 +1797,8 @@ static void ensureState(Ctx* c, double grow) {
   int nd = c->pool.ndomains;
   if (nd > 1024) throw StatusError(SPRAY_ERR_CAPACITY, "the tracer supports at most 1024 domains");
   if (countShmem(nd) > 65536)  // many domains: kCount's LDS beyond the default limit
-    for (const void* k : {(const void*)kCount<false>, (const void*)kCount<true>})
+    for (const void* k : {(const void*)kCount<false, false>, (const void*)kCount<true, false>,
+                          (const void*)kCount<false, true>, (const void*)kCount<true, true>})
       SPRAY_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)countShmem(nd)));
   int depths = std::min(cfg.bounces, kHistory);
   double spec = 2.0 * grow;  // headroom for speculative re-shading
@@ -1795,6 +1835,7 @@ static void ensureState(Ctx* c, double grow) {
     need(T, T->rkey, (size_t)T->r_cap * 4 + 64);
   }
   need(T, T->dmask, (size_t)T->mask_cap * ((nd + 63) / 64) * 8 + 64);
+  need(T, T->dfmt, (size_t)T->mask_cap + 64);
   need(T, T->dbox, dboxBytes(nd));
   need(T, T->ctr, sizeof(DevCounters));
   need(T, T->tile_rec, 64);
@@ -1919,6 +1960,7 @@ static TraceParams makeParams(Ctx* c) {
   P.ask_region = T->ask.bytes >= (size_t)kQueueGrid * askRegion(T->r_cap) * 48 ? askRegion(T->r_cap) : 0u;
   P.rkey = T->rkey.as<uint32_t>();
   P.dmask = T->dmask.as<unsigned long long>();
+  P.dfmt = T->dfmt.as<uint8_t>();
   P.mask_cap = T->mask_cap;
   P.q_cap[0] = T->q_cap[0];
   P.q_cap[1] = T->q_cap[1];
@@ -2062,10 +2104,16 @@ static void passQueues(Ctx* c, const TraceParams& P) {
   int nd = P.ndomains;
   kPassBegin<<<1, 1, 0, s>>>(P);
   kClearQ<<<(nd + 255) / 256, 256, 0, s>>>(P);
-  if (P.learned)
-    kCount<true><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+  const bool list = (nd + 63) / 64 > kHitListWords;  // packed hit lists (kScatter agrees)
+  const size_t sh = countShmem(nd);
+  if (P.learned && list)
+    kCount<true, true><<<kQueueGrid, kBlock, sh, s>>>(P);
+  else if (P.learned)
+    kCount<true, false><<<kQueueGrid, kBlock, sh, s>>>(P);
+  else if (list)
+    kCount<false, true><<<kQueueGrid, kBlock, sh, s>>>(P);
   else
-    kCount<false><<<kQueueGrid, kBlock, countShmem(nd), s>>>(P);
+    kCount<false, false><<<kQueueGrid, kBlock, sh, s>>>(P);
   // learned form: the predictor needs only kCount's ask records, so it runs
   // on the second stream beside the schedule and the scatter
   if (P.learned) {
@@ -2076,7 +2124,10 @@ static void passQueues(Ctx* c, const TraceParams& P) {
   }
   kSchedule<<<1, 1024, 0, s>>>(P);
   kBlockBases<<<2 * nd, kBlock, 0, s>>>(P);
-  kScatter<<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);
+  if (list)
+    kScatter<true><<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);
+  else
+    kScatter<false><<<kQueueGrid, kBlock, scatterShmem(nd), s>>>(P);
   if (P.learned) {
     SPRAY_HIP(hipStreamWaitEvent(s, T->join_ev, 0));
     predictorAssign(c, P);

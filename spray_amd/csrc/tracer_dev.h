@@ -10,6 +10,9 @@ namespace spray {
 
 constexpr int kMaxLights = 8;
 constexpr int kTopDepths = kListSize;  // list positions that carry score weight
+// a ray's hit domains as kTopDepths packed uint16 ids: the words of a mask of 256 domains
+constexpr int kHitListWords = kTopDepths * 2 / 8;
+static_assert(kTopDepths % 8 == 0 && kHitListWords == 4, "32-byte hit lists");
 
 // Ray pools are SoA streams of float4 (16-byte loads per lane):
 //   op = (org.xyz, pixid)  ds = (dir.xyz, samid)
@@ -106,6 +109,10 @@ struct TraceParams {
   uint32_t rng[4];    // range launches: radiance base, count, shadow base, count
   unsigned long long* dmask;  // [pass ray][(nd + 63) / 64]: domains hit (kCount -> kScatter)
   uint32_t mask_cap;          // pass rays the masks hold
+  // more than 256 domains: a ray with at most kTopDepths hit domains keeps
+  // their ids (uint16, 0xffff after the last) in the first 32 bytes of its
+  // mask row instead of the mask, and dfmt[pass ray] = 1 says so
+  uint8_t* dfmt;
   uint32_t q_cap[2];
   int refill;         // a wave refetches work once this many lanes are idle
   int trav_static;    // blocks of the trace kernel resident at once: their waves take a first range

@@ -340,8 +340,10 @@ def test_far_origins_at_vertices_and_edges(ctx_c1, ref_c1):
 @pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "batched"])
 def test_domain_grid_frame_matches_oracle(tmp_path, schedule):
     """512 domains (8x8x8), the largest scene among the parity tests: the
-    domain-list pass walks the domain grid; the frame matches the oracle's
-    brute-force domain lists."""
+    domain-list pass walks the domain grid and hands the scatter a packed list
+    of hit-domain ids for rays that hit at most 16 domains and a full mask for
+    the others (the form for more than 256 domains); the frame matches the
+    oracle's brute-force domain lists."""
     import spray_amd as sa
     d = str(tmp_path)
     sa.scenegen_write(d, "g", 8, 8, 8, 8, rgb=0xB3B3B3, extra_lines="light rect 1 1 8.5 0 6 0 6 0 0 4 4 4")
