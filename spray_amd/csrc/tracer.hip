@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(kBlock, 4) kCount(TraceParams P) {
     // a ray whose walk is done: its mask words, its score, (learned) its ask
     const bool fin = busy && W.left <= 0;
     if (fin) {
-      unsigned long long* row = P.dmask + (size_t)i * nw;
+      unsigned long long* row = P.dmask + (size_t)i * maskRowWords(nd);
       if (kList && W.hits <= kTopDepths) {
         // the top list holds every hit domain: 32 bytes instead of the mask
         uint32_t pk[kTopDepths / 2];
@@ -851,10 +851,10 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
     }
     // the same wave-uniform slab variant as kCount (same rays per wave)
     const bool safe = __ballot(valid && hasInfInv(inv)) != 0ull;
-    const int nw = (nd + 63) >> 6;
+    const unsigned long long* row = P.dmask + (size_t)i * maskRowWords(nd);
     if (kList && valid && P.dfmt[i]) {
       // packed ids of the ray's hit domains (kCount's top list)
-      const uint4* r4 = (const uint4*)(P.dmask + (size_t)i * nw);
+      const uint4* r4 = (const uint4*)row;
       const uint4 a = r4[0], b = r4[1];
       const uint32_t pk[kTopDepths / 2] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
@@ -864,8 +864,7 @@ __global__ void __launch_bounds__(kBlock) kScatter(TraceParams P) {
         scatterOne(box, o, inv, safe, dd, lc + kind * nd, lb + kind * nd, P, kind, r);
       }
     } else if (valid) {
-      scatterDomains(box, nd, o, inv, safe, P.dmask + (size_t)i * nw, lc + kind * nd,
-                     lb + kind * nd, P, kind, r);
+      scatterDomains(box, nd, o, inv, safe, row, lc + kind * nd, lb + kind * nd, P, kind, r);
     }
   }
 }
@@ -1763,7 +1762,7 @@ static double stateBytesPerSample(const Ctx* c) {
   const spray_config& cfg = c->cfg;
   const int nd = c->pool.ndomains;
   const double spec = 2.0, depths = std::min(cfg.bounces, kHistory), spv = spawnPerVertex(c);
-  const double per_ray = std::min(nd, 12), words = (nd + 63) / 64;
+  const double per_ray = std::min(nd, 12), words = maskRowWords(nd);
   const double r = 1.0 + spec * depths, s = spv * spec * depths, p = spec;
   const double q0 = spec * per_ray, q1 = spv * spec * per_ray * 0.5, m = spec * (1.0 + spv);
   // radiance queue: entry + phase-list index; shadow queue: entry
@@ -1834,7 +1833,7 @@ static void ensureState(Ctx* c, double grow) {
     need(T, T->ask_cnt, (size_t)kQueueGrid * 4 + 64);
     need(T, T->rkey, (size_t)T->r_cap * 4 + 64);
   }
-  need(T, T->dmask, (size_t)T->mask_cap * ((nd + 63) / 64) * 8 + 64);
+  need(T, T->dmask, (size_t)T->mask_cap * maskRowWords(nd) * 8 + 64);
   need(T, T->dfmt, (size_t)T->mask_cap + 64);
   need(T, T->dbox, dboxBytes(nd));
   need(T, T->ctr, sizeof(DevCounters));

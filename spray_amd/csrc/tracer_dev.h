@@ -13,6 +13,12 @@ constexpr int kTopDepths = kListSize;  // list positions that carry score weight
 // a ray's hit domains as kTopDepths packed uint16 ids: the words of a mask of 256 domains
 constexpr int kHitListWords = kTopDepths * 2 / 8;
 static_assert(kTopDepths % 8 == 0 && kHitListWords == 4, "32-byte hit lists");
+// words per hit-mask row: even above 256 domains, so that a row's packed list
+// (two 16-byte stores) stays 16-byte aligned
+__host__ __device__ inline int maskRowWords(int nd) {
+  const int nw = (nd + 63) >> 6;
+  return nw > kHitListWords ? (nw + 1) & ~1 : nw;
+}
 
 // Ray pools are SoA streams of float4 (16-byte loads per lane):
 //   op = (org.xyz, pixid)  ds = (dir.xyz, samid)
@@ -107,7 +113,7 @@ struct TraceParams {
   // kNoBound
   uint32_t* rkey;
   uint32_t rng[4];    // range launches: radiance base, count, shadow base, count
-  unsigned long long* dmask;  // [pass ray][(nd + 63) / 64]: domains hit (kCount -> kScatter)
+  unsigned long long* dmask;  // [pass ray][maskRowWords(nd)]: domains hit (kCount -> kScatter)
   uint32_t mask_cap;          // pass rays the masks hold
   // more than 256 domains: a ray with at most kTopDepths hit domains keeps
   // their ids (uint16, 0xffff after the last) in the first 32 bytes of its

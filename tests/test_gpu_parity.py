@@ -337,17 +337,21 @@ def test_far_origins_at_vertices_and_edges(ctx_c1, ref_c1):
         assert (occ == (rp >= 0)).mean() >= 0.9999
 
 
-@pytest.mark.parametrize("schedule", [0, 1], ids=["plain", "batched"])
-def test_domain_grid_frame_matches_oracle(tmp_path, schedule):
-    """512 domains (8x8x8), the largest scene among the parity tests: the
-    domain-list pass walks the domain grid and hands the scatter a packed list
-    of hit-domain ids for rays that hit at most 16 domains and a full mask for
-    the others (the form for more than 256 domains); the frame matches the
-    oracle's brute-force domain lists."""
+@pytest.mark.parametrize("nx,schedule", [(8, 0), (8, 1), (5, 1)],
+                         ids=["512-plain", "512-batched", "320-batched"])
+def test_domain_grid_frame_matches_oracle(tmp_path, nx, schedule):
+    """512 domains (8x8x8), the largest scene among the parity tests, and 320
+    (5x8x8, mask rows of 5 words padded to 6): the domain-list pass walks the
+    domain grid and hands the scatter a packed list of hit-domain ids for rays
+    that hit at most 16 domains and a full mask for the others (the form for
+    more than 256 domains); the frame matches the oracle's brute-force domain
+    lists."""
     import spray_amd as sa
     d = str(tmp_path)
-    sa.scenegen_write(d, "g", 8, 8, 8, 8, rgb=0xB3B3B3, extra_lines="light rect 1 1 8.5 0 6 0 6 0 0 4 4 4")
-    cam = dict(camera_pos=[4.0, 4.0, 18.0], camera_lookat=[4.0, 4.0, 4.0], camera_up=[0.0, 1.0, 0.0])
+    sa.scenegen_write(d, "g", nx, 8, 8, 8, rgb=0xB3B3B3,
+                      extra_lines=f"light rect 1 1 8.5 0 6 0 {nx - 2} 0 0 4 4 4")
+    cx = nx / 2.0
+    cam = dict(camera_pos=[cx, 4.0, 18.0], camera_lookat=[cx, 4.0, 4.0], camera_up=[0.0, 1.0, 0.0])
     pt = dict(image_w=96, image_h=96, pixel_samples=2, ao_mode=0, ao_samples=1, bounces=5, rr_start=3,
               max_samples_per_rank=4608)
     ref = orc.Scene(os.path.join(d, "g.spray"), d, nthreads=8)
@@ -355,7 +359,7 @@ def test_domain_grid_frame_matches_oracle(tmp_path, schedule):
                            cam_up=cam["camera_up"], **pt)
     with sa.Context(sa.default_config(has_camera=1, nthreads=8, schedule=schedule, **cam, **pt)) as ctx:
         ctx.load_scene(os.path.join(d, "g.spray"), d)
-        assert ctx.num_domains == 512
+        assert ctx.num_domains == nx * 64
         img, st = ctx.trace_frame()
     err = np.abs(img[..., :3] - rimg[..., :3])
     assert st["useful_rays"] == rst["useful_rays"]
